@@ -1,0 +1,254 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 multi-prime NTT engine.
+
+Default workload: one exact product of two 2^20-limb operands (SplitMix64
+limbs, seeds 1 and 2, as the reference benchmarks use), operands and result
+resident in HBM. A step is one MulWorkspace product on the device; the metric
+is operand limbs per second (Mlimbs/s). Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--workload mul_2p20|sqr_2p24|conv16_2p22|conv32s_2p22]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import nttanafis_amd as ntt  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def splitmix(length: int, seed: int) -> np.ndarray:
+    gamma = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (np.arange(1, length + 1, dtype=np.uint64) * gamma)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def workload(name: str):
+    """Returns (operands, step builder description)."""
+    if name == "mul_2p20":
+        return dict(kind="mul", a=splitmix(1 << 20, 1), b=splitmix(1 << 20, 2))
+    if name == "sqr_2p24":
+        return dict(kind="sqr", a=splitmix(1 << 24, 1))
+    if name == "conv16_2p22":
+        n = 1 << 22
+        return dict(kind="conv", a=splitmix(n, 1) & np.uint64(0xFFFF), b=splitmix(n, 2) & np.uint64(0xFFFF),
+                    max=65535, signed=False)
+    if name == "conv32s_2p22":
+        n = 1 << 22
+
+        def s(seed):
+            return (splitmix(n, seed) & np.uint64(0xFFFFFFFF)).astype(np.uint32).view(np.int32).astype(
+                np.int64).astype(np.uint64)
+
+        return dict(kind="conv", a=s(1), b=s(2), max=1 << 31, signed=True)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def kernel_bytes(name: str, primes: int, T: int, la: int, lb: int, out_len: int, square: bool) -> float:
+    """Algorithmic HBM bytes of one launch of each kernel kind: every residue
+    of a pass read and written once; packing reads the limbs once."""
+    if name == "fwd_pack_strided":
+        return 8.0 * la + 8.0 * primes * T  # per operand launch (la ~ lb here)
+    if name in ("fwd_rows", "fwd_strided", "inv_strided"):
+        return 16.0 * primes * T
+    if name == "fused_rows":
+        return (16.0 if square else 24.0) * primes * T
+    if name == "inv_strided_canon":
+        return 16.0 * primes * T
+    if name == "crt_carry":
+        return 8.0 * primes * min(T, out_len * 64) + 8.0 * out_len
+    if name == "crt_disjoint":
+        return 8.0 * primes * (la + lb - 1) + 8.0 * out_len
+    if name == "fused_single":
+        return 8.0 * (la + lb) + 8.0 * primes * T
+    return 0.0
+
+
+def cpu_baseline(wl, sample_seconds: float):
+    """Times the oracle (the C restatement of the reference CPU path) on a
+    bounded sample of the same workload, single-threaded."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    a = wl["a"]
+    t0 = time.perf_counter()
+    runs = 0
+    if wl["kind"] == "mul":
+        b = wl["b"]
+        while True:
+            oracle.mul(a, b, threads=1)
+            runs += 1
+            if time.perf_counter() - t0 > sample_seconds:
+                break
+        per = (time.perf_counter() - t0) / runs
+        return {"value": len(a) / per / 1e6, "unit": "Mlimbs/s", "cores": 1, "kind": "port",
+                "sample": f"{runs} products of 2^20 x 2^20 limbs, oracle/ntt_oracle.c, 1 thread"}
+    if wl["kind"] == "sqr":
+        n = 1 << 20
+        sub = a[:n]
+        while True:
+            oracle.sqr(sub, threads=1)
+            runs += 1
+            if time.perf_counter() - t0 > sample_seconds:
+                break
+        per = (time.perf_counter() - t0) / runs
+        return {"value": n / per / 1e6, "unit": "Mlimbs/s", "cores": 1, "kind": "port",
+                "sample": f"{runs} squares of 2^20 limbs (sub-sample of the 2^24 square), 1 thread"}
+    b = wl["b"]
+    while True:
+        oracle.convolve(a, b, wl["max"], wl["max"], wl["signed"], threads=1)
+        runs += 1
+        if time.perf_counter() - t0 > sample_seconds:
+            break
+    per = (time.perf_counter() - t0) / runs
+    return {"value": len(a) / per / 1e6, "unit": "Mcoeff/s", "cores": 1, "kind": "port",
+            "sample": f"{runs} convolutions of {len(a)} coefficients, 1 thread"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="mul_2p20")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        return 2
+    ntt.load()
+    if not ntt.device_available():
+        print("bench.py: no gfx950 device", file=sys.stderr)
+        return 1
+
+    wl = workload(args.workload)
+    a = np.ascontiguousarray(wl["a"])
+    kind = wl["kind"]
+    b = np.ascontiguousarray(wl["b"]) if kind != "sqr" else a
+    la, lb = len(a), len(b)
+    da = ntt.DeviceBuffer.from_host(a)
+    db = ntt.DeviceBuffer.from_host(b) if kind != "sqr" else da
+    if kind == "conv":
+        plan = ntt.ConvPlan.new(la, lb, wl["max"], wl["max"], wl["signed"])
+        out_len = plan.output_len()
+        ws = ntt.ConvWorkspace()
+        ws.prepare(plan)
+        primes, log_len = plan.primes, plan.log_len
+        chunk = 64
+
+        def step():
+            return ws.convolve_device(dout.ptr, out_len, da.ptr, la, db.ptr, lb)
+    else:
+        plan = ntt.MulPlan.new(la, lb)
+        out_len = la + lb
+        ws = ntt.MulWorkspace()
+        ws.prepare(plan)
+        primes, log_len, chunk = plan.primes, plan.log_len, plan.chunk_bits
+
+        if kind == "sqr":
+            def step():
+                return ws.sqr_device(dout.ptr, out_len, da.ptr, la)
+        else:
+            def step():
+                return ws.mul_device(dout.ptr, out_len, da.ptr, la, db.ptr, lb)
+    dout = ntt.DeviceBuffer(out_len * 8)
+    T = 1 << log_len
+
+    for _ in range(args.warmup):
+        assert step(), ntt.last_error()
+    assert ntt.synchronize()
+
+    # Timed region: exactly K steps, synchronized on both sides. Every kernel
+    # launch inside is bracketed by HIP events on its own stream.
+    ntt.Profile.reset()
+    ntt.Profile.enable(True)
+    assert ntt.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        if not step():
+            raise SystemExit("step failed: " + ntt.last_error())
+    assert ntt.synchronize()
+    elapsed = time.perf_counter() - t0
+    ntt.Profile.enable(False)
+    prof = ntt.Profile.read()
+
+    # The same K steps without events, for the event overhead.
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    assert ntt.synchronize()
+    clean = time.perf_counter() - t1
+
+    # Result check on the last step: the product modulo a 61-bit prime.
+    got = dout.download(out_len)
+    if kind != "conv":
+        q = (1 << 61) - 1
+        ia = int.from_bytes(a.astype("<u8").tobytes(), "little") % q
+        ib = ia if kind == "sqr" else int.from_bytes(b.astype("<u8").tobytes(), "little") % q
+        ig = int.from_bytes(got.astype("<u8").tobytes(), "little") % q
+        assert ig == ia * ib % q, "device product failed its modular check"
+
+    ms_step = 1000.0 * elapsed / args.steps
+    ms_clean = 1000.0 * clean / args.steps
+    per_kernel = {}
+    dominant, dom_total = None, -1.0
+    for name, (ms, count) in prof.items():
+        avg_ms = ms / max(count, 1)
+        nbytes = kernel_bytes(name, primes, T, la, lb, out_len, kind == "sqr")
+        per_kernel[name] = {"avg_us": round(avg_ms * 1000, 2), "launches": int(count),
+                            "GBps": round(nbytes / (avg_ms * 1e-3) / 1e9, 1) if nbytes else None}
+        if ms > dom_total and nbytes:
+            dominant, dom_total = name, ms
+    roofline = None
+    if dominant:
+        d = per_kernel[dominant]
+        nbytes = kernel_bytes(dominant, primes, T, la, lb, out_len, kind == "sqr")
+        achieved = nbytes / (d["avg_us"] * 1e-6) / 1e9
+        roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "algorithmic_bytes": nbytes}
+
+    unit = "Mcoeff/s" if kind == "conv" else "Mlimbs/s"
+    value = la / (ms_clean * 1e-3) / 1e6
+    result = {
+        "metric": "Mlimbs/s at 2^20-limb mul" if args.workload == "mul_2p20" else f"{unit} ({args.workload})",
+        "value": round(value, 2),
+        "unit": unit,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_clean, 4),
+        "ms_per_step_with_events": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (SplitMix64 limbs, seeds 1 and 2), operands resident in HBM",
+        "config": {"workload": args.workload, "len_a": la, "len_b": lb, "primes": primes,
+                   "chunk_bits": chunk, "transform_len": T},
+        "kernels": per_kernel,
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

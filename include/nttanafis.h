@@ -1,0 +1,200 @@
+/*
+ * nttanafis.h -- C ABI of the MI355X (gfx950) multi-prime NTT engine.
+ *
+ * Drop-in boundary for the reference crate's public multiplication API. Each
+ * entry point cites the reference item it replaces (paths relative to the
+ * reference crate root). Limbs are little-endian uint64_t words, least
+ * significant first (src/lib.rs:3-6).
+ *
+ * Shared semantics (src/lib.rs:21-26):
+ *   - plan constructors return false where the reference returns None;
+ *   - calls return false on rejected shapes or short buffers, and then leave
+ *     dst unchanged;
+ *   - success writes exactly the product span (a_len+b_len limbs, 2*a_len for a
+ *     square, output_len for a convolution) and preserves the rest of dst;
+ *   - the same pointer and length for both operands runs the square path
+ *     (core::ptr::eq on slices, src/plan.rs:96, src/convolve.rs:156);
+ *   - dst must not overlap an operand (guaranteed by borrows in the
+ *     reference, required here);
+ *   - device failures (allocation, launch) return false with dst unchanged:
+ *     results are staged in device memory and copied out last.
+ *
+ * Deviation: the GPU engine keeps no work in caller scratch. Device arenas
+ * belong to a process-wide context (plan path) or to the workspace, so the
+ * scratch lengths below are 0 and any scratch buffer, including NULL, passes.
+ */
+#ifndef NTTANAFIS_H
+#define NTTANAFIS_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Products                                                                  */
+
+/* MulPlan (src/plan.rs:33-36): plain data, copyable, no heap. */
+typedef struct ntt_mul_plan {
+    uint64_t len_a;
+    uint64_t len_b;
+    uint32_t primes;     /* catalog prefix of the device geometry */
+    uint32_t chunk_bits; /* bits per coefficient */
+    uint32_t log_len;    /* log2 of the transform length */
+    uint32_t valid;      /* 1 for a plan from ntt_mul_plan_new */
+} ntt_mul_plan;
+
+/* MulPlan::new, src/plan.rs:53-69. false <=> None: a zero length, an
+ * overflowing output length, or a transform order beyond the catalog. */
+bool ntt_mul_plan_new(size_t len_a, size_t len_b, ntt_mul_plan *out);
+
+/* MulPlan::scratch_len, src/plan.rs:73-76 (0, see the deviation above). */
+size_t ntt_mul_plan_scratch_len(const ntt_mul_plan *plan);
+
+/* MulPlan::sqr_scratch_len, src/plan.rs:81-83 (0). */
+size_t ntt_mul_plan_sqr_scratch_len(const ntt_mul_plan *plan);
+
+/* MulPlan::mul, src/plan.rs:94-102. Thread-safe. */
+bool ntt_mul_plan_mul(const ntt_mul_plan *plan, uint64_t *dst, size_t dst_len, const uint64_t *a,
+                      size_t a_len, const uint64_t *b, size_t b_len, uint64_t *scratch,
+                      size_t scratch_len);
+
+/* MulPlan::sqr, src/plan.rs:112-114. Thread-safe. */
+bool ntt_mul_plan_sqr(const ntt_mul_plan *plan, uint64_t *dst, size_t dst_len, const uint64_t *a,
+                      size_t a_len, uint64_t *scratch, size_t scratch_len);
+
+/* MulWorkspace (src/plan.rs:134-138): retained device tables and arenas.
+ * One handle must not be used by two threads at once. */
+typedef struct ntt_mul_workspace ntt_mul_workspace;
+
+/* MulWorkspace::new, src/plan.rs:143-148. NULL only when out of host memory. */
+ntt_mul_workspace *ntt_mul_workspace_new(void);
+
+/* Drop of MulWorkspace. NULL is ignored. */
+void ntt_mul_workspace_free(ntt_mul_workspace *ws);
+
+/* MulWorkspace::prepare, src/plan.rs:151-156: retains the plan and grows the
+ * device tables and arenas for it. */
+void ntt_mul_workspace_prepare(ntt_mul_workspace *ws, const ntt_mul_plan *plan);
+
+/* MulWorkspace::mul, src/plan.rs:164-170. An empty operand writes zero. */
+bool ntt_mul_workspace_mul(ntt_mul_workspace *ws, uint64_t *dst, size_t dst_len, const uint64_t *a,
+                           size_t a_len, const uint64_t *b, size_t b_len);
+
+/* MulWorkspace::sqr, src/plan.rs:176-179. */
+bool ntt_mul_workspace_sqr(ntt_mul_workspace *ws, uint64_t *dst, size_t dst_len, const uint64_t *a,
+                           size_t a_len);
+
+/* ------------------------------------------------------------------------ */
+/* Convolutions                                                              */
+
+/* ConvPlan (src/convolve.rs:51-57). */
+typedef struct ntt_conv_plan {
+    uint64_t len_a;
+    uint64_t len_b;
+    uint64_t max_a;
+    uint64_t max_b;
+    uint64_t stride;     /* limbs per result coefficient */
+    uint64_t output_len; /* (len_a + len_b - 1) * stride */
+    uint32_t is_signed;
+    uint32_t schoolbook; /* shorter operand of at most 24 terms: direct sums */
+    uint32_t primes;
+    uint32_t log_len;
+    uint32_t bound_bits; /* bits of t*max_a*max_b, plus a sign bit if signed */
+    uint32_t value_bits_a;
+    uint32_t value_bits_b;
+    uint32_t valid;
+} ntt_conv_plan;
+
+/* ConvPlan::new, src/convolve.rs:75-113. false <=> None: an empty operand,
+ * an unaddressable shape, a bound beyond the catalog, or a signed bound of
+ * 2^63 or more. */
+bool ntt_conv_plan_new(size_t len_a, size_t len_b, uint64_t max_a, uint64_t max_b, bool is_signed,
+                       ntt_conv_plan *out);
+
+/* ConvPlan::stride, src/convolve.rs:118-120. */
+size_t ntt_conv_plan_stride(const ntt_conv_plan *plan);
+
+/* ConvPlan::output_len, src/convolve.rs:124-126. */
+size_t ntt_conv_plan_output_len(const ntt_conv_plan *plan);
+
+/* ConvPlan::scratch_len, src/convolve.rs:132-138 (0, see above). */
+size_t ntt_conv_plan_scratch_len(const ntt_conv_plan *plan);
+
+/* ConvPlan::convolve, src/convolve.rs:148-162. Returns false for a length
+ * mismatch, a coefficient above its bound, or a short dst. Thread-safe. */
+bool ntt_conv_plan_convolve(const ntt_conv_plan *plan, uint64_t *dst, size_t dst_len,
+                            const uint64_t *a, size_t a_len, const uint64_t *b, size_t b_len,
+                            uint64_t *scratch, size_t scratch_len);
+
+/* ConvWorkspace (src/convolve.rs:228-232). */
+typedef struct ntt_conv_workspace ntt_conv_workspace;
+
+/* ConvWorkspace::new, src/convolve.rs:237-243. */
+ntt_conv_workspace *ntt_conv_workspace_new(void);
+
+/* Drop of ConvWorkspace. */
+void ntt_conv_workspace_free(ntt_conv_workspace *ws);
+
+/* ConvWorkspace::prepare, src/convolve.rs:247-255. */
+void ntt_conv_workspace_prepare(ntt_conv_workspace *ws, const ntt_conv_plan *plan);
+
+/* ConvWorkspace::convolve, src/convolve.rs:264-280. false when unprepared. */
+bool ntt_conv_workspace_convolve(ntt_conv_workspace *ws, uint64_t *dst, size_t dst_len,
+                                 const uint64_t *a, size_t a_len, const uint64_t *b, size_t b_len);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident operands (no counterpart in the reference, whose operands
+ * are host slices). Pointers are device addresses; `stream` is a hipStream_t
+ * or NULL for the workspace's own stream. The call returns once the work is
+ * queued on that stream; synchronize the stream before reading dst.
+ * Coefficient bounds of a convolution are not checked on this path. */
+
+bool ntt_mul_workspace_mul_device(ntt_mul_workspace *ws, uint64_t *d_dst, size_t dst_len,
+                                  const uint64_t *d_a, size_t a_len, const uint64_t *d_b,
+                                  size_t b_len, void *stream);
+
+bool ntt_mul_workspace_sqr_device(ntt_mul_workspace *ws, uint64_t *d_dst, size_t dst_len,
+                                  const uint64_t *d_a, size_t a_len, void *stream);
+
+bool ntt_conv_workspace_convolve_device(ntt_conv_workspace *ws, uint64_t *d_dst, size_t dst_len,
+                                        const uint64_t *d_a, size_t a_len, const uint64_t *d_b,
+                                        size_t b_len, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Engine utilities (no counterpart in the reference).                       */
+
+/* 1 when a gfx950 device is usable, 0 otherwise. Never initializes more than
+ * the HIP runtime. */
+int ntt_device_available(void);
+
+/* Device memory for callers without their own allocator. */
+void *ntt_device_alloc(size_t bytes);
+void ntt_device_free(void *ptr);
+bool ntt_copy_to_device(void *d_dst, const void *src, size_t bytes);
+bool ntt_copy_to_host(void *dst, const void *d_src, size_t bytes);
+bool ntt_device_synchronize(void);
+
+/* Last error message of the calling thread ("" if none). */
+const char *ntt_last_error(void);
+
+/* Per-kernel timing. While enabled, every kernel launch of the engine is
+ * bracketed by HIP events on the stream it runs on; ntt_profile_read returns
+ * per-kernel-kind totals. `names` receives up to `cap` NUL-separated kind
+ * names; `ms` and `counts` receive totals. Returns the number of kinds. */
+void ntt_profile_enable(int on);
+int ntt_profile_read(char *names, size_t names_len, double *ms, uint64_t *counts, int cap);
+void ntt_profile_reset(void);
+
+/* Geometry the engine chose for a product shape (for reports and tests). */
+bool ntt_mul_geometry(size_t len_a, size_t len_b, uint32_t *primes, uint32_t *chunk_bits,
+                      uint32_t *log_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,34 @@
+"""MI355X (gfx950) multi-prime NTT multiplication engine.
+
+Drop-in for the reference crate's MulPlan / MulWorkspace / ConvPlan /
+ConvWorkspace: exact big-integer products, squares and dense convolutions,
+computed by hand-written HIP kernels behind the C ABI of include/nttanafis.h.
+"""
+from .api import (  # noqa: F401
+    ConvPlan,
+    ConvWorkspace,
+    DeviceBuffer,
+    MulPlan,
+    MulWorkspace,
+    Profile,
+    device_available,
+    last_error,
+    mul_geometry,
+    synchronize,
+)
+from ._ffi import LIB_PATH, load  # noqa: F401
+
+__all__ = [
+    "MulPlan",
+    "MulWorkspace",
+    "ConvPlan",
+    "ConvWorkspace",
+    "DeviceBuffer",
+    "Profile",
+    "device_available",
+    "last_error",
+    "mul_geometry",
+    "synchronize",
+    "load",
+    "LIB_PATH",
+]

@@ -1,0 +1,129 @@
+"""ctypes declarations of the C ABI in include/nttanafis.h.
+
+The shared library is built in-tree (nttanafis_amd/libnttanafis_amd.so) by
+nttanafis_amd.build. Loading fails loudly when it is missing: there is no
+CPU fallback behind this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnttanafis_amd.so")
+
+u64 = ctypes.c_uint64
+u32 = ctypes.c_uint32
+size_t = ctypes.c_size_t
+P64 = ctypes.POINTER(ctypes.c_uint64)
+
+
+class MulPlanC(ctypes.Structure):
+    _fields_ = [
+        ("len_a", u64),
+        ("len_b", u64),
+        ("primes", u32),
+        ("chunk_bits", u32),
+        ("log_len", u32),
+        ("valid", u32),
+    ]
+
+
+class ConvPlanC(ctypes.Structure):
+    _fields_ = [
+        ("len_a", u64),
+        ("len_b", u64),
+        ("max_a", u64),
+        ("max_b", u64),
+        ("stride", u64),
+        ("output_len", u64),
+        ("is_signed", u32),
+        ("schoolbook", u32),
+        ("primes", u32),
+        ("log_len", u32),
+        ("bound_bits", u32),
+        ("value_bits_a", u32),
+        ("value_bits_b", u32),
+        ("valid", u32),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "ntt_mul_plan_new": (ctypes.c_bool, [size_t, size_t, ctypes.POINTER(MulPlanC)]),
+    "ntt_mul_plan_scratch_len": (size_t, [ctypes.POINTER(MulPlanC)]),
+    "ntt_mul_plan_sqr_scratch_len": (size_t, [ctypes.POINTER(MulPlanC)]),
+    "ntt_mul_plan_mul": (
+        ctypes.c_bool,
+        [ctypes.POINTER(MulPlanC), P64, size_t, P64, size_t, P64, size_t, P64, size_t],
+    ),
+    "ntt_mul_plan_sqr": (ctypes.c_bool, [ctypes.POINTER(MulPlanC), P64, size_t, P64, size_t, P64, size_t]),
+    "ntt_mul_workspace_new": (ctypes.c_void_p, []),
+    "ntt_mul_workspace_free": (None, [ctypes.c_void_p]),
+    "ntt_mul_workspace_prepare": (None, [ctypes.c_void_p, ctypes.POINTER(MulPlanC)]),
+    "ntt_mul_workspace_mul": (ctypes.c_bool, [ctypes.c_void_p, P64, size_t, P64, size_t, P64, size_t]),
+    "ntt_mul_workspace_sqr": (ctypes.c_bool, [ctypes.c_void_p, P64, size_t, P64, size_t]),
+    "ntt_conv_plan_new": (ctypes.c_bool, [size_t, size_t, u64, u64, ctypes.c_bool, ctypes.POINTER(ConvPlanC)]),
+    "ntt_conv_plan_stride": (size_t, [ctypes.POINTER(ConvPlanC)]),
+    "ntt_conv_plan_output_len": (size_t, [ctypes.POINTER(ConvPlanC)]),
+    "ntt_conv_plan_scratch_len": (size_t, [ctypes.POINTER(ConvPlanC)]),
+    "ntt_conv_plan_convolve": (
+        ctypes.c_bool,
+        [ctypes.POINTER(ConvPlanC), P64, size_t, P64, size_t, P64, size_t, P64, size_t],
+    ),
+    "ntt_conv_workspace_new": (ctypes.c_void_p, []),
+    "ntt_conv_workspace_free": (None, [ctypes.c_void_p]),
+    "ntt_conv_workspace_prepare": (None, [ctypes.c_void_p, ctypes.POINTER(ConvPlanC)]),
+    "ntt_conv_workspace_convolve": (ctypes.c_bool, [ctypes.c_void_p, P64, size_t, P64, size_t, P64, size_t]),
+    "ntt_mul_workspace_mul_device": (
+        ctypes.c_bool,
+        [ctypes.c_void_p, ctypes.c_void_p, size_t, ctypes.c_void_p, size_t, ctypes.c_void_p, size_t, ctypes.c_void_p],
+    ),
+    "ntt_mul_workspace_sqr_device": (
+        ctypes.c_bool,
+        [ctypes.c_void_p, ctypes.c_void_p, size_t, ctypes.c_void_p, size_t, ctypes.c_void_p],
+    ),
+    "ntt_conv_workspace_convolve_device": (
+        ctypes.c_bool,
+        [ctypes.c_void_p, ctypes.c_void_p, size_t, ctypes.c_void_p, size_t, ctypes.c_void_p, size_t, ctypes.c_void_p],
+    ),
+    "ntt_device_available": (ctypes.c_int, []),
+    "ntt_device_alloc": (ctypes.c_void_p, [size_t]),
+    "ntt_device_free": (None, [ctypes.c_void_p]),
+    "ntt_copy_to_device": (ctypes.c_bool, [ctypes.c_void_p, ctypes.c_void_p, size_t]),
+    "ntt_copy_to_host": (ctypes.c_bool, [ctypes.c_void_p, ctypes.c_void_p, size_t]),
+    "ntt_device_synchronize": (ctypes.c_bool, []),
+    "ntt_last_error": (ctypes.c_char_p, []),
+    "ntt_profile_enable": (None, [ctypes.c_int]),
+    "ntt_profile_read": (
+        ctypes.c_int,
+        [ctypes.c_char_p, size_t, ctypes.POINTER(ctypes.c_double), P64, ctypes.c_int],
+    ),
+    "ntt_profile_reset": (None, []),
+    "ntt_mul_geometry": (
+        ctypes.c_bool,
+        [size_t, size_t, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)],
+    ),
+}
+
+_lib = None
+
+
+def load(path: str | None = None) -> ctypes.CDLL:
+    """Loads the engine library and declares every entry point."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise ImportError(
+            f"nttanafis_amd: {path} is missing; build it with nttanafis_amd.build.build()"
+        )
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path == LIB_PATH:
+        _lib = lib
+    return lib

@@ -1,0 +1,309 @@
+"""Python mirror of the reference crate's public multiplication API.
+
+Names, argument meaning and failure behaviour follow the reference:
+``MulPlan.new`` returns ``None`` where ``MulPlan::new`` does (src/plan.rs:53),
+calls return ``False`` and leave ``dst`` unchanged on failure, success writes
+only the product span, and passing the same array as both operands squares
+(src/plan.rs:94-114, src/convolve.rs:148-162). Limbs are numpy ``uint64``
+arrays, least significant first. Every call runs on the gfx950 engine
+through the C ABI of include/nttanafis.h.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from . import _ffi
+
+
+def _lib():
+    return _ffi.load()
+
+
+def _ptr(arr: Optional[np.ndarray]):
+    if arr is None:
+        return ctypes.cast(None, _ffi.P64)
+    return arr.ctypes.data_as(_ffi.P64)
+
+
+def _limbs(arr, writable: bool = False) -> np.ndarray:
+    if not isinstance(arr, np.ndarray) or arr.dtype != np.uint64 or arr.ndim != 1:
+        raise TypeError("limb arrays are one-dimensional numpy uint64 arrays")
+    if not arr.flags["C_CONTIGUOUS"]:
+        raise TypeError("limb arrays must be contiguous")
+    if writable and not arr.flags["WRITEABLE"]:
+        raise TypeError("dst must be writable")
+    return arr
+
+
+def last_error() -> str:
+    msg = _lib().ntt_last_error()
+    return msg.decode() if msg else ""
+
+
+class MulPlan:
+    """An admitted product of two operand lengths (src/plan.rs:33-120)."""
+
+    __slots__ = ("_c",)
+
+    def __init__(self, c: _ffi.MulPlanC):
+        self._c = c
+
+    @classmethod
+    def new(cls, len_a: int, len_b: int) -> Optional["MulPlan"]:
+        c = _ffi.MulPlanC()
+        if not _lib().ntt_mul_plan_new(len_a, len_b, ctypes.byref(c)):
+            return None
+        return cls(c)
+
+    @property
+    def len_a(self) -> int:
+        return self._c.len_a
+
+    @property
+    def len_b(self) -> int:
+        return self._c.len_b
+
+    @property
+    def primes(self) -> int:
+        return self._c.primes
+
+    @property
+    def chunk_bits(self) -> int:
+        return self._c.chunk_bits
+
+    @property
+    def log_len(self) -> int:
+        return self._c.log_len
+
+    def scratch_len(self) -> int:
+        return _lib().ntt_mul_plan_scratch_len(ctypes.byref(self._c))
+
+    def sqr_scratch_len(self) -> int:
+        return _lib().ntt_mul_plan_sqr_scratch_len(ctypes.byref(self._c))
+
+    def mul(self, dst: np.ndarray, a: np.ndarray, b: np.ndarray, scratch: Optional[np.ndarray] = None) -> bool:
+        _limbs(dst, True), _limbs(a), _limbs(b)
+        if scratch is not None:
+            _limbs(scratch, True)
+        # The same array twice squares, as core::ptr::eq on slices does.
+        bb = a if (b is a or (b.ctypes.data == a.ctypes.data and len(a) == len(b))) else b
+        return bool(
+            _lib().ntt_mul_plan_mul(
+                ctypes.byref(self._c),
+                _ptr(dst), len(dst),
+                _ptr(a), len(a),
+                _ptr(bb), len(bb),
+                _ptr(scratch), 0 if scratch is None else len(scratch),
+            )
+        )
+
+    def sqr(self, dst: np.ndarray, a: np.ndarray, scratch: Optional[np.ndarray] = None) -> bool:
+        _limbs(dst, True), _limbs(a)
+        return bool(
+            _lib().ntt_mul_plan_sqr(
+                ctypes.byref(self._c), _ptr(dst), len(dst), _ptr(a), len(a),
+                _ptr(scratch), 0 if scratch is None else len(scratch),
+            )
+        )
+
+    def __repr__(self) -> str:
+        return (f"MulPlan(len_a={self.len_a}, len_b={self.len_b}, primes={self.primes}, "
+                f"chunk_bits={self.chunk_bits}, log_len={self.log_len})")
+
+
+class MulWorkspace:
+    """Reusable device tables and arenas (src/plan.rs:134-212)."""
+
+    def __init__(self):
+        self._h = _lib().ntt_mul_workspace_new()
+        if not self._h:
+            raise MemoryError("ntt_mul_workspace_new")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().ntt_mul_workspace_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prepare(self, plan: MulPlan) -> None:
+        _lib().ntt_mul_workspace_prepare(self._h, ctypes.byref(plan._c))
+
+    def mul(self, dst: np.ndarray, a: np.ndarray, b: np.ndarray) -> bool:
+        _limbs(dst, True), _limbs(a), _limbs(b)
+        bb = a if (b is a or (b.ctypes.data == a.ctypes.data and len(a) == len(b))) else b
+        return bool(_lib().ntt_mul_workspace_mul(self._h, _ptr(dst), len(dst), _ptr(a), len(a), _ptr(bb), len(bb)))
+
+    def sqr(self, dst: np.ndarray, a: np.ndarray) -> bool:
+        _limbs(dst, True), _limbs(a)
+        return bool(_lib().ntt_mul_workspace_sqr(self._h, _ptr(dst), len(dst), _ptr(a), len(a)))
+
+    # Device-resident operands (addresses from DeviceBuffer.ptr).
+    def mul_device(self, d_dst: int, dst_len: int, d_a: int, a_len: int, d_b: int, b_len: int,
+                   stream: Optional[int] = None) -> bool:
+        return bool(_lib().ntt_mul_workspace_mul_device(self._h, d_dst, dst_len, d_a, a_len, d_b, b_len, stream))
+
+    def sqr_device(self, d_dst: int, dst_len: int, d_a: int, a_len: int, stream: Optional[int] = None) -> bool:
+        return bool(_lib().ntt_mul_workspace_sqr_device(self._h, d_dst, dst_len, d_a, a_len, stream))
+
+
+class ConvPlan:
+    """Exact dense convolution plan (src/convolve.rs:51-185)."""
+
+    __slots__ = ("_c",)
+
+    def __init__(self, c: _ffi.ConvPlanC):
+        self._c = c
+
+    @classmethod
+    def new(cls, len_a: int, len_b: int, max_a: int, max_b: int, signed: bool) -> Optional["ConvPlan"]:
+        c = _ffi.ConvPlanC()
+        if not _lib().ntt_conv_plan_new(len_a, len_b, max_a, max_b, bool(signed), ctypes.byref(c)):
+            return None
+        return cls(c)
+
+    def stride(self) -> int:
+        return _lib().ntt_conv_plan_stride(ctypes.byref(self._c))
+
+    def output_len(self) -> int:
+        return _lib().ntt_conv_plan_output_len(ctypes.byref(self._c))
+
+    def scratch_len(self) -> int:
+        return _lib().ntt_conv_plan_scratch_len(ctypes.byref(self._c))
+
+    @property
+    def primes(self) -> int:
+        return self._c.primes
+
+    @property
+    def log_len(self) -> int:
+        return self._c.log_len
+
+    @property
+    def schoolbook(self) -> bool:
+        return bool(self._c.schoolbook)
+
+    def convolve(self, dst: np.ndarray, a: np.ndarray, b: np.ndarray, scratch: Optional[np.ndarray] = None) -> bool:
+        _limbs(dst, True), _limbs(a), _limbs(b)
+        bb = a if (b is a or (b.ctypes.data == a.ctypes.data and len(a) == len(b))) else b
+        return bool(
+            _lib().ntt_conv_plan_convolve(
+                ctypes.byref(self._c), _ptr(dst), len(dst), _ptr(a), len(a), _ptr(bb), len(bb),
+                _ptr(scratch), 0 if scratch is None else len(scratch),
+            )
+        )
+
+
+class ConvWorkspace:
+    """Reusable storage for repeated convolutions (src/convolve.rs:213-281)."""
+
+    def __init__(self):
+        self._h = _lib().ntt_conv_workspace_new()
+        if not self._h:
+            raise MemoryError("ntt_conv_workspace_new")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().ntt_conv_workspace_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prepare(self, plan: ConvPlan) -> None:
+        _lib().ntt_conv_workspace_prepare(self._h, ctypes.byref(plan._c))
+
+    def convolve(self, dst: np.ndarray, a: np.ndarray, b: np.ndarray) -> bool:
+        _limbs(dst, True), _limbs(a), _limbs(b)
+        bb = a if (b is a or (b.ctypes.data == a.ctypes.data and len(a) == len(b))) else b
+        return bool(_lib().ntt_conv_workspace_convolve(self._h, _ptr(dst), len(dst), _ptr(a), len(a), _ptr(bb), len(bb)))
+
+    def convolve_device(self, d_dst: int, dst_len: int, d_a: int, a_len: int, d_b: int, b_len: int,
+                        stream: Optional[int] = None) -> bool:
+        return bool(_lib().ntt_conv_workspace_convolve_device(self._h, d_dst, dst_len, d_a, a_len, d_b, b_len, stream))
+
+
+class DeviceBuffer:
+    """Device memory owned by the engine's allocator (hipMalloc)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = _lib().ntt_device_alloc(self.nbytes)
+        if not self.ptr:
+            raise MemoryError(f"ntt_device_alloc({nbytes})")
+
+    @classmethod
+    def from_host(cls, arr: np.ndarray) -> "DeviceBuffer":
+        buf = cls(arr.nbytes)
+        buf.upload(arr)
+        return buf
+
+    def upload(self, arr: np.ndarray) -> None:
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > self.nbytes or not _lib().ntt_copy_to_device(self.ptr, arr.ctypes.data, arr.nbytes):
+            raise RuntimeError("ntt_copy_to_device failed")
+
+    def download(self, count: int, dtype=np.uint64) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes > self.nbytes or not _lib().ntt_copy_to_host(out.ctypes.data, self.ptr, out.nbytes):
+            raise RuntimeError("ntt_copy_to_host failed")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            _lib().ntt_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_available() -> bool:
+    return bool(_lib().ntt_device_available())
+
+
+def synchronize() -> bool:
+    return bool(_lib().ntt_device_synchronize())
+
+
+def mul_geometry(len_a: int, len_b: int):
+    """(primes, chunk_bits, log_len) the engine picks for a product shape."""
+    p, w, k = _ffi.u32(), _ffi.u32(), _ffi.u32()
+    if not _lib().ntt_mul_geometry(len_a, len_b, ctypes.byref(p), ctypes.byref(w), ctypes.byref(k)):
+        return None
+    return p.value, w.value, k.value
+
+
+class Profile:
+    """Per-kernel event timing of the engine's launches."""
+
+    @staticmethod
+    def enable(on: bool = True) -> None:
+        _lib().ntt_profile_enable(1 if on else 0)
+
+    @staticmethod
+    def reset() -> None:
+        _lib().ntt_profile_reset()
+
+    @staticmethod
+    def read() -> dict:
+        cap = 64
+        names = ctypes.create_string_buffer(8192)
+        ms = (ctypes.c_double * cap)()
+        counts = (ctypes.c_uint64 * cap)()
+        n = _lib().ntt_profile_read(names, len(names), ms, counts, cap)
+        raw = names.raw.split(b"\0")
+        return {raw[i].decode(): (ms[i], counts[i]) for i in range(n)}

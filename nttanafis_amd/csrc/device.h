@@ -1,0 +1,116 @@
+// Device-side arithmetic and kernel argument blocks of the gfx950 engine.
+//
+// Residues are integer-valued binary64 values. A product of two residues is
+// reduced exactly with one FMA recovering the rounding error of the product
+// and one FMA subtracting the rounded quotient times p, the scheme of the
+// reference's src/arch/scalar/product.rs:96-109. Every kernel here is compiled
+// with -ffp-contract=off, so only the fma() calls below fuse.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nttamd {
+
+constexpr int kMaxPrimes = 8;
+constexpr int kMaxWords = 7;     // ceil(8 * 50 / 64): words of one CRT value
+constexpr int kPartBits = 49;    // bits of one packing part, below every prime
+constexpr int kMaxParts = 4;     // parts of the widest coefficient (196 bits)
+constexpr int kMinLog = 5;       // shortest transform: 32 points
+constexpr int kSinglePassMaxLog = 12;
+
+// Per-prime constants, passed by value to every kernel.
+struct Field {
+    double p[kMaxPrimes];
+    double pinv[kMaxPrimes];
+};
+
+// Exact residue product: |a*b| < 2p^2 gives |r| < 15p/16, |a*b| < 4p^2 gives
+// |r| < 11p/8 for every catalog prime.
+__device__ __forceinline__ double mulmod(double a, double b, double p, double pinv) {
+    const double high = a * b;
+    const double low = fma(a, b, -high);
+    const double q = rint(high * pinv);
+    return fma(-q, p, high) - low;
+}
+
+// Centered reduction of |x| < 8p to |r| < 9p/16 (|r| <= p/2 for |x| < p).
+__device__ __forceinline__ double reduce(double x, double p, double pinv) {
+    return fma(-rint(x * pinv), p, x);
+}
+
+// Canonical residue in [0, p) of |x| < p.
+__device__ __forceinline__ double canonical(double x, double p) { return x < 0.0 ? x + p : x; }
+
+// Operand description for packing coefficients straight from limbs.
+struct Source {
+    const uint64_t* limbs;  // little-endian limbs
+    uint64_t limb_len;
+    uint64_t digits;        // coefficients; later columns are zero
+    uint32_t width;         // bits between coefficients (64: one limb each)
+    uint32_t value_bits;    // significant bits per coefficient (<= width)
+    uint32_t is_signed;     // two's complement coefficients (width 64 only)
+    uint32_t pad;
+};
+
+// Arguments of every transform pass kernel.
+struct PassArgs {
+    double* data;            // rows of one operand, prime y at data + y * row_stride
+    const double* other;     // fused passes: the other operand's spectra (same layout)
+    const double* fwd;       // forward root table, prime y at fwd + y * table_stride
+    const double* inv;       // inverse root table
+    uint64_t row_stride;     // T
+    uint64_t table_stride;   // entries per prime in the root tables
+    int32_t log_t;           // K = log2(T)
+    int32_t level_start;     // global level of this pass's first level
+    uint64_t line_stride;    // element stride of a strided pass, 1 for rows
+    Source src_a;            // packing sources (first forward pass)
+    Source src_b;
+    Field field;
+    double tinv[kMaxPrimes];                 // centered T^-1 mod p
+    double powers[kMaxPrimes][kMaxParts];    // centered 2^(49k) mod p
+};
+
+// Arguments of the CRT and carry kernels.
+struct CrtArgs {
+    const double* residues;  // canonical residues, prime y at residues + y * row_stride
+    uint64_t row_stride;
+    uint64_t coefficients;   // number of convolution coefficients
+    uint64_t out_len;        // output limbs
+    uint64_t* out;           // output limbs (device)
+    uint32_t width;          // bits between coefficients (product) or stride*64 (conv)
+    uint32_t value_words;    // words of one reconstructed value
+    uint32_t value_bits;     // bits of one reconstructed value (bound)
+    uint32_t stride;         // conv: limbs per coefficient
+    uint32_t is_signed;
+    uint32_t primes;
+    Field field;
+    double pmod[kMaxPrimes][kMaxPrimes];  // centered p_j mod p_k (j < k)
+    double ginv[kMaxPrimes];              // centered (p_0..p_{k-1})^-1 mod p_k
+    uint64_t pint[kMaxPrimes];
+    // carry bookkeeping, one entry per carry block
+    uint32_t* agg_base;      // B_end of each block
+    uint32_t* agg_thresh;    // T_end of each block
+    uint32_t* first_thresh;  // t of the block's first limb
+    uint32_t* ripple_end;    // last limb index (relative) reached by a ripple
+    uint32_t* carry_in;      // carry into each block, from the scan
+    uint32_t blocks;
+};
+
+// Root-table generation arguments.
+struct TableArgs {
+    double* fwd;
+    double* inv;
+    uint64_t table_stride;
+    uint64_t len;
+    Field field;
+    double roots[kMaxPrimes][48];
+    double inv_roots[kMaxPrimes][48];
+};
+
+constexpr uint32_t kInf = 0xFFFFFFFFu;
+constexpr int kCrtThreads = 256;
+constexpr int kCrtLimbsPerThread = 4;
+constexpr int kCrtBlockLimbs = kCrtThreads * kCrtLimbsPerThread;
+
+}  // namespace nttamd

@@ -1,0 +1,1047 @@
+// Host side of the gfx950 multi-prime NTT engine: catalog constants, device
+// geometry, device context, the product/convolution pipelines and the C ABI
+// declared in include/nttanafis.h.
+#include "device.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nttanafis.h"
+
+namespace nttamd {
+
+typedef unsigned __int128 u128;
+
+bool launch_fwd_strided(int logl, const PassArgs& a, int primes, bool pack, hipStream_t s);
+bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, hipStream_t s);
+bool launch_fwd_rows(int logl, const PassArgs& a, int primes, hipStream_t s);
+bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, hipStream_t s);
+bool launch_crt_product(const CrtArgs& a, hipStream_t s);
+bool launch_crt_disjoint(const CrtArgs& a, hipStream_t s);
+bool launch_conv_columns(const uint64_t* a, uint64_t la, const uint64_t* b, uint64_t lb, uint64_t* out,
+                         uint32_t stride, uint32_t is_signed, hipStream_t s);
+
+bool launch_tables(const TableArgs& a, int primes, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Catalog: the reference's eight primes 2^49 < p < 2^50 with generators of
+// their two-power subgroups (src/field/catalog.rs:67-76). Everything else is
+// derived here at start-up with exact integer arithmetic.
+
+static const uint64_t kCatalog[kMaxPrimes][2] = {
+    {1108307720798209ULL, 11}, {1086317488242689ULL, 3}, {910395627798529ULL, 7},
+    {659706976665601ULL, 11},  {699289395265537ULL, 5},  {1022545813831681ULL, 7},
+    {1013749720809473ULL, 3},  {868614185943041ULL, 3},
+};
+constexpr int kMaxRootOrder = 41;  // min 2-adic valuation of p-1 over the catalog
+constexpr int kMaxChunkBits = kPartBits * kMaxParts;
+constexpr int kMaxLog = 30;
+constexpr uint64_t kSchoolbookMaxTerms = 24;  // src/convolve.rs:24
+
+static uint64_t pow_mod(uint64_t b, uint64_t e, uint64_t m) {
+    u128 v = 1, f = b % m;
+    while (e) {
+        if (e & 1) v = v * f % m;
+        f = f * f % m;
+        e >>= 1;
+    }
+    return (uint64_t)v;
+}
+
+static double centered(uint64_t v, uint64_t m) { return v > m / 2 ? (double)v - (double)m : (double)v; }
+
+struct Catalog {
+    uint64_t p[kMaxPrimes];
+    Field field;
+    double roots[kMaxPrimes][48];
+    double inv_roots[kMaxPrimes][48];
+    double powers[kMaxPrimes][kMaxParts];
+    uint32_t capacity[kMaxPrimes + 1];  // by prime count
+    // Garner constants for a prefix of `count` primes do not depend on count.
+    double pmod[kMaxPrimes][kMaxPrimes];
+    double ginv[kMaxPrimes];
+};
+
+static Catalog g_cat;
+static std::once_flag g_cat_once;
+
+static void init_catalog() {
+    Catalog& c = g_cat;
+    memset(&c, 0, sizeof c);
+    for (int i = 0; i < kMaxPrimes; ++i) {
+        const uint64_t m = kCatalog[i][0], g = kCatalog[i][1];
+        c.p[i] = m;
+        c.field.p[i] = (double)m;
+        c.field.pinv[i] = 1.0 / (double)m;
+        const int v = __builtin_ctzll(m - 1);
+        for (int level = 0; level < 48; ++level) {
+            if (level + 2 > v) continue;
+            const uint64_t r = pow_mod(g, (m - 1) >> (level + 2), m);
+            c.roots[i][level] = centered(r, m);
+            c.inv_roots[i][level] = centered(pow_mod(r, m - 2, m), m);
+        }
+        u128 power = 1;
+        for (int part = 0; part < kMaxParts; ++part) {
+            c.powers[i][part] = centered((uint64_t)power, m);
+            power = (power << kPartBits) % m;
+        }
+    }
+    // Capacity floor(log2(2P/3)) of each prefix (src/limb/cofactor.rs:239-255).
+    uint64_t prod[kMaxWords + 1] = {1};
+    int words = 1;
+    for (int count = 1; count <= kMaxPrimes; ++count) {
+        u128 carry = 0;
+        for (int w = 0; w < words; ++w) {
+            const u128 full = (u128)prod[w] * c.p[count - 1] + carry;
+            prod[w] = (uint64_t)full;
+            carry = full >> 64;
+        }
+        if (carry) prod[words++] = (uint64_t)carry;
+        const int bits = (words - 1) * 64 + (64 - __builtin_clzll(prod[words - 1]));
+        const int second = bits - 2;
+        c.capacity[count] = ((prod[second >> 6] >> (second & 63)) & 1) ? bits - 1 : second;
+    }
+    for (int k = 0; k < kMaxPrimes; ++k) {
+        const uint64_t pk = c.p[k];
+        u128 prefix = 1;
+        for (int j = 0; j < k; ++j) {
+            c.pmod[j][k] = centered(c.p[j] % pk, pk);
+            prefix = prefix * (c.p[j] % pk) % pk;
+        }
+        c.ginv[k] = k == 0 ? 1.0 : centered(pow_mod((uint64_t)prefix, pk - 2, pk), pk);
+    }
+}
+
+static const Catalog& cat() {
+    std::call_once(g_cat_once, init_catalog);
+    return g_cat;
+}
+
+// ---------------------------------------------------------------------------
+// Errors and profiling.
+
+static thread_local std::string t_error;
+static void set_error(const std::string& e) { t_error = e; }
+
+struct ProfEntry {
+    const char* name;
+    hipEvent_t start, stop;
+};
+static std::mutex g_prof_lock;
+static std::atomic<int> g_prof_on{0};
+static std::vector<ProfEntry> g_prof_pending;
+static std::vector<std::pair<std::string, std::pair<double, uint64_t>>> g_prof_totals;
+
+struct Timed {
+    const char* name;
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    bool on;
+    Timed(const char* n, hipStream_t st) : name(n), s(st), on(g_prof_on.load() != 0) {
+        if (on && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
+            hipEventRecord(a, s);
+        } else {
+            on = false;
+        }
+    }
+    ~Timed() {
+        if (!on) return;
+        hipEventRecord(b, s);
+        std::lock_guard<std::mutex> g(g_prof_lock);
+        g_prof_pending.push_back({name, a, b});
+    }
+};
+
+static void prof_drain() {
+    std::lock_guard<std::mutex> g(g_prof_lock);
+    for (auto& e : g_prof_pending) {
+        float ms = 0;
+        hipEventSynchronize(e.stop);
+        hipEventElapsedTime(&ms, e.start, e.stop);
+        bool found = false;
+        for (auto& t : g_prof_totals) {
+            if (t.first == e.name) {
+                t.second.first += ms;
+                t.second.second += 1;
+                found = true;
+                break;
+            }
+        }
+        if (!found) g_prof_totals.push_back({e.name, {ms, 1}});
+        hipEventDestroy(e.start);
+        hipEventDestroy(e.stop);
+    }
+    g_prof_pending.clear();
+}
+
+// ---------------------------------------------------------------------------
+// Geometry of one device operation.
+
+struct Geometry {
+    uint32_t primes = 0;
+    uint32_t width = 0;         // bits between operand coefficients
+    uint32_t value_bits_a = 0;  // significant bits per coefficient
+    uint32_t value_bits_b = 0;
+    uint64_t digits_a = 0, digits_b = 0;
+    uint64_t conv_len = 0;
+    int log_t = 0;
+    uint32_t bound_bits = 0;  // bits of every reconstructed coefficient
+    bool disjoint = false;
+    bool is_signed = false;
+    uint32_t stride = 0;  // limbs per result coefficient (convolution)
+};
+
+static int bitlen(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
+
+static int log2_ceil(uint64_t x) {
+    int k = 0;
+    while (((uint64_t)1 << k) < x) ++k;
+    return k;
+}
+
+static int crt_words(int np) {
+    static const int w[9] = {0, 1, 2, 3, 4, 4, 5, 6, 7};
+    return w[np];
+}
+
+static int passes_of(int K) { return K <= kSinglePassMaxLog ? 1 : (K <= 23 ? 2 : 3); }
+
+// Product geometry: for every prefix, the widest coefficient whose products
+// fit its capacity (the narrowing of src/profile/planner.rs:168-183), priced
+// by device work: HBM passes and transform levels over np*T points.
+static bool select_product(uint64_t la, uint64_t lb, Geometry* out) {
+    const Catalog& c = cat();
+    if (la == 0 || lb == 0 || la > (UINT64_MAX >> 7) || lb > (UINT64_MAX >> 7)) return false;
+    const uint64_t bits_a = la * 64, bits_b = lb * 64;
+    bool found = false;
+    double best = 0;
+    static const int forced = getenv("NTTAMD_PRIMES") ? atoi(getenv("NTTAMD_PRIMES")) : 0;
+    for (int np = 1; np <= kMaxPrimes; ++np) {
+        if (forced && np != forced) continue;
+        const uint32_t cap = c.capacity[np];
+        uint64_t w = std::min<uint64_t>(cap / 2, kMaxChunkBits);
+        uint64_t da = 0, db = 0, pb = 0;
+        bool ok = false;
+        for (;;) {
+            da = (bits_a + w - 1) / w;
+            db = (bits_b + w - 1) / w;
+            const uint64_t terms = std::min(da, db);
+            pb = 2 * w + bitlen(terms - 1);
+            if (pb <= cap) {
+                ok = true;
+                break;
+            }
+            const uint64_t cut = (pb - cap + 1) / 2;
+            if (cut >= w) break;
+            w -= cut;
+        }
+        if (!ok) continue;
+        const uint64_t conv = da + db - 1;
+        const int K = std::max(kMinLog, log2_ceil(conv));
+        if (K > kMaxLog || K > kMaxRootOrder - 1) continue;
+        const int nw = crt_words(np);
+        const uint64_t span = (64 * (uint64_t)kCrtBlockLimbs + pb) / w + 2;
+        if (span * nw > 4096) continue;
+        const double T = (double)((uint64_t)1 << K);
+        const double cost = np * T * (8.0 * passes_of(K) + K) + 4.0 * np * np * (double)conv;
+        if (!found || cost < best) {
+            found = true;
+            best = cost;
+            Geometry g;
+            g.primes = np;
+            g.width = (uint32_t)w;
+            g.value_bits_a = g.value_bits_b = (uint32_t)w;
+            g.digits_a = da;
+            g.digits_b = db;
+            g.conv_len = conv;
+            g.log_t = K;
+            g.bound_bits = (uint32_t)pb;
+            *out = g;
+        }
+    }
+    return found;
+}
+
+// bound_bits of src/convolve.rs:358-369.
+static bool conv_bound_bits(uint64_t terms, uint64_t max_a, uint64_t max_b, uint64_t* bits) {
+    const u128 product = (u128)max_a * max_b;
+    const uint64_t hi = (uint64_t)(product >> 64), lo = (uint64_t)product;
+    const u128 t_lo = (u128)lo * terms;
+    const u128 t_hi = (u128)hi * terms + (t_lo >> 64);
+    auto bitlen128 = [](u128 v) { return v >> 64 ? 64 + bitlen((uint64_t)(v >> 64)) : bitlen((uint64_t)v); };
+    if (t_hi >> 64) {
+        *bits = (uint64_t)bitlen128(product) + (uint64_t)bitlen(terms - 1);
+    } else {
+        *bits = (uint64_t)bitlen128((t_hi << 64) | (uint64_t)t_lo);
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Device context: root tables shared by every caller (grow-only; a table for
+// T serves every shorter transform) and device buffers of one executor.
+
+static std::mutex g_table_lock;
+static double* g_fwd = nullptr;
+static double* g_inv = nullptr;
+static uint64_t g_table_len = 0;  // entries per prime
+static std::vector<void*> g_retired;
+
+static bool ensure_tables(uint64_t need, hipStream_t s, const double** fwd, const double** inv,
+                          uint64_t* stride) {
+    std::lock_guard<std::mutex> g(g_table_lock);
+    if (need > g_table_len) {
+        uint64_t len = std::max<uint64_t>(need, 1 << 12);
+        double *f = nullptr, *v = nullptr;
+        if (hipMalloc(&f, len * kMaxPrimes * sizeof(double)) != hipSuccess) return false;
+        if (hipMalloc(&v, len * kMaxPrimes * sizeof(double)) != hipSuccess) {
+            hipFree(f);
+            return false;
+        }
+        const Catalog& c = cat();
+        TableArgs a;
+        memset(&a, 0, sizeof a);
+        a.fwd = f;
+        a.inv = v;
+        a.table_stride = len;
+        a.len = len;
+        a.field = c.field;
+        memcpy(a.roots, c.roots, sizeof a.roots);
+        memcpy(a.inv_roots, c.inv_roots, sizeof a.inv_roots);
+        {
+            Timed t("tables", s);
+            if (!launch_tables(a, kMaxPrimes, s)) return false;
+        }
+        if (hipStreamSynchronize(s) != hipSuccess) return false;
+        // Older tables may still be read by queued work of other executors.
+        if (g_fwd) g_retired.push_back(g_fwd);
+        if (g_inv) g_retired.push_back(g_inv);
+        g_fwd = f;
+        g_inv = v;
+        g_table_len = len;
+    }
+    *fwd = g_fwd;
+    *inv = g_inv;
+    *stride = g_table_len;
+    return true;
+}
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    bool ensure(size_t n) {
+        if (n <= bytes) return true;
+        if (ptr) hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        size_t want = std::max<size_t>(n, 4096);
+        if (hipMalloc(&ptr, want) != hipSuccess) {
+            ptr = nullptr;
+            return false;
+        }
+        bytes = want;
+        return true;
+    }
+    void release() {
+        if (ptr) hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+struct Executor {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    DevBuf spec_a, spec_b, in_a, in_b, out, carry;
+    std::mutex lock;
+
+    bool init() {
+        if (stream) return true;
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+            stream = nullptr;
+            return false;
+        }
+        own_stream = true;
+        return true;
+    }
+    ~Executor() {
+        spec_a.release();
+        spec_b.release();
+        in_a.release();
+        in_b.release();
+        out.release();
+        carry.release();
+        if (own_stream && stream) hipStreamDestroy(stream);
+    }
+};
+
+static Executor* plan_executor() {
+    static Executor* ex = new Executor();
+    return ex;
+}
+
+// ---------------------------------------------------------------------------
+// Pipelines.
+
+struct Split {
+    int n;
+    int logs[3];  // strided passes, then the row pass (last)
+};
+
+static Split split_of(int K) {
+    Split s{};
+    if (K <= kSinglePassMaxLog) {
+        s.n = 1;
+        s.logs[0] = K;
+        return s;
+    }
+    int rows = 12;
+    if (const char* e = getenv("NTTAMD_ROW_LOG")) rows = atoi(e);
+    if (K <= 23) {
+        int r = std::min(rows, K - 5);
+        r = std::max(r, K - 11);
+        r = std::min(r, 12);
+        s.n = 2;
+        s.logs[0] = K - r;
+        s.logs[1] = r;
+        return s;
+    }
+    const int r = 12;
+    const int k1 = (K - r + 1) / 2;
+    s.n = 3;
+    s.logs[0] = k1;
+    s.logs[1] = K - r - k1;
+    s.logs[2] = r;
+    return s;
+}
+
+static PassArgs base_args(const Geometry& g, double* data, const double* fwd, const double* inv,
+                          uint64_t table_stride) {
+    const Catalog& c = cat();
+    PassArgs a;
+    memset(&a, 0, sizeof a);
+    a.data = data;
+    a.fwd = fwd;
+    a.inv = inv;
+    a.row_stride = (uint64_t)1 << g.log_t;
+    a.table_stride = table_stride;
+    a.log_t = g.log_t;
+    a.field = c.field;
+    memcpy(a.powers, c.powers, sizeof a.powers);
+    for (uint32_t y = 0; y < g.primes; ++y) {
+        const uint64_t m = c.p[y];
+        const uint64_t tinv = m - ((m - 1) >> g.log_t);  // T^-1 mod p, T | p-1
+        a.tinv[y] = centered(tinv, m);
+    }
+    return a;
+}
+
+static Source source_of(const uint64_t* limbs, uint64_t len, uint64_t digits, uint32_t width,
+                        uint32_t value_bits, bool is_signed) {
+    Source s;
+    s.limbs = limbs;
+    s.limb_len = len;
+    s.digits = digits;
+    s.width = width;
+    s.value_bits = value_bits;
+    s.is_signed = is_signed ? 1 : 0;
+    s.pad = 0;
+    return s;
+}
+
+// Forward transform of operand `src` into `rows` through all passes but the
+// row pass; returns the args of the row pass.
+static bool forward_head(const Split& sp, const PassArgs& a0, const Source& src, int primes,
+                         hipStream_t s) {
+    PassArgs a = a0;
+    a.src_a = src;
+    int level = 0;
+    for (int i = 0; i + 1 < sp.n; ++i) {
+        a.level_start = level;
+        a.line_stride = (uint64_t)1 << (a.log_t - level - sp.logs[i]);
+        Timed t(i == 0 ? "fwd_pack_strided" : "fwd_strided", s);
+        if (!launch_fwd_strided(sp.logs[i], a, primes, i == 0, s)) return false;
+        level += sp.logs[i];
+    }
+    return true;
+}
+
+static bool inverse_tail(const Split& sp, const PassArgs& a0, int primes, hipStream_t s) {
+    PassArgs a = a0;
+    int level = a.log_t - sp.logs[sp.n - 1];
+    for (int i = sp.n - 2; i >= 0; --i) {
+        level -= sp.logs[i];
+        a.level_start = level;
+        a.line_stride = (uint64_t)1 << (a.log_t - level - sp.logs[i]);
+        Timed t(i == 0 ? "inv_strided_canon" : "inv_strided", s);
+        if (!launch_inv_strided(sp.logs[i], a, primes, i == 0, s)) return false;
+    }
+    return true;
+}
+
+// Computes the canonical residues of the convolution into ex.spec_b (or
+// spec_a for a square / single pass) and returns their pointer.
+static bool residues(Executor& ex, const Geometry& g, const uint64_t* d_a, uint64_t la,
+                     const uint64_t* d_b, uint64_t lb, bool square, double** rows_out) {
+    const hipStream_t s = ex.stream;
+    const uint64_t T = (uint64_t)1 << g.log_t;
+    const int np = (int)g.primes;
+    const double *fwd = nullptr, *inv = nullptr;
+    uint64_t tstride = 0;
+    if (!ensure_tables(T / 2, s, &fwd, &inv, &tstride)) {
+        set_error("root table allocation failed");
+        return false;
+    }
+    const size_t row_bytes = (size_t)np * T * sizeof(double);
+    if (!ex.spec_a.ensure(row_bytes) || (!square && !ex.spec_b.ensure(row_bytes))) {
+        set_error("spectrum allocation failed");
+        return false;
+    }
+    const Source sa = source_of(d_a, la, g.digits_a, g.width, g.value_bits_a, g.is_signed);
+    const Source sb = source_of(d_b, lb, g.digits_b, g.width, g.value_bits_b, g.is_signed);
+    const Split sp = split_of(g.log_t);
+    double* A = ex.spec_a.as<double>();
+    double* B = square ? A : ex.spec_b.as<double>();
+    if (sp.n == 1) {
+        PassArgs a = base_args(g, A, fwd, inv, tstride);
+        a.src_a = sa;
+        a.src_b = sb;
+        a.level_start = 0;
+        Timed t("fused_single", s);
+        if (!launch_fused(g.log_t, a, np, true, square, s)) return false;
+        *rows_out = A;
+        return true;
+    }
+    const int row_log = sp.logs[sp.n - 1];
+    const int row_level = g.log_t - row_log;
+    // Operand a: every forward pass.
+    PassArgs pa = base_args(g, A, fwd, inv, tstride);
+    if (!forward_head(sp, pa, sa, np, s)) return false;
+    if (!square) {
+        pa.level_start = row_level;
+        pa.line_stride = 1;
+        {
+            Timed t("fwd_rows", s);
+            if (!launch_fwd_rows(row_log, pa, np, s)) return false;
+        }
+        PassArgs pb = base_args(g, B, fwd, inv, tstride);
+        if (!forward_head(sp, pb, sb, np, s)) return false;
+    }
+    PassArgs pf = base_args(g, B, fwd, inv, tstride);
+    pf.other = A;
+    pf.level_start = row_level;
+    pf.line_stride = 1;
+    {
+        Timed t("fused_rows", s);
+        if (!launch_fused(row_log, pf, np, false, square, s)) return false;
+    }
+    if (!inverse_tail(sp, pf, np, s)) return false;
+    *rows_out = B;
+    return true;
+}
+
+static CrtArgs crt_args(const Geometry& g, const double* rows, uint64_t* out, uint64_t out_len) {
+    const Catalog& c = cat();
+    CrtArgs a;
+    memset(&a, 0, sizeof a);
+    a.residues = rows;
+    a.row_stride = (uint64_t)1 << g.log_t;
+    a.out_len = out_len;
+    a.out = out;
+    a.primes = g.primes;
+    a.field = c.field;
+    memcpy(a.pmod, c.pmod, sizeof a.pmod);
+    memcpy(a.ginv, c.ginv, sizeof a.ginv);
+    memcpy(a.pint, c.p, sizeof a.pint);
+    a.value_bits = g.bound_bits;
+    a.value_words = (uint32_t)crt_words((int)g.primes);
+    a.is_signed = g.is_signed ? 1 : 0;
+    return a;
+}
+
+// Product of device operands into d_out (exactly la+lb limbs).
+static bool device_product(Executor& ex, const Geometry& g, const uint64_t* d_a, uint64_t la,
+                           const uint64_t* d_b, uint64_t lb, bool square, uint64_t* d_out) {
+    double* rows = nullptr;
+    if (!residues(ex, g, d_a, la, d_b, lb, square, &rows)) return false;
+    const uint64_t out_len = la + lb;
+    CrtArgs a = crt_args(g, rows, d_out, out_len);
+    a.width = g.width;
+    const uint64_t coefficients = std::min<uint64_t>(g.conv_len, (out_len * 64 + g.width - 1) / g.width);
+    a.coefficients = coefficients;
+    const uint64_t blocks = (out_len + kCrtBlockLimbs - 1) / kCrtBlockLimbs;
+    a.blocks = (uint32_t)blocks;
+    if (!ex.carry.ensure(blocks * 5 * sizeof(uint32_t))) {
+        set_error("carry allocation failed");
+        return false;
+    }
+    uint32_t* cb = ex.carry.as<uint32_t>();
+    a.agg_base = cb;
+    a.agg_thresh = cb + blocks;
+    a.first_thresh = cb + 2 * blocks;
+    a.ripple_end = cb + 3 * blocks;
+    a.carry_in = cb + 4 * blocks;
+    Timed t("crt_carry", ex.stream);
+    return launch_crt_product(a, ex.stream);
+}
+
+static bool device_convolution(Executor& ex, const Geometry& g, const uint64_t* d_a, uint64_t la,
+                               const uint64_t* d_b, uint64_t lb, bool square, uint64_t* d_out) {
+    double* rows = nullptr;
+    if (!residues(ex, g, d_a, la, d_b, lb, square, &rows)) return false;
+    CrtArgs a = crt_args(g, rows, d_out, (la + lb - 1) * g.stride);
+    a.coefficients = la + lb - 1;
+    a.stride = g.stride;
+    a.width = g.stride * 64;
+    Timed t("crt_disjoint", ex.stream);
+    return launch_crt_disjoint(a, ex.stream);
+}
+
+// ---------------------------------------------------------------------------
+// Host-buffer calls: operands to device, run, result back, dst written last.
+
+static bool stage_and_run(Executor& ex, uint64_t out_len, const uint64_t* a, uint64_t la,
+                          const uint64_t* b, uint64_t lb, bool same,
+                          bool (*run)(Executor&, void*, const uint64_t*, const uint64_t*, uint64_t*),
+                          void* ctx, uint64_t* dst) {
+    std::lock_guard<std::mutex> guard(ex.lock);
+    if (!ex.init()) {
+        set_error("stream creation failed");
+        return false;
+    }
+    if (!ex.in_a.ensure(la * 8) || (!same && !ex.in_b.ensure(lb * 8)) || !ex.out.ensure(out_len * 8)) {
+        set_error("operand allocation failed");
+        return false;
+    }
+    const hipStream_t s = ex.stream;
+    uint64_t* da = ex.in_a.as<uint64_t>();
+    uint64_t* db = same ? da : ex.in_b.as<uint64_t>();
+    if (hipMemcpyAsync(da, a, la * 8, hipMemcpyHostToDevice, s) != hipSuccess) return false;
+    if (!same && hipMemcpyAsync(db, b, lb * 8, hipMemcpyHostToDevice, s) != hipSuccess) return false;
+    uint64_t* dout = ex.out.as<uint64_t>();
+    if (!run(ex, ctx, da, db, dout)) {
+        hipStreamSynchronize(s);
+        if (t_error.empty()) set_error("kernel launch failed");
+        return false;
+    }
+    std::vector<uint64_t> host(out_len);
+    if (hipMemcpyAsync(host.data(), dout, out_len * 8, hipMemcpyDeviceToHost, s) != hipSuccess) return false;
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        set_error("device execution failed");
+        return false;
+    }
+    memcpy(dst, host.data(), out_len * 8);
+    return true;
+}
+
+struct MulCtx {
+    const Geometry* g;
+    uint64_t la, lb;
+    bool square;
+};
+
+static bool run_mul(Executor& ex, void* ctx, const uint64_t* da, const uint64_t* db, uint64_t* dout) {
+    MulCtx* m = static_cast<MulCtx*>(ctx);
+    return device_product(ex, *m->g, da, m->la, db, m->lb, m->square, dout);
+}
+
+struct ConvCtx {
+    const Geometry* g;
+    const ntt_conv_plan* plan;
+    bool square;
+};
+
+static bool run_conv(Executor& ex, void* ctx, const uint64_t* da, const uint64_t* db, uint64_t* dout) {
+    ConvCtx* c = static_cast<ConvCtx*>(ctx);
+    const ntt_conv_plan* p = c->plan;
+    if (p->schoolbook) {
+        Timed t("conv_columns", ex.stream);
+        return launch_conv_columns(da, p->len_a, db, p->len_b, dout, (uint32_t)p->stride, p->is_signed, ex.stream);
+    }
+    return device_convolution(ex, *c->g, da, p->len_a, db, p->len_b, c->square, dout);
+}
+
+static bool product_geometry(const ntt_mul_plan* plan, Geometry* g) {
+    if (!plan || !plan->valid) return false;
+    return select_product(plan->len_a, plan->len_b, g);
+}
+
+static bool conv_geometry(const ntt_conv_plan* p, Geometry* g) {
+    if (!p || !p->valid) return false;
+    g->primes = p->primes;
+    g->width = 64;
+    g->value_bits_a = p->value_bits_a;
+    g->value_bits_b = p->value_bits_b;
+    g->digits_a = p->len_a;
+    g->digits_b = p->len_b;
+    g->conv_len = p->len_a + p->len_b - 1;
+    g->log_t = (int)p->log_len;
+    g->bound_bits = p->bound_bits;
+    g->disjoint = true;
+    g->is_signed = p->is_signed != 0;
+    g->stride = (uint32_t)p->stride;
+    return true;
+}
+
+static bool mul_host(Executor& ex, const ntt_mul_plan* plan, uint64_t* dst, size_t dst_len,
+                     const uint64_t* a, size_t a_len, const uint64_t* b, size_t b_len, bool square) {
+    t_error.clear();
+    Geometry g;
+    if (!product_geometry(plan, &g)) return false;
+    if (a_len != plan->len_a || b_len != plan->len_b) return false;
+    const uint64_t out_len = (uint64_t)a_len + b_len;
+    if (!dst || dst_len < out_len || !a || (!square && !b)) return false;
+    MulCtx ctx{&g, a_len, b_len, square};
+    return stage_and_run(ex, out_len, a, a_len, b, b_len, square, run_mul, &ctx, dst);
+}
+
+// ConvPlan::admits, src/convolve.rs:169-184.
+static bool conv_admits(const ntt_conv_plan* p, const uint64_t* a, size_t a_len, const uint64_t* b,
+                        size_t b_len) {
+    if (a_len != p->len_a || b_len != p->len_b || !a || !b) return false;
+    const uint64_t* ops[2] = {a, b};
+    const size_t lens[2] = {a_len, b_len};
+    const uint64_t maxes[2] = {p->max_a, p->max_b};
+    for (int s = 0; s < 2; ++s) {
+        if (p->is_signed) {
+            for (size_t i = 0; i < lens[s]; ++i) {
+                const int64_t v = (int64_t)ops[s][i];
+                const uint64_t mag = v < 0 ? 0 - (uint64_t)v : (uint64_t)v;
+                if (mag > maxes[s]) return false;
+            }
+        } else {
+            uint64_t u = 0;
+            for (size_t i = 0; i < lens[s]; ++i) u |= ops[s][i];
+            if (u > maxes[s])
+                for (size_t i = 0; i < lens[s]; ++i)
+                    if (ops[s][i] > maxes[s]) return false;
+        }
+    }
+    return true;
+}
+
+static bool conv_host(Executor& ex, const ntt_conv_plan* plan, uint64_t* dst, size_t dst_len,
+                      const uint64_t* a, size_t a_len, const uint64_t* b, size_t b_len) {
+    t_error.clear();
+    if (!plan || !plan->valid) return false;
+    if (!conv_admits(plan, a, a_len, b, b_len)) return false;
+    if (!dst || dst_len < plan->output_len) return false;
+    Geometry g;
+    if (!conv_geometry(plan, &g)) return false;
+    const bool square = a == b && a_len == b_len;
+    ConvCtx ctx{&g, plan, square && !plan->schoolbook};
+    return stage_and_run(ex, plan->output_len, a, a_len, b, b_len, square, run_conv, &ctx, dst);
+}
+
+}  // namespace nttamd
+
+// ---------------------------------------------------------------------------
+// C ABI.
+
+using namespace nttamd;
+
+struct ntt_mul_workspace {
+    Executor ex;
+    ntt_mul_plan plan;
+    bool has_plan = false;
+};
+
+struct ntt_conv_workspace {
+    Executor ex;
+    ntt_conv_plan plan;
+    bool has_plan = false;
+};
+
+extern "C" {
+
+bool ntt_mul_plan_new(size_t len_a, size_t len_b, ntt_mul_plan* out) {
+    if (!out || len_a == 0 || len_b == 0) return false;
+    if (len_a + len_b < len_a) return false;
+    Geometry g;
+    if (!select_product(len_a, len_b, &g)) return false;
+    ntt_mul_plan p;
+    memset(&p, 0, sizeof p);
+    p.len_a = len_a;
+    p.len_b = len_b;
+    p.primes = g.primes;
+    p.chunk_bits = g.width;
+    p.log_len = (uint32_t)g.log_t;
+    p.valid = 1;
+    *out = p;
+    return true;
+}
+
+size_t ntt_mul_plan_scratch_len(const ntt_mul_plan*) { return 0; }
+size_t ntt_mul_plan_sqr_scratch_len(const ntt_mul_plan*) { return 0; }
+
+bool ntt_mul_plan_mul(const ntt_mul_plan* plan, uint64_t* dst, size_t dst_len, const uint64_t* a,
+                      size_t a_len, const uint64_t* b, size_t b_len, uint64_t*, size_t) {
+    const bool square = a == b && a_len == b_len;
+    return mul_host(*plan_executor(), plan, dst, dst_len, a, a_len, b, b_len, square);
+}
+
+bool ntt_mul_plan_sqr(const ntt_mul_plan* plan, uint64_t* dst, size_t dst_len, const uint64_t* a,
+                      size_t a_len, uint64_t*, size_t) {
+    if (!plan || plan->len_a != plan->len_b) return false;
+    return mul_host(*plan_executor(), plan, dst, dst_len, a, a_len, a, a_len, true);
+}
+
+ntt_mul_workspace* ntt_mul_workspace_new(void) { return new (std::nothrow) ntt_mul_workspace(); }
+
+void ntt_mul_workspace_free(ntt_mul_workspace* ws) { delete ws; }
+
+void ntt_mul_workspace_prepare(ntt_mul_workspace* ws, const ntt_mul_plan* plan) {
+    if (!ws || !plan || !plan->valid) return;
+    ws->plan = *plan;
+    ws->has_plan = true;
+    Geometry g;
+    if (!select_product(plan->len_a, plan->len_b, &g)) return;
+    std::lock_guard<std::mutex> guard(ws->ex.lock);
+    if (!ws->ex.init()) return;
+    const uint64_t T = (uint64_t)1 << g.log_t;
+    const double *f, *i;
+    uint64_t st;
+    ensure_tables(T / 2, ws->ex.stream, &f, &i, &st);
+    ws->ex.spec_a.ensure((size_t)g.primes * T * 8);
+    ws->ex.spec_b.ensure((size_t)g.primes * T * 8);
+}
+
+static bool ws_run(ntt_mul_workspace* ws, uint64_t* dst, size_t dst_len, const uint64_t* a, size_t a_len,
+                   const uint64_t* b, size_t b_len, bool square) {
+    if (!ws) return false;
+    // MulWorkspace::run, src/plan.rs:182-205: an empty operand writes zero.
+    if (a_len == 0 || b_len == 0) {
+        const uint64_t out_len = (uint64_t)a_len + b_len;
+        if (dst_len < out_len || (out_len && !dst)) return false;
+        if (out_len) memset(dst, 0, out_len * 8);
+        return true;
+    }
+    if (!ws->has_plan || ws->plan.len_a != a_len || ws->plan.len_b != b_len) {
+        ntt_mul_plan p;
+        if (!ntt_mul_plan_new(a_len, b_len, &p)) return false;
+        ws->plan = p;
+        ws->has_plan = true;
+    }
+    return mul_host(ws->ex, &ws->plan, dst, dst_len, a, a_len, b, b_len, square);
+}
+
+bool ntt_mul_workspace_mul(ntt_mul_workspace* ws, uint64_t* dst, size_t dst_len, const uint64_t* a,
+                           size_t a_len, const uint64_t* b, size_t b_len) {
+    const bool square = a == b && a_len == b_len;
+    return ws_run(ws, dst, dst_len, a, a_len, b, b_len, square);
+}
+
+bool ntt_mul_workspace_sqr(ntt_mul_workspace* ws, uint64_t* dst, size_t dst_len, const uint64_t* a,
+                           size_t a_len) {
+    return ws_run(ws, dst, dst_len, a, a_len, a, a_len, true);
+}
+
+bool ntt_conv_plan_new(size_t len_a, size_t len_b, uint64_t max_a, uint64_t max_b, bool is_signed,
+                       ntt_conv_plan* out) {
+    if (!out || len_a == 0 || len_b == 0) return false;
+    if (len_a + len_b < len_a) return false;
+    const Catalog& c = cat();
+    const uint64_t terms = std::min<uint64_t>(len_a, len_b);
+    const uint64_t sign = is_signed ? 1 : 0;
+    uint64_t bits = 0;
+    conv_bound_bits(terms, max_a, max_b, &bits);
+    bits += sign;
+    const uint64_t stride = std::max<uint64_t>((bits + 63) / 64, 1);
+    uint64_t vba = (uint64_t)bitlen(max_a) + sign, vbb = (uint64_t)bitlen(max_b) + sign;
+    if (vba == 0) vba = 1;
+    if (vbb == 0) vbb = 1;
+    if (vba > 64 || vbb > 64) return false;
+    int np = 0;
+    for (int count = 1; count <= kMaxPrimes; ++count)
+        if (c.capacity[count] >= bits) {
+            np = count;
+            break;
+        }
+    if (!np) return false;
+    const uint64_t conv = (uint64_t)len_a + len_b - 1;
+    const int K = std::max(kMinLog, log2_ceil(conv));
+    if (K > kMaxRootOrder - 1) return false;
+    if (conv > UINT64_MAX / stride) return false;
+    ntt_conv_plan p;
+    memset(&p, 0, sizeof p);
+    p.len_a = len_a;
+    p.len_b = len_b;
+    p.max_a = max_a;
+    p.max_b = max_b;
+    p.stride = stride;
+    p.output_len = conv * stride;
+    p.is_signed = is_signed ? 1 : 0;
+    p.schoolbook = terms <= kSchoolbookMaxTerms ? 1 : 0;
+    p.primes = (uint32_t)np;
+    p.log_len = (uint32_t)K;
+    p.bound_bits = (uint32_t)bits;
+    p.value_bits_a = (uint32_t)vba;
+    p.value_bits_b = (uint32_t)vbb;
+    p.valid = 1;
+    *out = p;
+    return true;
+}
+
+size_t ntt_conv_plan_stride(const ntt_conv_plan* plan) { return plan ? plan->stride : 0; }
+size_t ntt_conv_plan_output_len(const ntt_conv_plan* plan) { return plan ? plan->output_len : 0; }
+size_t ntt_conv_plan_scratch_len(const ntt_conv_plan*) { return 0; }
+
+bool ntt_conv_plan_convolve(const ntt_conv_plan* plan, uint64_t* dst, size_t dst_len, const uint64_t* a,
+                            size_t a_len, const uint64_t* b, size_t b_len, uint64_t*, size_t) {
+    return conv_host(*plan_executor(), plan, dst, dst_len, a, a_len, b, b_len);
+}
+
+ntt_conv_workspace* ntt_conv_workspace_new(void) { return new (std::nothrow) ntt_conv_workspace(); }
+
+void ntt_conv_workspace_free(ntt_conv_workspace* ws) { delete ws; }
+
+void ntt_conv_workspace_prepare(ntt_conv_workspace* ws, const ntt_conv_plan* plan) {
+    if (!ws || !plan || !plan->valid) return;
+    ws->plan = *plan;
+    ws->has_plan = true;
+    if (plan->schoolbook) return;
+    std::lock_guard<std::mutex> guard(ws->ex.lock);
+    if (!ws->ex.init()) return;
+    const uint64_t T = (uint64_t)1 << plan->log_len;
+    const double *f, *i;
+    uint64_t st;
+    ensure_tables(T / 2, ws->ex.stream, &f, &i, &st);
+    ws->ex.spec_a.ensure((size_t)plan->primes * T * 8);
+    ws->ex.spec_b.ensure((size_t)plan->primes * T * 8);
+}
+
+bool ntt_conv_workspace_convolve(ntt_conv_workspace* ws, uint64_t* dst, size_t dst_len, const uint64_t* a,
+                                 size_t a_len, const uint64_t* b, size_t b_len) {
+    if (!ws || !ws->has_plan) return false;
+    return conv_host(ws->ex, &ws->plan, dst, dst_len, a, a_len, b, b_len);
+}
+
+// Device-resident operands.
+
+bool ntt_mul_workspace_mul_device(ntt_mul_workspace* ws, uint64_t* d_dst, size_t dst_len, const uint64_t* d_a,
+                                  size_t a_len, const uint64_t* d_b, size_t b_len, void* stream) {
+    t_error.clear();
+    if (!ws || a_len == 0 || b_len == 0 || !d_dst || !d_a || !d_b) return false;
+    const bool square = d_a == d_b && a_len == b_len;
+    if (dst_len < (uint64_t)a_len + b_len) return false;
+    if (!ws->has_plan || ws->plan.len_a != a_len || ws->plan.len_b != b_len) {
+        ntt_mul_plan p;
+        if (!ntt_mul_plan_new(a_len, b_len, &p)) return false;
+        ws->plan = p;
+        ws->has_plan = true;
+    }
+    Geometry g;
+    if (!product_geometry(&ws->plan, &g)) return false;
+    std::lock_guard<std::mutex> guard(ws->ex.lock);
+    if (!ws->ex.init()) return false;
+    hipStream_t saved = ws->ex.stream;
+    if (stream) ws->ex.stream = (hipStream_t)stream;
+    const bool ok = device_product(ws->ex, g, d_a, a_len, d_b, b_len, square, d_dst);
+    ws->ex.stream = saved;
+    return ok;
+}
+
+bool ntt_mul_workspace_sqr_device(ntt_mul_workspace* ws, uint64_t* d_dst, size_t dst_len, const uint64_t* d_a,
+                                  size_t a_len, void* stream) {
+    return ntt_mul_workspace_mul_device(ws, d_dst, dst_len, d_a, a_len, d_a, a_len, stream);
+}
+
+bool ntt_conv_workspace_convolve_device(ntt_conv_workspace* ws, uint64_t* d_dst, size_t dst_len,
+                                        const uint64_t* d_a, size_t a_len, const uint64_t* d_b, size_t b_len,
+                                        void* stream) {
+    t_error.clear();
+    if (!ws || !ws->has_plan || !d_dst || !d_a || !d_b) return false;
+    const ntt_conv_plan* p = &ws->plan;
+    if (a_len != p->len_a || b_len != p->len_b || dst_len < p->output_len) return false;
+    Geometry g;
+    if (!conv_geometry(p, &g)) return false;
+    std::lock_guard<std::mutex> guard(ws->ex.lock);
+    if (!ws->ex.init()) return false;
+    hipStream_t saved = ws->ex.stream;
+    if (stream) ws->ex.stream = (hipStream_t)stream;
+    ConvCtx ctx{&g, p, d_a == d_b && a_len == b_len && !p->schoolbook};
+    const bool ok = run_conv(ws->ex, &ctx, d_a, d_b, d_dst);
+    ws->ex.stream = saved;
+    return ok;
+}
+
+// Utilities.
+
+int ntt_device_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+    return strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+void* ntt_device_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+    return p;
+}
+
+void ntt_device_free(void* ptr) {
+    if (ptr) hipFree(ptr);
+}
+
+bool ntt_copy_to_device(void* d_dst, const void* src, size_t bytes) {
+    return hipMemcpy(d_dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+}
+
+bool ntt_copy_to_host(void* dst, const void* d_src, size_t bytes) {
+    return hipMemcpy(dst, d_src, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+}
+
+bool ntt_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess; }
+
+const char* ntt_last_error(void) { return t_error.c_str(); }
+
+void ntt_profile_enable(int on) { g_prof_on.store(on ? 1 : 0); }
+
+void ntt_profile_reset(void) {
+    prof_drain();
+    std::lock_guard<std::mutex> g(g_prof_lock);
+    g_prof_totals.clear();
+}
+
+int ntt_profile_read(char* names, size_t names_len, double* ms, uint64_t* counts, int cap) {
+    prof_drain();
+    std::lock_guard<std::mutex> g(g_prof_lock);
+    int n = 0;
+    size_t off = 0;
+    for (auto& t : g_prof_totals) {
+        if (n >= cap) break;
+        const size_t len = t.first.size() + 1;
+        if (names && off + len <= names_len) {
+            memcpy(names + off, t.first.c_str(), len);
+            off += len;
+        }
+        if (ms) ms[n] = t.second.first;
+        if (counts) counts[n] = t.second.second;
+        ++n;
+    }
+    return n;
+}
+
+bool ntt_mul_geometry(size_t len_a, size_t len_b, uint32_t* primes, uint32_t* chunk_bits, uint32_t* log_len) {
+    Geometry g;
+    if (!select_product(len_a, len_b, &g)) return false;
+    if (primes) *primes = g.primes;
+    if (chunk_bits) *chunk_bits = g.width;
+    if (log_len) *log_len = (uint32_t)g.log_t;
+    return true;
+}
+
+}  // extern "C"

@@ -1,0 +1,81 @@
+"""The C ABI library loads, exports every entry point of include/nttanafis.h,
+and its host-only planning matches the reference's admission rules. No test
+here launches device work."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+import nttanafis_amd as ntt
+from nttanafis_amd import _ffi
+
+HEADER = os.path.join(ROOT, "include", "nttanafis.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ntt_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(_ffi.LIB_PATH), "build the library first (nttanafis_amd.build)"
+    out = subprocess.run(["nm", "-D", "--defined-only", _ffi.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\b(ntt_[a-z0-9_]+)\b", out))
+    missing = [s for s in declared_symbols() if s not in exported]
+    assert not missing, missing
+    assert set(declared_symbols()) == set(_ffi.SIGNATURES)
+
+
+def test_library_loads_and_binds():
+    lib = ntt.load()
+    for name in _ffi.SIGNATURES:
+        assert getattr(lib, name)
+
+
+def test_mul_plan_admission():
+    assert ntt.MulPlan.new(0, 5) is None
+    assert ntt.MulPlan.new(5, 0) is None
+    plan = ntt.MulPlan.new(1024, 1024)
+    assert plan.len_a == 1024 and plan.len_b == 1024
+    assert plan.scratch_len() == 0 and plan.sqr_scratch_len() == 0
+    assert ntt.MulPlan.new(1, 1) is not None
+    assert ntt.MulPlan.new(1 << 24, 1 << 24) is not None
+
+
+def test_conv_plan_admission(kat):
+    for terms, ma, mb, stride in kat["stride_table"]:
+        plan = ntt.ConvPlan.new(terms, terms, ma, mb, False)
+        assert plan.stride() == stride
+        assert plan.output_len() == (2 * terms - 1) * stride
+    assert ntt.ConvPlan.new(3, 3, 1 << 63, 1, True) is None
+    assert ntt.ConvPlan.new(0, 3, 1, 1, False) is None
+    plan = ntt.ConvPlan.new(1 << 22, 1 << 22, 1 << 31, 1 << 31, True)
+    assert plan.stride() == 2 and plan.primes == 2 and plan.log_len == 23
+    plan = ntt.ConvPlan.new(1 << 22, 1 << 22, 65535, 65535, False)
+    assert plan.stride() == 1 and plan.primes == 2 and plan.log_len == 23
+    assert ntt.ConvPlan.new(24, 1000, 5, 5, False).schoolbook
+    assert not ntt.ConvPlan.new(25, 1000, 5, 5, False).schoolbook
+    assert ntt.ConvPlan.new(25, 1000, 5, 5, False).scratch_len() == 0
+
+
+@pytest.mark.parametrize("la,lb", [(1, 1), (3, 700), (192, 65536), (1024, 1024), (1 << 20, 1 << 20),
+                                   (1 << 24, 1 << 24), (4194304, 256), (65535, 65537)])
+def test_device_geometry_fits_capacity(kat, la, lb):
+    """Every coefficient of the engine's geometry stays below the CRT capacity:
+    2w + ceil(log2(terms)) <= capacity (src/profile/planner.rs:558-563)."""
+    primes, w, log_len = ntt.mul_geometry(la, lb)
+    da, db = -(-64 * la // w), -(-64 * lb // w)
+    terms = min(da, db)
+    assert 2 * w + (terms - 1).bit_length() <= kat["capacities"][primes - 1]
+    assert (1 << log_len) >= da + db - 1
+
+
+def test_headline_geometry():
+    # 2^20 x 2^20 limbs: the engine's geometry holds the whole product in a
+    # power-of-two transform of at most 2^21 points.
+    primes, w, log_len = ntt.mul_geometry(1 << 20, 1 << 20)
+    assert primes * (1 << log_len) <= 6 * (1 << 20)
