@@ -193,6 +193,27 @@ void ntt_profile_reset(void);
 bool ntt_mul_geometry(size_t len_a, size_t len_b, uint32_t *primes, uint32_t *chunk_bits,
                       uint32_t *log_len);
 
+/* Diagnostics (no counterpart in the reference; used by tests). */
+
+/* Checks the arithmetic the kernels share with the host (residue products,
+ * packing, Garner recombination, carry functions) against exact integer
+ * arithmetic on the host. Returns the number of failures; needs no GPU. */
+int ntt_selftest_host(uint64_t seed);
+
+/* Copies the device root tables of all eight primes at `len` entries each. */
+bool ntt_debug_tables(uint64_t len, double *fwd, double *inv);
+
+/* Runs the transforms of a product and copies the canonical residue rows
+ * (primes x transform length) to `out`; `geometry` receives primes, chunk
+ * bits and log2 of the transform length. With out == NULL only the geometry
+ * is reported. */
+bool ntt_debug_residues(const uint64_t *a, size_t la, const uint64_t *b, size_t lb, int square,
+                        double *out, size_t out_len, uint32_t *geometry);
+
+/* Runs the CRT and carry kernels of a product shape on host-given residue
+ * rows and copies the la+lb result limbs to `out`. */
+bool ntt_debug_crt(size_t la, size_t lb, const double *rows, uint64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -104,6 +104,13 @@ SIGNATURES = {
         ctypes.c_bool,
         [size_t, size_t, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)],
     ),
+    "ntt_selftest_host": (ctypes.c_int, [u64]),
+    "ntt_debug_tables": (ctypes.c_bool, [u64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "ntt_debug_residues": (
+        ctypes.c_bool,
+        [P64, size_t, P64, size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double), size_t, ctypes.POINTER(u32)],
+    ),
+    "ntt_debug_crt": (ctypes.c_bool, [size_t, size_t, ctypes.POINTER(ctypes.c_double), P64]),
 }
 
 _lib = None

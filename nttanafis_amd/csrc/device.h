@@ -8,6 +8,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
 
 namespace nttamd {
@@ -27,20 +28,21 @@ struct Field {
 
 // Exact residue product: |a*b| < 2p^2 gives |r| < 15p/16, |a*b| < 4p^2 gives
 // |r| < 11p/8 for every catalog prime.
-__device__ __forceinline__ double mulmod(double a, double b, double p, double pinv) {
+// `low` is the exact rounding error a*b - high, so a*b - q*p = (high - q*p) + low.
+__host__ __device__ __forceinline__ double mulmod(double a, double b, double p, double pinv) {
     const double high = a * b;
     const double low = fma(a, b, -high);
     const double q = rint(high * pinv);
-    return fma(-q, p, high) - low;
+    return fma(-q, p, high) + low;
 }
 
 // Centered reduction of |x| < 8p to |r| < 9p/16 (|r| <= p/2 for |x| < p).
-__device__ __forceinline__ double reduce(double x, double p, double pinv) {
+__host__ __device__ __forceinline__ double reduce(double x, double p, double pinv) {
     return fma(-rint(x * pinv), p, x);
 }
 
 // Canonical residue in [0, p) of |x| < p.
-__device__ __forceinline__ double canonical(double x, double p) { return x < 0.0 ? x + p : x; }
+__host__ __device__ __forceinline__ double canonical(double x, double p) { return x < 0.0 ? x + p : x; }
 
 // Operand description for packing coefficients straight from limbs.
 struct Source {

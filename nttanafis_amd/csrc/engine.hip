@@ -1,7 +1,7 @@
 // Host side of the gfx950 multi-prime NTT engine: catalog constants, device
 // geometry, device context, the product/convolution pipelines and the C ABI
 // declared in include/nttanafis.h.
-#include "device.h"
+#include "arith.h"
 
 #include <hip/hip_runtime.h>
 
@@ -746,6 +746,84 @@ static bool conv_host(Executor& ex, const ntt_conv_plan* plan, uint64_t* dst, si
 
 }  // namespace nttamd
 
+namespace nttamd {
+// Host self-test of the arithmetic the kernels share with the host
+// (mulmod, reduce, packing, Garner, carry functions) against exact integer
+// arithmetic. Returns the number of failed checks.
+
+namespace {
+struct Rng {
+    uint64_t s;
+    uint64_t next() {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    int64_t range(int64_t lo, int64_t hi) {  // [lo, hi)
+        return lo + (int64_t)(next() % (uint64_t)(hi - lo));
+    }
+};
+
+typedef __int128 i128;
+
+bool check_mod(double r, i128 expect, uint64_t p) {
+    if (r != (double)(int64_t)r) return false;
+    i128 d = ((i128)(int64_t)r - expect) % (i128)p;
+    return d == 0;
+}
+
+template <int NP>
+int garner_case(const Catalog& c, Rng& rng, bool balanced) {
+    constexpr int NW = CrtWords<NP>::value;
+    const uint32_t cap = c.capacity[NP];
+    uint64_t X[kMaxWords + 1] = {0};
+    // Random value below 2^(cap - balanced), then negated half the time when balanced.
+    const uint32_t bits = cap - (balanced ? 1 : 0);
+    for (uint32_t w = 0; w * 64 < bits; ++w) X[w] = rng.next();
+    const uint32_t top = bits % 64;
+    if (top) X[(bits - 1) / 64] &= (1ull << top) - 1;
+    for (int w = (bits + 63) / 64; w < NW; ++w) X[w] = 0;
+    bool neg = balanced && (rng.next() & 1);
+    if (neg) {  // two's complement over NW words
+        uint64_t carry = 1;
+        for (int w = 0; w < NW; ++w) {
+            X[w] = ~X[w] + carry;
+            carry = carry && X[w] == 0;
+        }
+    }
+    double residues[NP];
+    for (int k = 0; k < NP; ++k) {
+        // X mod p_k from the words, most significant first.
+        const uint64_t pk = c.p[k];
+        unsigned __int128 r = 0;
+        for (int w = NW - 1; w >= 0; --w) r = ((r << 64) | X[w]) % pk;
+        // Signed X: subtract 2^(64 NW) mod p_k.
+        if (neg) {
+            unsigned __int128 m = 1;
+            for (int w = 0; w < NW; ++w) m = (m << 64) % pk;
+            r = (r + pk - (uint64_t)m) % pk;
+        }
+        residues[k] = (double)(uint64_t)r;
+    }
+    CrtArgs a;
+    memset(&a, 0, sizeof a);
+    a.residues = residues;
+    a.row_stride = 1;
+    a.field = c.field;
+    memcpy(a.pmod, c.pmod, sizeof a.pmod);
+    memcpy(a.ginv, c.ginv, sizeof a.ginv);
+    memcpy(a.pint, c.p, sizeof a.pint);
+    uint64_t got[NW];
+    garner<NP>(a, 0, got, balanced);
+    int fails = 0;
+    for (int w = 0; w < NW; ++w) fails += got[w] != X[w];
+    return fails ? 1 : 0;
+}
+}  // namespace
+}  // namespace nttamd
+
 // ---------------------------------------------------------------------------
 // C ABI.
 
@@ -1033,6 +1111,172 @@ int ntt_profile_read(char* names, size_t names_len, double* ms, uint64_t* counts
         ++n;
     }
     return n;
+}
+
+
+// Diagnostics: intermediate stages of the pipeline, for tests.
+
+int ntt_selftest_host(uint64_t seed) {
+    const Catalog& c = cat();
+    Rng rng{seed};
+    int fails = 0;
+    for (int y = 0; y < kMaxPrimes; ++y) {
+        const uint64_t p = c.p[y];
+        const double pf = c.field.p[y], pinv = c.field.pinv[y];
+        const int64_t P = (int64_t)p;
+        for (int i = 0; i < 4000; ++i) {
+            // mulmod: |a| < 8p against a centered factor, and both operands below 2p.
+            const int64_t a = i & 1 ? rng.range(-8 * P + 1, 8 * P) : rng.range(-2 * P + 1, 2 * P);
+            const int64_t b = i & 1 ? rng.range(-P / 2, P / 2 + 1) : rng.range(-2 * P + 1, 2 * P);
+            const i128 prod = (i128)a * b;
+            const i128 lim = (i128)4 * P * P;
+            if (prod >= lim || -prod >= lim) continue;
+            const double r = mulmod((double)a, (double)b, pf, pinv);
+            const bool small = prod < (i128)2 * P * P && -prod < (i128)2 * P * P;
+            const double bound = small ? 15.0 * pf / 16.0 : 11.0 * pf / 8.0;
+            if (!check_mod(r, prod, p) || fabs(r) >= bound) ++fails;
+            const int64_t x = rng.range(-8 * P + 1, 8 * P);
+            const double q = reduce((double)x, pf, pinv);
+            if (!check_mod(q, x, p) || fabs(q) >= 9.0 * pf / 16.0) ++fails;
+        }
+        // Packing of every chunk width shape against exact digits.
+        static const uint32_t widths[] = {22, 49, 50, 64, 69, 98, 99, 139, 143, 147, 196};
+        uint64_t limbs[24];
+        for (uint32_t width : widths) {
+            for (int t = 0; t < 3; ++t) {
+                for (auto& l : limbs) l = t == 2 ? ~0ull : rng.next();
+                if (t == 1) limbs[23] = 0;
+                Source s;
+                s.limbs = limbs;
+                s.limb_len = 24;
+                s.digits = (24 * 64 + width - 1) / width;
+                s.width = width;
+                s.value_bits = width;
+                s.is_signed = 0;
+                s.pad = 0;
+                for (uint64_t n = 0; n < s.digits + 2; ++n) {
+                    // Exact digit n mod p from the bits.
+                    unsigned __int128 r = 0;
+                    for (int bit = (int)width - 1; bit >= 0; --bit) {
+                        const uint64_t pos = n * width + bit;
+                        const uint64_t v = pos / 64 < 24 ? (limbs[pos / 64] >> (pos % 64)) & 1 : 0;
+                        r = ((r << 1) | v) % p;
+                    }
+                    if (n >= s.digits) r = 0;
+                    const double got = pack_coef(s, n, c.powers[y], pf, pinv);
+                    if (!check_mod(got, (i128)r, p) || fabs(got) >= pf) ++fails;
+                }
+            }
+        }
+        // Signed whole-limb coefficients of 1..64 bits with their sign.
+        for (uint32_t vb : {1u, 2u, 16u, 33u, 49u, 50u, 63u, 64u}) {
+            for (int t = 0; t < 24; ++t) {
+                const int64_t lim = vb >= 64 ? INT64_MAX : (int64_t)1 << (vb - 1);
+                int64_t v = (int64_t)rng.next();
+                if (vb < 64) v = rng.range(-lim, lim);
+                limbs[t] = (uint64_t)v;
+            }
+            Source s;
+            s.limbs = limbs;
+            s.limb_len = 24;
+            s.digits = 24;
+            s.width = 64;
+            s.value_bits = vb;
+            s.is_signed = 1;
+            s.pad = 0;
+            for (uint64_t n = 0; n < 24; ++n) {
+                const double got = pack_coef(s, n, c.powers[y], pf, pinv);
+                if (!check_mod(got, (i128)(int64_t)limbs[n], p) || fabs(got) >= pf) ++fails;
+            }
+        }
+    }
+    for (int t = 0; t < 200; ++t) {
+        fails += garner_case<1>(c, rng, false) + garner_case<2>(c, rng, false) + garner_case<3>(c, rng, false) +
+                 garner_case<4>(c, rng, false) + garner_case<5>(c, rng, false) + garner_case<6>(c, rng, false) +
+                 garner_case<7>(c, rng, false) + garner_case<8>(c, rng, false);
+        fails += garner_case<1>(c, rng, true) + garner_case<2>(c, rng, true) + garner_case<3>(c, rng, true) +
+                 garner_case<4>(c, rng, true);
+    }
+    // Carry functions: compose(f1, f2) is f2 after f1 for every small carry.
+    for (int t = 0; t < 20000; ++t) {
+        CarryFn f1{(uint32_t)(rng.next() % 5), (uint32_t)(rng.next() % 7 + 1)};
+        CarryFn f2{(uint32_t)(rng.next() % 5), (uint32_t)(rng.next() % 7 + 1)};
+        if (rng.next() % 4 == 0) f1.thresh = kInf;
+        if (rng.next() % 4 == 0) f2.thresh = kInf;
+        const CarryFn g = compose(f1, f2);
+        for (uint32_t cin = 0; cin < 8; ++cin) {
+            const uint32_t mid = f1.base + (cin >= f1.thresh ? 1u : 0u);
+            const uint32_t out = f2.base + (mid >= f2.thresh ? 1u : 0u);
+            if (out != g.base + (cin >= g.thresh ? 1u : 0u)) ++fails;
+        }
+    }
+    return fails;
+}
+
+bool ntt_debug_tables(uint64_t len, double* fwd, double* inv) {
+    Executor* ex = plan_executor();
+    std::lock_guard<std::mutex> guard(ex->lock);
+    if (!ex->init()) return false;
+    const double *f = nullptr, *i = nullptr;
+    uint64_t stride = 0;
+    if (!ensure_tables(len, ex->stream, &f, &i, &stride)) return false;
+    for (int y = 0; y < kMaxPrimes; ++y) {
+        if (hipMemcpy(fwd + (uint64_t)y * len, f + (uint64_t)y * stride, len * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            return false;
+        if (hipMemcpy(inv + (uint64_t)y * len, i + (uint64_t)y * stride, len * 8, hipMemcpyDeviceToHost) != hipSuccess)
+            return false;
+    }
+    return true;
+}
+
+bool ntt_debug_residues(const uint64_t* a, size_t la, const uint64_t* b, size_t lb, int square, double* out,
+                        size_t out_len, uint32_t* geometry) {
+    Geometry g;
+    if (!select_product(la, lb, &g)) return false;
+    const uint64_t T = (uint64_t)1 << g.log_t;
+    if (geometry) {
+        geometry[0] = g.primes;
+        geometry[1] = g.width;
+        geometry[2] = (uint32_t)g.log_t;
+    }
+    if (!out) return true;
+    if (out_len < g.primes * T) return false;
+    Executor* ex = plan_executor();
+    std::lock_guard<std::mutex> guard(ex->lock);
+    if (!ex->init() || !ex->in_a.ensure(la * 8) || !ex->in_b.ensure(lb * 8)) return false;
+    if (hipMemcpy(ex->in_a.ptr, a, la * 8, hipMemcpyHostToDevice) != hipSuccess) return false;
+    if (hipMemcpy(ex->in_b.ptr, b, lb * 8, hipMemcpyHostToDevice) != hipSuccess) return false;
+    double* rows = nullptr;
+    if (!residues(*ex, g, ex->in_a.as<uint64_t>(), la, ex->in_b.as<uint64_t>(), lb, square != 0, &rows))
+        return false;
+    if (hipStreamSynchronize(ex->stream) != hipSuccess) return false;
+    return hipMemcpy(out, rows, g.primes * T * 8, hipMemcpyDeviceToHost) == hipSuccess;
+}
+
+bool ntt_debug_crt(size_t la, size_t lb, const double* rows, uint64_t* out) {
+    Geometry g;
+    if (!select_product(la, lb, &g)) return false;
+    const uint64_t T = (uint64_t)1 << g.log_t;
+    Executor* ex = plan_executor();
+    std::lock_guard<std::mutex> guard(ex->lock);
+    const uint64_t out_len = la + lb;
+    if (!ex->init() || !ex->spec_b.ensure(g.primes * T * 8) || !ex->out.ensure(out_len * 8)) return false;
+    if (hipMemcpy(ex->spec_b.ptr, rows, g.primes * T * 8, hipMemcpyHostToDevice) != hipSuccess) return false;
+    CrtArgs c = crt_args(g, ex->spec_b.as<double>(), ex->out.as<uint64_t>(), out_len);
+    c.width = g.width;
+    c.coefficients = std::min<uint64_t>(g.conv_len, (out_len * 64 + g.width - 1) / g.width);
+    const uint64_t blocks = (out_len + kCrtBlockLimbs - 1) / kCrtBlockLimbs;
+    c.blocks = (uint32_t)blocks;
+    if (!ex->carry.ensure(blocks * 5 * sizeof(uint32_t))) return false;
+    uint32_t* cb = ex->carry.as<uint32_t>();
+    c.agg_base = cb;
+    c.agg_thresh = cb + blocks;
+    c.first_thresh = cb + 2 * blocks;
+    c.ripple_end = cb + 3 * blocks;
+    c.carry_in = cb + 4 * blocks;
+    if (!launch_crt_product(c, ex->stream)) return false;
+    if (hipStreamSynchronize(ex->stream) != hipSuccess) return false;
+    return hipMemcpy(out, ex->out.ptr, out_len * 8, hipMemcpyDeviceToHost) == hipSuccess;
 }
 
 bool ntt_mul_geometry(size_t len_a, size_t len_b, uint32_t* primes, uint32_t* chunk_bits, uint32_t* log_len) {

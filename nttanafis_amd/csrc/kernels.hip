@@ -15,7 +15,7 @@
 // undoes it without any bit reversal. Root of node H*2^l + x of a row whose
 // first node is H is W[x] * W[H*2^l]; rows build their roots in LDS that way
 // instead of reading a full-length table.
-#include "device.h"
+#include "arith.h"
 
 namespace nttamd {
 
@@ -159,48 +159,6 @@ __device__ __forceinline__ void inv_rounds(double* lds, Tw tw, int gbase, int lo
         __syncthreads();
         inv_rounds<LOGL, LINES, NT, R - 1>(lds, tw, gbase, log_t, p, pinv);
     }
-}
-
-// ---------------------------------------------------------------------------
-// Packing (limb split): coefficient n is bits [n*width, n*width + value_bits)
-// of the operand, split into 49-bit parts; its residue is
-// part_0 + sum_k part_k * (2^(49k) mod p), reduced once. Restates the
-// reference's Packing::pack (src/limb/packing.rs:113-292,
-// src/arch/schedule/expression/packing.rs:68-109).
-
-__device__ __forceinline__ uint64_t window64(const uint64_t* limbs, uint64_t len, uint64_t pos) {
-    const uint64_t idx = pos >> 6;
-    const uint32_t off = (uint32_t)(pos & 63);
-    uint64_t v = idx < len ? limbs[idx] >> off : 0;
-    if (off > 64 - kPartBits && idx + 1 < len) v |= limbs[idx + 1] << (64 - off);
-    return v;
-}
-
-__device__ __forceinline__ double pack_coef(const Source& s, uint64_t n, const double* pw, double p,
-                                            double pinv) {
-    if (n >= s.digits) return 0.0;
-    const uint32_t vb = s.value_bits;
-    const int parts = (int)((vb + kPartBits - 1) / kPartBits);
-    const int high_bits = (int)vb - kPartBits * (parts - 1);
-    const uint64_t bitpos = n * (uint64_t)s.width;
-    double sum = 0.0;
-    for (int k = 0; k < parts; ++k) {
-        const uint64_t word = window64(s.limbs, s.limb_len, bitpos + (uint64_t)(kPartBits * k));
-        double d;
-        if (k + 1 == parts) {
-            const uint64_t masked = word & ((1ull << high_bits) - 1);
-            if (s.is_signed) {
-                const uint64_t flip = 1ull << (high_bits - 1);
-                d = (double)((int64_t)(masked ^ flip) - (int64_t)flip);
-            } else {
-                d = (double)masked;
-            }
-        } else {
-            d = (double)(word & ((1ull << kPartBits) - 1));
-        }
-        sum = k == 0 ? d : sum + mulmod(d, pw[k], p, pinv);
-    }
-    return parts > 1 ? reduce(sum, p, pinv) : sum;
 }
 
 // ---------------------------------------------------------------------------
@@ -429,71 +387,6 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     } else {
         for (int m = threadIdx.x; m < L; m += NT) row[m] = lds[pidx(m)];
     }
-}
-
-// ---------------------------------------------------------------------------
-// CRT: Garner's mixed-radix digits in FP64, then the integer value
-// X = y0 + p0*(y1 + p1*(y2 + ...)) in 64-bit words. The reference recombines
-// with cofactors and one approximate quotient (src/limb/cofactor.rs:59-111);
-// both return the same integer below the prefix capacity.
-
-template <int NP>
-struct CrtWords {
-    static constexpr int value = NP == 1 ? 1 : NP == 2 ? 2 : NP == 3 ? 3 : NP <= 5 ? 4 : NP == 6 ? 5 : NP == 7 ? 6 : 7;
-};
-
-template <int NP>
-__device__ __forceinline__ void garner(const CrtArgs& a, uint64_t i, uint64_t* X, bool balanced) {
-    constexpr int NW = CrtWords<NP>::value;
-    double r[NP], y[NP];
-#pragma unroll
-    for (int k = 0; k < NP; ++k) r[k] = a.residues[(uint64_t)k * a.row_stride + i];
-    y[0] = r[0];
-#pragma unroll
-    for (int k = 1; k < NP; ++k) {
-        const double pk = a.field.p[k], pinvk = a.field.pinv[k];
-        double u = y[k - 1];
-#pragma unroll
-        for (int j = k - 2; j >= 0; --j) u = reduce(y[j] + mulmod(u, a.pmod[j][k], pk, pinvk), pk, pinvk);
-        y[k] = canonical(mulmod(r[k] - u, a.ginv[k], pk, pinvk), pk);
-    }
-    int64_t top = (int64_t)y[NP - 1];
-    if (balanced && 2 * (uint64_t)top > a.pint[NP - 1]) top -= (int64_t)a.pint[NP - 1];
-    X[0] = (uint64_t)top;
-#pragma unroll
-    for (int w = 1; w < NW; ++w) X[w] = top < 0 ? ~0ull : 0ull;
-#pragma unroll
-    for (int k = NP - 2; k >= 0; --k) {
-        const uint64_t pk = a.pint[k];
-        uint64_t carry = (uint64_t)y[k];
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            uint64_t lo = X[w] * pk;
-            uint64_t hi = __umul64hi(X[w], pk);
-            lo += carry;
-            hi += lo < carry;
-            X[w] = lo;
-            carry = hi;
-        }
-    }
-}
-
-// Carry function of a limb range: carry_out(c) = base + (c >= thresh).
-struct CarryFn {
-    uint32_t base, thresh;
-};
-
-// f1 runs first, then f2.
-__device__ __forceinline__ CarryFn compose(CarryFn f1, CarryFn f2) {
-    CarryFn r;
-    r.base = f2.base + (f1.base >= f2.thresh ? 1u : 0u);
-    r.thresh = (f1.base + 1u == f2.thresh) ? f1.thresh : kInf;
-    return r;
-}
-
-__device__ __forceinline__ uint32_t limb_thresh(uint64_t v) {
-    const uint64_t t = 0 - v;  // 2^64 - v, zero for v == 0
-    return (v != 0 && t < (uint64_t)kInf) ? (uint32_t)t : kInf;
 }
 
 // Product reconstruction, one block of kCrtBlockLimbs output limbs: every

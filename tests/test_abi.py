@@ -79,3 +79,10 @@ def test_headline_geometry():
     # power-of-two transform of at most 2^21 points.
     primes, w, log_len = ntt.mul_geometry(1 << 20, 1 << 20)
     assert primes * (1 << log_len) <= 6 * (1 << 20)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_host_selftest_of_shared_arithmetic(seed):
+    """Residue products, packing, Garner and carry functions, compiled for
+    the host from the same source as the kernels, against exact integers."""
+    assert ntt.load().ntt_selftest_host(seed) == 0
