@@ -29,24 +29,31 @@ struct Lay {
     static constexpr int FIRST = LOGL - 4 * (ROUNDS - 1);  // levels of round 0
 };
 
-// Twiddle view: roots of node `node` at local level `lvl` of line `c`.
+// Roots of the tree are read from the root table of one prime in global
+// memory (L1/L2-resident: a row pass reads each root a handful of times).
+// Line c of a tile starts at node h0 + c*hstep of the pass's first level, so
+// its node x at local level l is table[(h0 + c*hstep) << l | x].
 struct Tw {
-    const double* base;
-    int line_stride;  // 0 when lines share roots
-    bool heap;        // heap index (2^lvl + node) or prefix index (node)
+    const double* table;
+    uint64_t h0;
+    int hstep;  // 1 when every line is its own node (rows), 0 when lines share one
     __device__ __forceinline__ double at(int c, int lvl, int node) const {
-        return base[c * line_stride + (heap ? (1 << lvl) + node : node)];
+        return table[((h0 + (uint64_t)(c * hstep)) << lvl) + node];
     }
 };
 
 // One DIF round: local levels [L0, L0 + RLOG) of every line of the tile.
-// Reduction schedule (global level g): the upper input is reduced when
-// g % 3 == 2, which keeps every level input below 27p/8 and every product
-// below 2p^2; the last level of the transform reduces both outputs, so
-// spectra lie below 9p/16.
+//
+// Reduction schedule, fixed at compile time: the upper input of a butterfly
+// is reduced at the odd levels of a round. With |root| <= p/2, a level maps
+// inputs below B to outputs below B + 15p/16 (the product stays below 2p^2),
+// and a reducing level maps inputs below 27p/8 to outputs below
+// 9p/16 + 15p/16 = 3p/2. Every round therefore ends below 3p/2 (its last
+// level is odd) or, for a short first round ending on an even level, below
+// 39p/16; values never exceed 27p/8, and spectra stay below 3p/2, so a
+// pointwise product stays below 9p^2/4.
 template <int LOGL, int LINES, int NT, int L0, int RLOG>
-__device__ __forceinline__ void fwd_round(double* lds, Tw tw, int gbase, int last_level, double p,
-                                          double pinv) {
+__device__ __forceinline__ void fwd_round(double* lds, Tw tw, double p, double pinv) {
     constexpr int L = 1 << LOGL;
     constexpr int RADIX = 1 << RLOG;
     constexpr int D = L >> (L0 + RLOG);
@@ -65,9 +72,6 @@ __device__ __forceinline__ void fwd_round(double* lds, Tw tw, int gbase, int las
         for (int k = 0; k < RADIX; ++k) x[k] = line[pidx(base + k * D)];
 #pragma unroll
         for (int s = 0; s < RLOG; ++s) {
-            const int lvl = gbase + L0 + s;
-            const bool red = (lvl % 3) == 2;
-            const bool last = lvl == last_level;
             const int half = RADIX >> (s + 1);
 #pragma unroll
             for (int blk = 0; blk < (1 << s); ++blk) {
@@ -75,16 +79,10 @@ __device__ __forceinline__ void fwd_round(double* lds, Tw tw, int gbase, int las
 #pragma unroll
                 for (int j = 0; j < half; ++j) {
                     const int k = blk * 2 * half + j;
-                    double u = x[k];
+                    const double u = (s & 1) ? reduce(x[k], p, pinv) : x[k];
                     const double t = mulmod(x[k + half], w, p, pinv);
-                    if (last) {
-                        x[k] = reduce(u + t, p, pinv);
-                        x[k + half] = reduce(u - t, p, pinv);
-                    } else {
-                        if (red) u = reduce(u, p, pinv);
-                        x[k] = u + t;
-                        x[k + half] = u - t;
-                    }
+                    x[k] = u + t;
+                    x[k + half] = u - t;
                 }
             }
         }
@@ -93,11 +91,14 @@ __device__ __forceinline__ void fwd_round(double* lds, Tw tw, int gbase, int las
     }
 }
 
-// One inverse (DIT) round, levels in decreasing order. The sum is reduced when
-// (K-1-g) is odd, which keeps inputs below 15p/8 and differences below 15p/4.
+// One inverse (DIT) round, levels in decreasing order; the sum is reduced at
+// the even levels of a round. Inputs below 11p/8 give differences below
+// 11p/4 (products below 2p^2, outputs below 15p/16) and sums below 11p/4 at
+// an odd level; an even level then sees inputs below 11p/4, reduces sums
+// below 11p/2 to 9p/16 and keeps differences times a root below 4p^2
+// (outputs below 11p/8). Every round ends on level 0 below 11p/8.
 template <int LOGL, int LINES, int NT, int L0, int RLOG>
-__device__ __forceinline__ void inv_round(double* lds, Tw tw, int gbase, int log_t, double p,
-                                          double pinv) {
+__device__ __forceinline__ void inv_round(double* lds, Tw tw, double p, double pinv) {
     constexpr int L = 1 << LOGL;
     constexpr int RADIX = 1 << RLOG;
     constexpr int D = L >> (L0 + RLOG);
@@ -116,8 +117,6 @@ __device__ __forceinline__ void inv_round(double* lds, Tw tw, int gbase, int log
         for (int k = 0; k < RADIX; ++k) x[k] = line[pidx(base + k * D)];
 #pragma unroll
         for (int s = RLOG - 1; s >= 0; --s) {
-            const int lvl = gbase + L0 + s;
-            const bool red = ((log_t - 1 - lvl) & 1) == 1;
             const int half = RADIX >> (s + 1);
 #pragma unroll
             for (int blk = 0; blk < (1 << s); ++blk) {
@@ -127,7 +126,7 @@ __device__ __forceinline__ void inv_round(double* lds, Tw tw, int gbase, int log
                     const int k = blk * 2 * half + j;
                     const double u = x[k], v = x[k + half];
                     const double sum = u + v;
-                    x[k] = red ? reduce(sum, p, pinv) : sum;
+                    x[k] = (s & 1) ? sum : reduce(sum, p, pinv);
                     x[k + half] = mulmod(u - v, w, p, pinv);
                 }
             }
@@ -138,26 +137,24 @@ __device__ __forceinline__ void inv_round(double* lds, Tw tw, int gbase, int log
 }
 
 template <int LOGL, int LINES, int NT, int R>
-__device__ __forceinline__ void fwd_rounds(double* lds, Tw tw, int gbase, int last_level, double p,
-                                           double pinv) {
+__device__ __forceinline__ void fwd_rounds(double* lds, Tw tw, double p, double pinv) {
     if constexpr (R < Lay<LOGL>::ROUNDS) {
         constexpr int L0 = R == 0 ? 0 : Lay<LOGL>::FIRST + 4 * (R - 1);
         constexpr int RL = R == 0 ? Lay<LOGL>::FIRST : 4;
-        fwd_round<LOGL, LINES, NT, L0, RL>(lds, tw, gbase, last_level, p, pinv);
+        fwd_round<LOGL, LINES, NT, L0, RL>(lds, tw, p, pinv);
         __syncthreads();
-        fwd_rounds<LOGL, LINES, NT, R + 1>(lds, tw, gbase, last_level, p, pinv);
+        fwd_rounds<LOGL, LINES, NT, R + 1>(lds, tw, p, pinv);
     }
 }
 
 template <int LOGL, int LINES, int NT, int R>
-__device__ __forceinline__ void inv_rounds(double* lds, Tw tw, int gbase, int log_t, double p,
-                                           double pinv) {
+__device__ __forceinline__ void inv_rounds(double* lds, Tw tw, double p, double pinv) {
     if constexpr (R >= 0) {
         constexpr int L0 = R == 0 ? 0 : Lay<LOGL>::FIRST + 4 * (R - 1);
         constexpr int RL = R == 0 ? Lay<LOGL>::FIRST : 4;
-        inv_round<LOGL, LINES, NT, L0, RL>(lds, tw, gbase, log_t, p, pinv);
+        inv_round<LOGL, LINES, NT, L0, RL>(lds, tw, p, pinv);
         __syncthreads();
-        inv_rounds<LOGL, LINES, NT, R - 1>(lds, tw, gbase, log_t, p, pinv);
+        inv_rounds<LOGL, LINES, NT, R - 1>(lds, tw, p, pinv);
     }
 }
 
@@ -188,29 +185,6 @@ __global__ void k_tables(TableArgs a) {
     }
 }
 
-// Fills the heap of roots for first node H: tw[2^l + x] = W[H*2^l + x] for
-// l < LOGL, x < 2^l, as W[x] * W[H*2^l].
-template <int LOGL, int NT>
-__device__ __forceinline__ void build_heap(double* tw, const double* table, uint64_t H, double p,
-                                           double pinv) {
-    constexpr int L = 1 << LOGL;
-    for (int j = threadIdx.x + 1; j < L; j += NT) {
-        const int l = 31 - __clz(j);
-        const int x = j - (1 << l);
-        if (H == 0) {
-            tw[j] = table[x];
-        } else {
-            tw[j] = reduce(mulmod(table[x], table[H << l], p, pinv), p, pinv);
-        }
-    }
-}
-
-template <int LOGL, int NT>
-__device__ __forceinline__ void load_prefix(double* tw, const double* table) {
-    constexpr int L = 1 << LOGL;
-    for (int j = threadIdx.x; j < L / 2; j += NT) tw[j] = table[j];
-}
-
 // ---------------------------------------------------------------------------
 // Strided passes: the tile is LINES consecutive lines lo0..lo0+LINES-1 of
 // first node H; element m of line lo sits at H*(T >> level_start) + m*S + lo.
@@ -220,7 +194,6 @@ __global__ __launch_bounds__(NT) void k_fwd_strided(PassArgs a) {
     constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
     __shared__ double lds[LINES * LS];
-    __shared__ double tws[L];
     const int y = blockIdx.y;
     const double p = a.field.p[y], pinv = a.field.pinv[y];
     const uint64_t S = a.line_stride;
@@ -230,13 +203,6 @@ __global__ __launch_bounds__(NT) void k_fwd_strided(PassArgs a) {
     const uint64_t span = (uint64_t)1 << (a.log_t - a.level_start);
     const uint64_t base = H * span + lo0;
     double* row = a.data + (uint64_t)y * a.row_stride;
-    const double* table = a.fwd + (uint64_t)y * a.table_stride;
-    const bool heap = a.level_start != 0;
-    if (heap) {
-        build_heap<LOGL, NT>(tws, table, H, p, pinv);
-    } else {
-        load_prefix<LOGL, NT>(tws, table);
-    }
     if constexpr (PACK) {
         const Source& src = a.src_a;
         const double* pw = a.powers[y];
@@ -251,8 +217,8 @@ __global__ __launch_bounds__(NT) void k_fwd_strided(PassArgs a) {
         }
     }
     __syncthreads();
-    const Tw tw{tws, 0, heap};
-    fwd_rounds<LOGL, LINES, NT, 0>(lds, tw, a.level_start, a.log_t - 1, p, pinv);
+    const Tw tw{a.fwd + (uint64_t)y * a.table_stride, H, 0};
+    fwd_rounds<LOGL, LINES, NT, 0>(lds, tw, p, pinv);
     for (int f = threadIdx.x; f < LINES * L; f += NT) {
         const int c = f % LINES, m = f / LINES;
         row[base + (uint64_t)m * S + c] = lds[c * LS + pidx(m)];
@@ -266,7 +232,6 @@ __global__ __launch_bounds__(NT) void k_inv_strided(PassArgs a) {
     constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
     __shared__ double lds[LINES * LS];
-    __shared__ double tws[L];
     const int y = blockIdx.y;
     const double p = a.field.p[y], pinv = a.field.pinv[y];
     const uint64_t S = a.line_stride;
@@ -276,20 +241,13 @@ __global__ __launch_bounds__(NT) void k_inv_strided(PassArgs a) {
     const uint64_t span = (uint64_t)1 << (a.log_t - a.level_start);
     const uint64_t base = H * span + lo0;
     double* row = a.data + (uint64_t)y * a.row_stride;
-    const double* table = a.inv + (uint64_t)y * a.table_stride;
-    const bool heap = a.level_start != 0;
-    if (heap) {
-        build_heap<LOGL, NT>(tws, table, H, p, pinv);
-    } else {
-        load_prefix<LOGL, NT>(tws, table);
-    }
     for (int f = threadIdx.x; f < LINES * L; f += NT) {
         const int c = f % LINES, m = f / LINES;
         lds[c * LS + pidx(m)] = row[base + (uint64_t)m * S + c];
     }
     __syncthreads();
-    const Tw tw{tws, 0, heap};
-    inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, tw, a.level_start, a.log_t, p, pinv);
+    const Tw tw{a.inv + (uint64_t)y * a.table_stride, H, 0};
+    inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, tw, p, pinv);
     const double tinv = a.tinv[y];
     for (int f = threadIdx.x; f < LINES * L; f += NT) {
         const int c = f % LINES, m = f / LINES;
@@ -308,20 +266,17 @@ __global__ __launch_bounds__(NT) void k_fwd_rows(PassArgs a) {
     constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
     __shared__ double lds[LINES * LS];
-    __shared__ double tws[LINES * L];
     const int y = blockIdx.y;
     const double p = a.field.p[y], pinv = a.field.pinv[y];
     const uint64_t H0 = (uint64_t)blockIdx.x * LINES;
     double* row = a.data + (uint64_t)y * a.row_stride + H0 * L;
-    const double* table = a.fwd + (uint64_t)y * a.table_stride;
-    for (int c = 0; c < LINES; ++c) build_heap<LOGL, NT>(tws + c * L, table, H0 + c, p, pinv);
     for (int f = threadIdx.x; f < LINES * L; f += NT) {
         const int c = f / L, m = f % L;
         lds[c * LS + pidx(m)] = row[f];
     }
     __syncthreads();
-    const Tw tw{tws, L, true};
-    fwd_rounds<LOGL, LINES, NT, 0>(lds, tw, a.level_start, a.log_t - 1, p, pinv);
+    const Tw tw{a.fwd + (uint64_t)y * a.table_stride, H0, 1};
+    fwd_rounds<LOGL, LINES, NT, 0>(lds, tw, p, pinv);
     for (int f = threadIdx.x; f < LINES * L; f += NT) {
         const int c = f / L, m = f % L;
         row[f] = lds[c * LS + pidx(m)];
@@ -337,35 +292,25 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
     constexpr int TILES = (PACK && !SQUARE) ? 2 : 1;
-    constexpr int TWS = PACK ? L : 2 * L;
     __shared__ double lds[TILES * LS];
-    __shared__ double tws[TWS];
     const int y = blockIdx.y;
     const double p = a.field.p[y], pinv = a.field.pinv[y];
     const uint64_t H = blockIdx.x;
     double* row = a.data + (uint64_t)y * a.row_stride + H * L;
-    const double* ftable = a.fwd + (uint64_t)y * a.table_stride;
-    const double* itable = a.inv + (uint64_t)y * a.table_stride;
-    double* twf = tws;
-    double* twi = PACK ? tws + L / 2 : tws + L;
     if constexpr (PACK) {
-        load_prefix<LOGL, NT>(twf, ftable);
-        load_prefix<LOGL, NT>(twi, itable);
         const double* pw = a.powers[y];
         for (int m = threadIdx.x; m < L; m += NT) {
             lds[pidx(m)] = pack_coef(a.src_a, (uint64_t)m, pw, p, pinv);
             if constexpr (!SQUARE) lds[LS + pidx(m)] = pack_coef(a.src_b, (uint64_t)m, pw, p, pinv);
         }
     } else {
-        build_heap<LOGL, NT>(twf, ftable, H, p, pinv);
-        build_heap<LOGL, NT>(twi, itable, H, p, pinv);
         for (int m = threadIdx.x; m < L; m += NT) lds[pidx(m)] = row[m];
     }
     __syncthreads();
-    const Tw twF{twf, 0, !PACK};
-    const Tw twI{twi, 0, !PACK};
-    fwd_rounds<LOGL, 1, NT, 0>(lds, twF, a.level_start, a.log_t - 1, p, pinv);
-    if constexpr (TILES == 2) fwd_rounds<LOGL, 1, NT, 0>(lds + LS, twF, a.level_start, a.log_t - 1, p, pinv);
+    const Tw twF{a.fwd + (uint64_t)y * a.table_stride, H, 0};
+    const Tw twI{a.inv + (uint64_t)y * a.table_stride, H, 0};
+    fwd_rounds<LOGL, 1, NT, 0>(lds, twF, p, pinv);
+    if constexpr (TILES == 2) fwd_rounds<LOGL, 1, NT, 0>(lds + LS, twF, p, pinv);
     if constexpr (SQUARE) {
         for (int m = threadIdx.x; m < L; m += NT) {
             const double v = lds[pidx(m)];
@@ -379,7 +324,7 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
         for (int m = threadIdx.x; m < L; m += NT) lds[pidx(m)] = mulmod(lds[pidx(m)], other[m], p, pinv);
     }
     __syncthreads();
-    inv_rounds<LOGL, 1, NT, Lay<LOGL>::ROUNDS - 1>(lds, twI, a.level_start, a.log_t, p, pinv);
+    inv_rounds<LOGL, 1, NT, Lay<LOGL>::ROUNDS - 1>(lds, twI, p, pinv);
     if constexpr (PACK) {
         const double tinv = a.tinv[y];
         for (int m = threadIdx.x; m < L; m += NT)
@@ -578,6 +523,12 @@ __global__ __launch_bounds__(256) void k_crt_disjoint(CrtArgs a) {
 template <int LOGL>
 static constexpr int strided_lines() { return LOGL <= 10 ? 16 : (LOGL == 11 ? 8 : 4); }
 
+// Threads of a strided tile: one radix-16 group per thread and round.
+template <int LOGL>
+static constexpr int strided_threads() {
+    return strided_lines<LOGL>() * (1 << LOGL) / 16 >= 512 ? 512 : 256;
+}
+
 template <int LOGL>
 static bool launch_fwd_strided_t(const PassArgs& a, int primes, bool pack, hipStream_t s) {
     constexpr int LINES = strided_lines<LOGL>();
@@ -585,9 +536,9 @@ static bool launch_fwd_strided_t(const PassArgs& a, int primes, bool pack, hipSt
     const uint64_t tiles = nodes * (a.line_stride / LINES);
     dim3 grid((unsigned)tiles, (unsigned)primes);
     if (pack)
-        hipLaunchKernelGGL((k_fwd_strided<LOGL, LINES, 256, true>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_fwd_strided<LOGL, LINES, strided_threads<LOGL>(), true>), grid, dim3(strided_threads<LOGL>()), 0, s, a);
     else
-        hipLaunchKernelGGL((k_fwd_strided<LOGL, LINES, 256, false>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_fwd_strided<LOGL, LINES, strided_threads<LOGL>(), false>), grid, dim3(strided_threads<LOGL>()), 0, s, a);
     return NTT_CHECK_LAUNCH();
 }
 
@@ -598,9 +549,9 @@ static bool launch_inv_strided_t(const PassArgs& a, int primes, bool canon, hipS
     const uint64_t tiles = nodes * (a.line_stride / LINES);
     dim3 grid((unsigned)tiles, (unsigned)primes);
     if (canon)
-        hipLaunchKernelGGL((k_inv_strided<LOGL, LINES, 256, true>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_inv_strided<LOGL, LINES, strided_threads<LOGL>(), true>), grid, dim3(strided_threads<LOGL>()), 0, s, a);
     else
-        hipLaunchKernelGGL((k_inv_strided<LOGL, LINES, 256, false>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_inv_strided<LOGL, LINES, strided_threads<LOGL>(), false>), grid, dim3(strided_threads<LOGL>()), 0, s, a);
     return NTT_CHECK_LAUNCH();
 }
 
