@@ -200,6 +200,10 @@ bool ntt_mul_geometry(size_t len_a, size_t len_b, uint32_t *primes, uint32_t *ch
  * arithmetic on the host. Returns the number of failures; needs no GPU. */
 int ntt_selftest_host(uint64_t seed);
 
+/* Copies `bytes` (a multiple of 16) between device buffers with a plain copy
+ * kernel, the calibration reference for hardware byte counters. */
+bool ntt_debug_copy(void *d_dst, const void *d_src, uint64_t bytes);
+
 /* Copies the device root tables of all eight primes at `len` entries each. */
 bool ntt_debug_tables(uint64_t len, double *fwd, double *inv);
 

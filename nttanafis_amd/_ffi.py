@@ -105,6 +105,7 @@ SIGNATURES = {
         [size_t, size_t, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)],
     ),
     "ntt_selftest_host": (ctypes.c_int, [u64]),
+    "ntt_debug_copy": (ctypes.c_bool, [ctypes.c_void_p, ctypes.c_void_p, u64]),
     "ntt_debug_tables": (ctypes.c_bool, [u64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "ntt_debug_residues": (
         ctypes.c_bool,

@@ -19,24 +19,65 @@ __host__ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
 // reference's Packing::pack (src/limb/packing.rs:113-292,
 // src/arch/schedule/expression/packing.rs:68-109).
 
+// 64 bits of the operand from bit `pos`, zero beyond its end. Both loads are
+// unconditional (clamped indices), so unrolled callers can issue them early.
 __host__ __device__ __forceinline__ uint64_t window64(const uint64_t* limbs, uint64_t len, uint64_t pos) {
     const uint64_t idx = pos >> 6;
     const uint32_t off = (uint32_t)(pos & 63);
-    uint64_t v = idx < len ? limbs[idx] >> off : 0;
-    if (off > 64 - kPartBits && idx + 1 < len) v |= limbs[idx + 1] << (64 - off);
-    return v;
+    const uint64_t last = len - 1;
+    const uint64_t lo_raw = limbs[idx < last ? idx : last];
+    const uint64_t hi_raw = limbs[idx + 1 < last ? idx + 1 : last];
+    const uint64_t lo = idx < len ? lo_raw : 0;
+    const uint64_t hi = idx + 1 < len ? hi_raw : 0;
+    return (lo >> off) | ((hi << 1) << (63 - off));
 }
 
 __host__ __device__ __forceinline__ double pack_coef(const Source& s, uint64_t n, const double* pw, double p,
-                                            double pinv) {
-    if (n >= s.digits) return 0.0;
+                                                     double pinv) {
     const uint32_t vb = s.value_bits;
+    if (s.width == 64) {
+        // One coefficient per limb: at most two parts, both in the limb.
+        const uint64_t last = s.limb_len - 1;
+        const uint64_t limb = s.limbs[n < last ? n : last];
+        const int high_bits = vb > (uint32_t)kPartBits ? (int)vb - kPartBits : (int)vb;
+        double r;
+        if (vb > (uint32_t)kPartBits) {
+            const double d0 = (double)(limb & ((1ull << kPartBits) - 1));
+            const uint64_t masked = (limb >> kPartBits) & ((1ull << high_bits) - 1);
+            const uint64_t flip = s.is_signed ? 1ull << (high_bits - 1) : 0;
+            const double d1 = (double)((int64_t)(masked ^ flip) - (int64_t)flip);
+            r = reduce(d0 + mulmod(d1, pw[1], p, pinv), p, pinv);
+        } else {
+            const uint64_t masked = limb & ((1ull << high_bits) - 1);
+            const uint64_t flip = s.is_signed ? 1ull << (high_bits - 1) : 0;
+            r = (double)((int64_t)(masked ^ flip) - (int64_t)flip);
+        }
+        return n < s.digits ? r : 0.0;
+    }
+    // General width (at most 192 bits): one window of four limbs covers the
+    // coefficient at any bit offset; its parts are cut from registers.
     const int parts = (int)((vb + kPartBits - 1) / kPartBits);
     const int high_bits = (int)vb - kPartBits * (parts - 1);
     const uint64_t bitpos = n * (uint64_t)s.width;
+    const uint64_t idx = bitpos >> 6;
+    const uint32_t off = (uint32_t)(bitpos & 63);
+    const uint64_t last = s.limb_len - 1;
+    uint64_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint64_t i = idx + q;
+        const uint64_t raw = s.limbs[i < last ? i : last];
+        w[q] = i < s.limb_len ? raw : 0;
+    }
     double sum = 0.0;
-    for (int k = 0; k < parts; ++k) {
-        const uint64_t word = window64(s.limbs, s.limb_len, bitpos + (uint64_t)(kPartBits * k));
+#pragma unroll
+    for (int k = 0; k < kMaxParts; ++k) {
+        if (k >= parts) break;
+        const uint32_t bit = off + (uint32_t)(kPartBits * k);
+        const uint32_t q = bit >> 6, sh = bit & 63;
+        const uint64_t lo = q == 0 ? w[0] : q == 1 ? w[1] : q == 2 ? w[2] : w[3];
+        const uint64_t hi = q == 0 ? w[1] : q == 1 ? w[2] : q == 2 ? w[3] : 0;
+        const uint64_t word = (lo >> sh) | ((hi << 1) << (63 - sh));
         double d;
         if (k + 1 == parts) {
             const uint64_t masked = word & ((1ull << high_bits) - 1);
@@ -51,14 +92,9 @@ __host__ __device__ __forceinline__ double pack_coef(const Source& s, uint64_t n
         }
         sum = k == 0 ? d : sum + mulmod(d, pw[k], p, pinv);
     }
-    return parts > 1 ? reduce(sum, p, pinv) : sum;
+    const double r = parts > 1 ? reduce(sum, p, pinv) : sum;
+    return n < s.digits ? r : 0.0;
 }
-
-// ---------------------------------------------------------------------------
-// CRT: Garner's mixed-radix digits in FP64, then the integer value
-// X = y0 + p0*(y1 + p1*(y2 + ...)) in 64-bit words. The reference recombines
-// with cofactors and one approximate quotient (src/limb/cofactor.rs:59-111);
-// both return the same integer below the prefix capacity.
 
 template <int NP>
 struct CrtWords {
@@ -66,11 +102,9 @@ struct CrtWords {
 };
 
 template <int NP>
-__host__ __device__ __forceinline__ void garner(const CrtArgs& a, uint64_t i, uint64_t* X, bool balanced) {
+__host__ __device__ __forceinline__ void garner_r(const CrtArgs& a, const double* r, uint64_t* X, bool balanced) {
     constexpr int NW = CrtWords<NP>::value;
-    double r[NP], y[NP];
-#pragma unroll
-    for (int k = 0; k < NP; ++k) r[k] = a.residues[(uint64_t)k * a.row_stride + i];
+    double y[NP];
     y[0] = r[0];
 #pragma unroll
     for (int k = 1; k < NP; ++k) {
@@ -99,6 +133,14 @@ __host__ __device__ __forceinline__ void garner(const CrtArgs& a, uint64_t i, ui
             carry = hi;
         }
     }
+}
+
+template <int NP>
+__host__ __device__ __forceinline__ void garner(const CrtArgs& a, uint64_t i, uint64_t* X, bool balanced) {
+    double r[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) r[k] = a.residues[(uint64_t)k * a.row_stride + i];
+    garner_r<NP>(a, r, X, balanced);
 }
 
 // Carry function of a limb range: carry_out(c) = base + (c >= thresh).

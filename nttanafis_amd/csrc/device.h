@@ -65,6 +65,8 @@ struct PassArgs {
     uint64_t table_stride;   // entries per prime in the root tables
     int32_t log_t;           // K = log2(T)
     int32_t level_start;     // global level of this pass's first level
+    int32_t primes;          // primes of the operation (packing passes)
+    int32_t pad0;
     uint64_t line_stride;    // element stride of a strided pass, 1 for rows
     Source src_a;            // packing sources (first forward pass)
     Source src_b;
@@ -81,6 +83,7 @@ struct CrtArgs {
     uint64_t out_len;        // output limbs
     uint64_t* out;           // output limbs (device)
     uint32_t width;          // bits between coefficients (product) or stride*64 (conv)
+    uint32_t width_magic;    // ceil(2^32 / width): x / width == mulhi(x, magic) for x < 2^24
     uint32_t value_words;    // words of one reconstructed value
     uint32_t value_bits;     // bits of one reconstructed value (bound)
     uint32_t stride;         // conv: limbs per coefficient

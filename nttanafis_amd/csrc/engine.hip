@@ -29,6 +29,7 @@ bool launch_conv_columns(const uint64_t* a, uint64_t la, const uint64_t* b, uint
                          uint32_t stride, uint32_t is_signed, hipStream_t s);
 
 bool launch_tables(const TableArgs& a, int primes, hipStream_t s);
+bool launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Catalog: the reference's eight primes 2^49 < p < 2^50 with generators of
@@ -41,7 +42,9 @@ static const uint64_t kCatalog[kMaxPrimes][2] = {
     {1013749720809473ULL, 3},  {868614185943041ULL, 3},
 };
 constexpr int kMaxRootOrder = 41;  // min 2-adic valuation of p-1 over the catalog
-constexpr int kMaxChunkBits = kPartBits * kMaxParts;
+// Device packing reads one four-limb window per coefficient, so device
+// coefficients are at most 192 bits wide (the reference admits 196).
+constexpr int kMaxChunkBits = 192;
 constexpr int kMaxLog = 30;
 constexpr uint64_t kSchoolbookMaxTerms = 24;  // src/convolve.rs:24
 
@@ -566,6 +569,8 @@ static CrtArgs crt_args(const Geometry& g, const double* rows, uint64_t* out, ui
     memcpy(a.pint, c.p, sizeof a.pint);
     a.value_bits = g.bound_bits;
     a.value_words = (uint32_t)crt_words((int)g.primes);
+    const uint32_t wbits = g.disjoint ? g.stride * 64 : g.width;
+    a.width_magic = (uint32_t)(((1ull << 32) + wbits - 1) / wbits);
     a.is_signed = g.is_signed ? 1 : 0;
     return a;
 }
@@ -1140,7 +1145,7 @@ int ntt_selftest_host(uint64_t seed) {
             if (!check_mod(q, x, p) || fabs(q) >= 9.0 * pf / 16.0) ++fails;
         }
         // Packing of every chunk width shape against exact digits.
-        static const uint32_t widths[] = {22, 49, 50, 64, 69, 98, 99, 139, 143, 147, 196};
+        static const uint32_t widths[] = {22, 49, 50, 64, 69, 98, 99, 139, 143, 147, 191, 192};
         uint64_t limbs[24];
         for (uint32_t width : widths) {
             for (int t = 0; t < 3; ++t) {
@@ -1211,6 +1216,12 @@ int ntt_selftest_host(uint64_t seed) {
         }
     }
     return fails;
+}
+
+bool ntt_debug_copy(void* d_dst, const void* d_src, uint64_t bytes) {
+    if (bytes % 16) return false;
+    if (!launch_copy(d_src, d_dst, bytes, nullptr)) return false;
+    return hipDeviceSynchronize() == hipSuccess;
 }
 
 bool ntt_debug_tables(uint64_t len, double* fwd, double* inv) {

@@ -19,26 +19,33 @@
 
 namespace nttamd {
 
-__device__ __forceinline__ int pidx(int e) { return e + (e >> 4); }
+__device__ __forceinline__ int pidx(int e) { return e + (e >> 5); }
 
 template <int LOGL>
 struct Lay {
     static constexpr int L = 1 << LOGL;
-    static constexpr int STRIDE = L + (L >> 4) + 1;  // padded line in LDS
+    static constexpr int STRIDE = L + (L >> 5) + 1;  // padded line in LDS
     static constexpr int ROUNDS = (LOGL + 3) / 4;
     static constexpr int FIRST = LOGL - 4 * (ROUNDS - 1);  // levels of round 0
 };
 
-// Roots of the tree are read from the root table of one prime in global
-// memory (L1/L2-resident: a row pass reads each root a handful of times).
+// Roots of the tree come from the root table of one prime in global memory.
 // Line c of a tile starts at node h0 + c*hstep of the pass's first level, so
-// its node x at local level l is table[(h0 + c*hstep) << l | x].
+// its node x at local level l has the root W[(h0 + c*hstep) << l | x]. From
+// local level `split` on, where those indices spread over the whole table,
+// the root is formed as W[x] * W[(h0 + c*hstep) << l] instead: the table is
+// bit-reversed, so W[j 2^l + x] = W[j 2^l] W[x] for x < 2^l, and W[x] comes
+// from the short, cache-resident prefix of the table.
 struct Tw {
     const double* table;
     uint64_t h0;
     int hstep;  // 1 when every line is its own node (rows), 0 when lines share one
+    int split;  // first local level whose roots are formed as products
+    double p, pinv;
     __device__ __forceinline__ double at(int c, int lvl, int node) const {
-        return table[((h0 + (uint64_t)(c * hstep)) << lvl) + node];
+        const uint64_t h = (h0 + (uint64_t)(c * hstep)) << lvl;
+        if (lvl >= split) return reduce(mulmod(table[node], table[h], p, pinv), p, pinv);
+        return table[h + node];
     }
 };
 
@@ -186,50 +193,166 @@ __global__ void k_tables(TableArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Tile movement. Every loop below has a compile-time trip count and is fully
+// unrolled, so a thread issues all of its global loads before it waits on
+// the first one.
+
+// Strided tile: LINES consecutive lines lo0.. of one node; element m of line c
+// at g[m*S + c]. Consecutive threads take consecutive lines, so each group of
+// LINES threads reads one LINES*8-byte run.
+template <int LOGL, int LINES, int NT>
+__device__ __forceinline__ void load_strided(double* lds, const double* g, uint64_t S) {
+    constexpr int L = 1 << LOGL;
+    constexpr int LS = Lay<LOGL>::STRIDE;
+    constexpr int TOTAL = LINES * L;
+    constexpr int R = (TOTAL + NT - 1) / NT;
+    double v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int f = threadIdx.x + r * NT;
+        if (TOTAL % NT == 0 || f < TOTAL) v[r] = g[(uint64_t)(f / LINES) * S + (f % LINES)];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int f = threadIdx.x + r * NT;
+        if (TOTAL % NT == 0 || f < TOTAL) lds[(f % LINES) * LS + pidx(f / LINES)] = v[r];
+    }
+}
+
+template <int LOGL, int LINES, int NT, bool CANON>
+__device__ __forceinline__ void store_strided(const double* lds, double* g, uint64_t S, double tinv, double p,
+                                              double pinv) {
+    constexpr int L = 1 << LOGL;
+    constexpr int LS = Lay<LOGL>::STRIDE;
+    constexpr int TOTAL = LINES * L;
+    constexpr int R = (TOTAL + NT - 1) / NT;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int f = threadIdx.x + r * NT;
+        if (TOTAL % NT != 0 && f >= TOTAL) continue;
+        double v = lds[(f % LINES) * LS + pidx(f / LINES)];
+        if constexpr (CANON) v = canonical(mulmod(v, tinv, p, pinv), p);
+        g[(uint64_t)(f / LINES) * S + (f % LINES)] = v;
+    }
+}
+
+// Strided tile packed from limbs: element m of line c is coefficient
+// n0 + m*S + c of the operand.
+template <int LOGL, int LINES, int NT>
+__device__ __forceinline__ void pack_strided(double* lds, const Source& src, uint64_t n0, uint64_t S,
+                                             const double* pw, double p, double pinv) {
+    constexpr int L = 1 << LOGL;
+    constexpr int LS = Lay<LOGL>::STRIDE;
+    constexpr int TOTAL = LINES * L;
+    constexpr int R = (TOTAL + NT - 1) / NT;
+    constexpr int BATCH = (R >= 4 && R % 4 == 0) ? 4 : 1;
+#pragma unroll
+    for (int r0 = 0; r0 < R; r0 += BATCH) {
+        double v[BATCH];
+#pragma unroll
+        for (int r = 0; r < BATCH; ++r) {
+            const int f = threadIdx.x + (r0 + r) * NT;
+            if (TOTAL % NT == 0 || f < TOTAL)
+                v[r] = pack_coef(src, n0 + (uint64_t)(f / LINES) * S + (f % LINES), pw, p, pinv);
+        }
+#pragma unroll
+        for (int r = 0; r < BATCH; ++r) {
+            const int f = threadIdx.x + (r0 + r) * NT;
+            if (TOTAL % NT == 0 || f < TOTAL) lds[(f % LINES) * LS + pidx(f / LINES)] = v[r];
+        }
+    }
+}
+
+// Row tile: LINES rows of L contiguous points; thread t moves points 2t, 2t+1
+// of each 2*NT-point stripe with 16-byte accesses.
+template <int LOGL, int LINES, int NT>
+__device__ __forceinline__ void load_rows(double* lds, const double* g) {
+    constexpr int L = 1 << LOGL;
+    constexpr int LS = Lay<LOGL>::STRIDE;
+    constexpr int PAIRS = LINES * L / 2;
+    constexpr int R = (PAIRS + NT - 1) / NT;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (PAIRS % NT == 0 || threadIdx.x + r * NT < PAIRS)
+            v[r] = reinterpret_cast<const double2*>(g)[threadIdx.x + r * NT];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (PAIRS % NT != 0 && threadIdx.x + r * NT >= PAIRS) continue;
+        const int f = 2 * (threadIdx.x + r * NT);
+        const int c = f / L, m = f % L;
+        lds[c * LS + pidx(m)] = v[r].x;
+        lds[c * LS + pidx(m + 1)] = v[r].y;
+    }
+}
+
+template <int LOGL, int LINES, int NT>
+__device__ __forceinline__ void store_rows(const double* lds, double* g) {
+    constexpr int L = 1 << LOGL;
+    constexpr int LS = Lay<LOGL>::STRIDE;
+    constexpr int PAIRS = LINES * L / 2;
+    constexpr int R = (PAIRS + NT - 1) / NT;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (PAIRS % NT != 0 && threadIdx.x + r * NT >= PAIRS) continue;
+        const int f = 2 * (threadIdx.x + r * NT);
+        const int c = f / L, m = f % L;
+        double2 v;
+        v.x = lds[c * LS + pidx(m)];
+        v.y = lds[c * LS + pidx(m + 1)];
+        reinterpret_cast<double2*>(g)[threadIdx.x + r * NT] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Strided passes: the tile is LINES consecutive lines lo0..lo0+LINES-1 of
 // first node H; element m of line lo sits at H*(T >> level_start) + m*S + lo.
 
+// Workgroup b of a packing pass runs on XCD b % 8 (round-robin dispatch).
+// Tiles are dealt to XCDs so that the workgroups of all primes for one tile
+// are consecutive on one XCD: they read the same limbs, which the first one
+// brings into that XCD's L2.
+__device__ __forceinline__ bool xcd_tile(int primes, uint64_t tiles, uint64_t* tile, int* y) {
+    const uint64_t b = blockIdx.x;
+    const uint64_t slot = b >> 3;
+    const uint64_t t = (slot / (uint64_t)primes) * 8 + (b & 7);
+    *y = (int)(slot % (uint64_t)primes);
+    *tile = t;
+    return t < tiles;
+}
+
 template <int LOGL, int LINES, int NT, bool PACK>
 __global__ __launch_bounds__(NT) void k_fwd_strided(PassArgs a) {
-    constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
     __shared__ double lds[LINES * LS];
-    const int y = blockIdx.y;
-    const double p = a.field.p[y], pinv = a.field.pinv[y];
     const uint64_t S = a.line_stride;
     const uint64_t tiles_per_node = S / LINES;
-    const uint64_t H = blockIdx.x / tiles_per_node;
-    const uint64_t lo0 = (blockIdx.x % tiles_per_node) * LINES;
+    uint64_t tile = blockIdx.x;
+    int y = blockIdx.y;
+    if constexpr (PACK) {
+        if (!xcd_tile(a.primes, tiles_per_node << a.level_start, &tile, &y)) return;
+    }
+    const double p = a.field.p[y], pinv = a.field.pinv[y];
+    const uint64_t H = tile / tiles_per_node;
+    const uint64_t lo0 = (tile % tiles_per_node) * LINES;
     const uint64_t span = (uint64_t)1 << (a.log_t - a.level_start);
     const uint64_t base = H * span + lo0;
     double* row = a.data + (uint64_t)y * a.row_stride;
     if constexpr (PACK) {
-        const Source& src = a.src_a;
-        const double* pw = a.powers[y];
-        for (int f = threadIdx.x; f < LINES * L; f += NT) {
-            const int c = f % LINES, m = f / LINES;
-            lds[c * LS + pidx(m)] = pack_coef(src, base + (uint64_t)m * S + c, pw, p, pinv);
-        }
+        pack_strided<LOGL, LINES, NT>(lds, a.src_a, base, S, a.powers[y], p, pinv);
     } else {
-        for (int f = threadIdx.x; f < LINES * L; f += NT) {
-            const int c = f % LINES, m = f / LINES;
-            lds[c * LS + pidx(m)] = row[base + (uint64_t)m * S + c];
-        }
+        load_strided<LOGL, LINES, NT>(lds, row + base, S);
     }
     __syncthreads();
-    const Tw tw{a.fwd + (uint64_t)y * a.table_stride, H, 0};
+    const Tw tw{a.fwd + (uint64_t)y * a.table_stride, H, 0, 64, p, pinv};
     fwd_rounds<LOGL, LINES, NT, 0>(lds, tw, p, pinv);
-    for (int f = threadIdx.x; f < LINES * L; f += NT) {
-        const int c = f % LINES, m = f / LINES;
-        row[base + (uint64_t)m * S + c] = lds[c * LS + pidx(m)];
-    }
+    store_strided<LOGL, LINES, NT, false>(lds, row + base, S, 0.0, p, pinv);
 }
 
 // Inverse strided pass; CANON (level_start == 0) multiplies by T^-1 and stores
 // canonical residues in [0, p).
 template <int LOGL, int LINES, int NT, bool CANON>
 __global__ __launch_bounds__(NT) void k_inv_strided(PassArgs a) {
-    constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
     __shared__ double lds[LINES * LS];
     const int y = blockIdx.y;
@@ -241,21 +364,15 @@ __global__ __launch_bounds__(NT) void k_inv_strided(PassArgs a) {
     const uint64_t span = (uint64_t)1 << (a.log_t - a.level_start);
     const uint64_t base = H * span + lo0;
     double* row = a.data + (uint64_t)y * a.row_stride;
-    for (int f = threadIdx.x; f < LINES * L; f += NT) {
-        const int c = f % LINES, m = f / LINES;
-        lds[c * LS + pidx(m)] = row[base + (uint64_t)m * S + c];
-    }
+    load_strided<LOGL, LINES, NT>(lds, row + base, S);
     __syncthreads();
-    const Tw tw{a.inv + (uint64_t)y * a.table_stride, H, 0};
+    const Tw tw{a.inv + (uint64_t)y * a.table_stride, H, 0, 64, p, pinv};
     inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, tw, p, pinv);
-    const double tinv = a.tinv[y];
-    for (int f = threadIdx.x; f < LINES * L; f += NT) {
-        const int c = f % LINES, m = f / LINES;
-        double v = lds[c * LS + pidx(m)];
-        if constexpr (CANON) v = canonical(mulmod(v, tinv, p, pinv), p);
-        row[base + (uint64_t)m * S + c] = v;
-    }
+    store_strided<LOGL, LINES, NT, CANON>(lds, row + base, S, a.tinv[y], p, pinv);
 }
+
+// Row passes form the roots of their last four levels as products.
+__host__ __device__ constexpr int row_split(int logl) { return logl > 8 ? logl - 4 : 64; }
 
 // ---------------------------------------------------------------------------
 // Row passes: rows H0..H0+LINES-1 of 2^LOGL contiguous points, the last
@@ -270,17 +387,11 @@ __global__ __launch_bounds__(NT) void k_fwd_rows(PassArgs a) {
     const double p = a.field.p[y], pinv = a.field.pinv[y];
     const uint64_t H0 = (uint64_t)blockIdx.x * LINES;
     double* row = a.data + (uint64_t)y * a.row_stride + H0 * L;
-    for (int f = threadIdx.x; f < LINES * L; f += NT) {
-        const int c = f / L, m = f % L;
-        lds[c * LS + pidx(m)] = row[f];
-    }
+    load_rows<LOGL, LINES, NT>(lds, row);
     __syncthreads();
-    const Tw tw{a.fwd + (uint64_t)y * a.table_stride, H0, 1};
+    const Tw tw{a.fwd + (uint64_t)y * a.table_stride, H0, 1, row_split(LOGL), p, pinv};
     fwd_rounds<LOGL, LINES, NT, 0>(lds, tw, p, pinv);
-    for (int f = threadIdx.x; f < LINES * L; f += NT) {
-        const int c = f / L, m = f % L;
-        row[f] = lds[c * LS + pidx(m)];
-    }
+    store_rows<LOGL, LINES, NT>(lds, row);
 }
 
 // Fused row pass: forward levels of one row, pointwise product with the other
@@ -292,61 +403,90 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
     constexpr int TILES = (PACK && !SQUARE) ? 2 : 1;
+    constexpr int R = (L + NT - 1) / NT;
+    constexpr bool FULL = L % NT == 0;
     __shared__ double lds[TILES * LS];
     const int y = blockIdx.y;
     const double p = a.field.p[y], pinv = a.field.pinv[y];
     const uint64_t H = blockIdx.x;
     double* row = a.data + (uint64_t)y * a.row_stride + H * L;
     if constexpr (PACK) {
-        const double* pw = a.powers[y];
-        for (int m = threadIdx.x; m < L; m += NT) {
-            lds[pidx(m)] = pack_coef(a.src_a, (uint64_t)m, pw, p, pinv);
-            if constexpr (!SQUARE) lds[LS + pidx(m)] = pack_coef(a.src_b, (uint64_t)m, pw, p, pinv);
-        }
+        pack_strided<LOGL, 1, NT>(lds, a.src_a, 0, 1, a.powers[y], p, pinv);
+        if constexpr (!SQUARE) pack_strided<LOGL, 1, NT>(lds + LS, a.src_b, 0, 1, a.powers[y], p, pinv);
     } else {
-        for (int m = threadIdx.x; m < L; m += NT) lds[pidx(m)] = row[m];
+        load_rows<LOGL, 1, NT>(lds, row);
     }
     __syncthreads();
-    const Tw twF{a.fwd + (uint64_t)y * a.table_stride, H, 0};
-    const Tw twI{a.inv + (uint64_t)y * a.table_stride, H, 0};
+    const Tw twF{a.fwd + (uint64_t)y * a.table_stride, H, 0, PACK ? 64 : row_split(LOGL), p, pinv};
+    const Tw twI{a.inv + (uint64_t)y * a.table_stride, H, 0, PACK ? 64 : row_split(LOGL), p, pinv};
     fwd_rounds<LOGL, 1, NT, 0>(lds, twF, p, pinv);
     if constexpr (TILES == 2) fwd_rounds<LOGL, 1, NT, 0>(lds + LS, twF, p, pinv);
     if constexpr (SQUARE) {
-        for (int m = threadIdx.x; m < L; m += NT) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int m = threadIdx.x + r * NT;
+            if (!FULL && m >= L) continue;
             const double v = lds[pidx(m)];
             lds[pidx(m)] = mulmod(v, v, p, pinv);
         }
     } else if constexpr (PACK) {
-        for (int m = threadIdx.x; m < L; m += NT)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int m = threadIdx.x + r * NT;
+            if (!FULL && m >= L) continue;
             lds[pidx(m)] = mulmod(lds[pidx(m)], lds[LS + pidx(m)], p, pinv);
+        }
     } else {
         const double* other = a.other + (uint64_t)y * a.row_stride + H * L;
-        for (int m = threadIdx.x; m < L; m += NT) lds[pidx(m)] = mulmod(lds[pidx(m)], other[m], p, pinv);
+        double o[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (FULL || threadIdx.x + r * NT < L) o[r] = other[threadIdx.x + r * NT];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int m = threadIdx.x + r * NT;
+            if (!FULL && m >= L) continue;
+            lds[pidx(m)] = mulmod(lds[pidx(m)], o[r], p, pinv);
+        }
     }
     __syncthreads();
     inv_rounds<LOGL, 1, NT, Lay<LOGL>::ROUNDS - 1>(lds, twI, p, pinv);
     if constexpr (PACK) {
         const double tinv = a.tinv[y];
-        for (int m = threadIdx.x; m < L; m += NT)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int m = threadIdx.x + r * NT;
+            if (!FULL && m >= L) continue;
             row[m] = canonical(mulmod(lds[pidx(m)], tinv, p, pinv), p);
+        }
     } else {
-        for (int m = threadIdx.x; m < L; m += NT) row[m] = lds[pidx(m)];
+        store_rows<LOGL, 1, NT>(lds, row);
     }
 }
 
 // Product reconstruction, one block of kCrtBlockLimbs output limbs: every
 // coefficient X_i touching the block is rebuilt once into LDS, every limb
 // sums its windows of the shifted coefficients, and carries inside the block
-// resolve by a scan of carry functions. Blocks publish their carry function
-// for k_carry_scan / k_carry_fix (the block merge of src/limb/
+// resolve by a scan of carry functions (per wave with lane shuffles, then
+// across the block's waves). Blocks publish their carry function for
+// k_carry_apply (the block merge of src/limb/
 // reconstruction.rs:211-247).
 constexpr int kCrtLdsWords = 4096;
+
+__device__ __forceinline__ CarryFn shfl_up_fn(CarryFn f, int d) {
+    CarryFn r;
+    r.base = __shfl_up(f.base, d, 64);
+    r.thresh = __shfl_up(f.thresh, d, 64);
+    return r;
+}
 
 template <int NP>
 __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
     constexpr int NW = CrtWords<NP>::value;
+    constexpr int RMAX = (kCrtLdsWords / NW + kCrtThreads - 1) / kCrtThreads;
+    constexpr int WAVES = kCrtThreads / 64;
     __shared__ uint64_t xs[kCrtLdsWords];
-    __shared__ CarryFn scan[kCrtThreads];
+    __shared__ CarryFn wave_total[WAVES];
     __shared__ uint32_t ripple;
     const uint64_t L0 = (uint64_t)blockIdx.x * kCrtBlockLimbs;
     const uint64_t w = a.width;
@@ -356,39 +496,58 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
     uint64_t i_hi = (lo_bit + 64 * (uint64_t)kCrtBlockLimbs - 1) / w + 1;
     if (i_hi > a.coefficients) i_hi = a.coefficients;
     const int count = i_hi > i_lo ? (int)(i_hi - i_lo) : 0;
-    for (int idx = threadIdx.x; idx < count; idx += kCrtThreads) {
-        uint64_t X[NW];
-        garner<NP>(a, i_lo + idx, X, false);
+    // Residues of this thread's coefficients, every load issued first.
+    double r[RMAX][NP];
 #pragma unroll
-        for (int q = 0; q < NW; ++q) xs[idx * NW + q] = X[q];
+    for (int q = 0; q < RMAX; ++q) {
+        const int idx = threadIdx.x + q * kCrtThreads;
+#pragma unroll
+        for (int k = 0; k < NP; ++k)
+            r[q][k] = idx < count ? a.residues[(uint64_t)k * a.row_stride + i_lo + idx] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < RMAX; ++q) {
+        const int idx = threadIdx.x + q * kCrtThreads;
+        if (idx < count) {
+            uint64_t X[NW];
+            garner_r<NP>(a, r[q], X, false);
+#pragma unroll
+            for (int m = 0; m < NW; ++m) xs[idx * NW + m] = X[m];
+        }
     }
     if (threadIdx.x == 0) ripple = 0;
     __syncthreads();
 
+    // Limb j sums the windows of coefficients i with i*w < 64(j+1) and
+    // i*w + xb > 64j. Offsets are taken from bit i_lo*w of the block, which
+    // keeps them below 2^24, so divisions by w are multiplications.
     uint64_t v[kCrtLimbsPerThread];
     CarryFn f[kCrtLimbsPerThread];
+    const uint32_t wr = (uint32_t)w;
+    const uint32_t rel0 = (uint32_t)(lo_bit - i_lo * w);  // bit of limb L0 from coefficient i_lo
+    const uint32_t xb32 = (uint32_t)xb;
+    const uint32_t magic = a.width_magic;
 #pragma unroll
     for (int q = 0; q < kCrtLimbsPerThread; ++q) {
+        const uint32_t jr = rel0 + 64u * (threadIdx.x * kCrtLimbsPerThread + q);  // bit of limb j from i_lo
         const uint64_t j = L0 + (uint64_t)threadIdx.x * kCrtLimbsPerThread + q;
         uint64_t sum = 0;
         uint32_t carry = 0;
         if (j < a.out_len && count > 0) {
-            const uint64_t jb = 64 * j;
-            uint64_t ia = jb + 1 > xb ? (jb + 1 - xb + w - 1) / w : 0;
-            if (ia < i_lo) ia = i_lo;
-            uint64_t ib = (jb + 63) / w;
-            if (ib >= i_hi) ib = i_hi - 1;
-            for (uint64_t i = ia; i <= ib && ib < i_hi; ++i) {
-                const uint64_t* X = xs + (i - i_lo) * NW;
-                const int64_t o = (int64_t)jb - (int64_t)(i * w);
+            const uint32_t ia = jr + 1 > xb32 ? __umulhi(jr + 1 - xb32 + wr - 1, magic) : 0u;
+            uint32_t ib = __umulhi(jr + 63, magic);
+            if (ib >= (uint32_t)count) ib = (uint32_t)count - 1;
+            for (uint32_t ii = ia; ii <= ib; ++ii) {
+                const uint64_t* X = xs + ii * NW;
+                const int32_t o = (int32_t)jr - (int32_t)(ii * wr);
                 uint64_t piece;
                 if (o >= 0) {
-                    const int qw = (int)(o >> 6), sh = (int)(o & 63);
+                    const int qw = o >> 6, sh = o & 63;
                     const uint64_t lo = qw < NW ? X[qw] : 0;
                     const uint64_t hi = qw + 1 < NW ? X[qw + 1] : 0;
                     piece = sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
                 } else {
-                    piece = X[0] << (int)(-o);
+                    piece = X[0] << (-o);
                 }
                 sum += piece;
                 carry += sum < piece;
@@ -397,39 +556,54 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
         v[q] = sum;
         f[q] = CarryFn{carry, limb_thresh(sum)};
     }
+    // Inclusive scan of the threads' carry functions within each wave.
     CarryFn agg = f[0];
 #pragma unroll
     for (int q = 1; q < kCrtLimbsPerThread; ++q) agg = compose(agg, f[q]);
-    scan[threadIdx.x] = agg;
-    __syncthreads();
-    for (int d = 1; d < kCrtThreads; d <<= 1) {
-        CarryFn mine = scan[threadIdx.x];
-        CarryFn prev = threadIdx.x >= (unsigned)d ? scan[threadIdx.x - d] : CarryFn{0, 0};
-        __syncthreads();
-        if (threadIdx.x >= (unsigned)d) scan[threadIdx.x] = compose(prev, mine);
-        __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const CarryFn prev = shfl_up_fn(agg, d);
+        if (lane >= d) agg = compose(prev, agg);
     }
-    // Exclusive prefix: carries into this thread's first limb with zero
-    // carry into the block.
-    bool have = threadIdx.x > 0;
-    CarryFn g = have ? scan[threadIdx.x - 1] : CarryFn{0, kInf};
+    if (lane == 63) wave_total[wave] = agg;
+    const CarryFn lane_excl = shfl_up_fn(agg, 1);
+    __syncthreads();
+    // Exclusive prefix of this thread: earlier waves, then earlier lanes.
+    bool have = false;
+    CarryFn g{0, kInf};
+    for (int u = 0; u < wave; ++u) {
+        g = have ? compose(g, wave_total[u]) : wave_total[u];
+        have = true;
+    }
+    if (lane > 0) {
+        g = have ? compose(g, lane_excl) : lane_excl;
+        have = true;
+    }
     const uint32_t first_t = f[0].thresh;
     uint32_t my_ripple = 0;
+    uint64_t outv[kCrtLimbsPerThread];
 #pragma unroll
     for (int q = 0; q < kCrtLimbsPerThread; ++q) {
         const uint32_t rel = threadIdx.x * kCrtLimbsPerThread + q;
-        const uint64_t j = L0 + rel;
-        const uint32_t cin = have ? g.base : 0u;
         if (have && g.thresh != kInf) my_ripple = rel;
-        if (j < a.out_len) a.out[j] = v[q] + cin;
+        outv[q] = v[q] + (have ? g.base : 0u);
         g = have ? compose(g, f[q]) : f[q];
         have = true;
     }
     if (my_ripple) atomicMax(&ripple, my_ripple);
+    __syncthreads();  // every xs read is done; reuse it to coalesce the stores
+#pragma unroll
+    for (int q = 0; q < kCrtLimbsPerThread; ++q) xs[threadIdx.x * kCrtLimbsPerThread + q] = outv[q];
     __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kCrtLimbsPerThread; ++q) {
+        const uint64_t j = L0 + threadIdx.x + q * kCrtThreads;
+        if (j < a.out_len) a.out[j] = xs[threadIdx.x + q * kCrtThreads];
+    }
     if (threadIdx.x == kCrtThreads - 1) {
-        a.agg_base[blockIdx.x] = scan[kCrtThreads - 1].base;
-        a.agg_thresh[blockIdx.x] = scan[kCrtThreads - 1].thresh;
+        a.agg_base[blockIdx.x] = g.base;
+        a.agg_thresh[blockIdx.x] = g.thresh;
     }
     if (threadIdx.x == 0) {
         a.first_thresh[blockIdx.x] = first_t;
@@ -437,65 +611,55 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
     }
 }
 
-// Carry into every block: a scan of the blocks' carry functions applied to a
-// zero carry into block 0. One workgroup.
-constexpr int kScanThreads = 1024;
-
-__global__ __launch_bounds__(kScanThreads) void k_carry_scan(CrtArgs a) {
-    __shared__ CarryFn scan[kScanThreads];
-    __shared__ uint32_t has[kScanThreads];
-    const uint32_t n = a.blocks;
-    const uint32_t chunk = (n + kScanThreads - 1) / kScanThreads;
-    const uint32_t first = threadIdx.x * chunk;
-    const uint32_t end = first + chunk < n ? first + chunk : n;
-    CarryFn agg{0, kInf};
-    bool any = false;
-    for (uint32_t b = first; b < end; ++b) {
-        const CarryFn fb{a.agg_base[b], a.agg_thresh[b]};
-        agg = any ? compose(agg, fb) : fb;
-        any = true;
-    }
-    scan[threadIdx.x] = agg;
-    has[threadIdx.x] = any;
+// Carry into every block and its application. Block b's carry in is
+// F_{b-1}(F_{b-2}(... F_0(0))) for the published carry functions F. A block
+// whose function has no threshold outputs its base whatever comes in, so the
+// look-back stops at the nearest such block and applies the functions after
+// it. Then the first limb takes the carry, and the limbs up to the recorded
+// ripple end take one more when the carry reaches the first limb's threshold.
+__global__ __launch_bounds__(kCrtThreads) void k_carry_apply(CrtArgs a) {
+    __shared__ uint32_t found;
+    __shared__ uint32_t cin_s;
+    const int b = blockIdx.x;
+    if (b == 0) return;
+    if (threadIdx.x == 0) found = 0xFFFFFFFFu;
     __syncthreads();
-    for (int d = 1; d < kScanThreads; d <<= 1) {
-        CarryFn mine = scan[threadIdx.x];
-        uint32_t mine_has = has[threadIdx.x];
-        CarryFn prev{0, kInf};
-        uint32_t prev_has = 0;
-        if (threadIdx.x >= (unsigned)d) {
-            prev = scan[threadIdx.x - d];
-            prev_has = has[threadIdx.x - d];
-        }
-        __syncthreads();
-        if (threadIdx.x >= (unsigned)d && prev_has) {
-            scan[threadIdx.x] = mine_has ? compose(prev, mine) : prev;
-            has[threadIdx.x] = 1;
-        }
-        __syncthreads();
-    }
     uint32_t c = 0;
-    if (threadIdx.x > 0 && has[threadIdx.x - 1]) {
-        const CarryFn e = scan[threadIdx.x - 1];
-        c = e.base;  // applied to a zero carry into block 0 (thresholds >= 1)
+    int end = b;  // functions of blocks [k, end) still to apply
+    int start = 0;
+    bool resolved = false;
+    while (!resolved) {
+        const int first = end - kCrtThreads;
+        const int k = end - 1 - (int)threadIdx.x;
+        if (k >= 0 && k >= first && a.agg_thresh[k] == kInf) atomicMin(&found, (uint32_t)threadIdx.x);
+        __syncthreads();
+        const uint32_t t = found;
+        __syncthreads();
+        if (t != 0xFFFFFFFFu) {
+            start = end - 1 - (int)t;  // block whose output is fixed
+            resolved = true;
+        } else if (first <= 0) {
+            start = -1;  // no fixed block: begin with a zero carry into block 0
+            resolved = true;
+        } else {
+            end = first;
+            if (threadIdx.x == 0) found = 0xFFFFFFFFu;
+            __syncthreads();
+        }
     }
-    for (uint32_t b = first; b < end; ++b) {
-        a.carry_in[b] = c;
-        c = a.agg_base[b] + (c >= a.agg_thresh[b] ? 1u : 0u);
+    if (threadIdx.x == 0) {
+        uint32_t cc = start >= 0 ? a.agg_base[start] : 0u;
+        for (int k = start + 1; k < b; ++k) cc = a.agg_base[k] + (cc >= a.agg_thresh[k] ? 1u : 0u);
+        cin_s = cc;
     }
-}
-
-// Applies each block's carry: the first limb takes the carry, and limbs up to
-// the recorded ripple end take one more when the carry reaches the first
-// limb's threshold.
-__global__ __launch_bounds__(kCrtThreads) void k_carry_fix(CrtArgs a) {
-    const uint32_t cin = a.carry_in[blockIdx.x];
-    if (cin == 0) return;
-    const uint64_t L0 = (uint64_t)blockIdx.x * kCrtBlockLimbs;
-    if (threadIdx.x == 0 && L0 < a.out_len) a.out[L0] += cin;
-    if (cin >= a.first_thresh[blockIdx.x]) {
-        const uint32_t end = a.ripple_end[blockIdx.x];
-        for (uint32_t rel = 1 + threadIdx.x; rel <= end; rel += kCrtThreads) {
+    __syncthreads();
+    c = cin_s;
+    if (c == 0) return;
+    const uint64_t L0 = (uint64_t)b * kCrtBlockLimbs;
+    if (threadIdx.x == 0 && L0 < a.out_len) a.out[L0] += c;
+    if (c >= a.first_thresh[b]) {
+        const uint32_t rend = a.ripple_end[b];
+        for (uint32_t rel = 1 + threadIdx.x; rel <= rend; rel += kCrtThreads) {
             const uint64_t j = L0 + rel;
             if (j < a.out_len) a.out[j] += 1;
         }
@@ -535,10 +699,14 @@ static bool launch_fwd_strided_t(const PassArgs& a, int primes, bool pack, hipSt
     const uint64_t nodes = (uint64_t)1 << a.level_start;
     const uint64_t tiles = nodes * (a.line_stride / LINES);
     dim3 grid((unsigned)tiles, (unsigned)primes);
-    if (pack)
-        hipLaunchKernelGGL((k_fwd_strided<LOGL, LINES, strided_threads<LOGL>(), true>), grid, dim3(strided_threads<LOGL>()), 0, s, a);
-    else
+    if (pack) {
+        PassArgs b = a;
+        b.primes = primes;
+        const dim3 flat((unsigned)(((tiles + 7) / 8) * 8 * primes), 1);
+        hipLaunchKernelGGL((k_fwd_strided<LOGL, LINES, strided_threads<LOGL>(), true>), flat, dim3(strided_threads<LOGL>()), 0, s, b);
+    } else {
         hipLaunchKernelGGL((k_fwd_strided<LOGL, LINES, strided_threads<LOGL>(), false>), grid, dim3(strided_threads<LOGL>()), 0, s, a);
+    }
     return NTT_CHECK_LAUNCH();
 }
 
@@ -657,9 +825,7 @@ bool launch_crt_product(const CrtArgs& a, hipStream_t s) {
         default: return false;
     }
     if (!NTT_CHECK_LAUNCH()) return false;
-    hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(kScanThreads), 0, s, a);
-    if (!NTT_CHECK_LAUNCH()) return false;
-    hipLaunchKernelGGL(k_carry_fix, dim3(a.blocks), dim3(kCrtThreads), 0, s, a);
+    if (a.blocks > 1) hipLaunchKernelGGL(k_carry_apply, dim3(a.blocks), dim3(kCrtThreads), 0, s, a);
     return NTT_CHECK_LAUNCH();
 }
 
@@ -709,6 +875,19 @@ __global__ __launch_bounds__(256) void k_conv_columns(const uint64_t* a, uint64_
     }
     const uint64_t words[3] = {(uint64_t)low, (uint64_t)(low >> 64), high};
     for (uint32_t q = 0; q < stride; ++q) out[k * stride + q] = q < 3 ? words[q] : 0;
+}
+
+// Plain device copy of 16-byte words: the calibration kernel for HBM byte
+// counters (reads and writes each byte once).
+__global__ __launch_bounds__(256) void k_copy(const uint4* src, uint4* dst, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+bool launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s) {
+    const uint64_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
+    return hipGetLastError() == hipSuccess;
 }
 
 bool launch_conv_columns(const uint64_t* a, uint64_t la, const uint64_t* b, uint64_t lb, uint64_t* out,
