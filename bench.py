@@ -6,7 +6,11 @@ limbs, seeds 1 and 2, as the reference benchmarks use), operands and result
 resident in HBM. A step is one MulWorkspace product on the device; the metric
 is operand limbs per second (Mlimbs/s). Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--workload mul_2p20|sqr_2p24|conv16_2p22|conv32s_2p22]
+    python bench.py [--steps K] [--warmup W] [--workload mul_2p20|sqr_2p24|conv16_2p22|conv32s_2p22|batch4096]
+
+The line also carries the batched form ("batched": 4,096 uneven products in
+one graph launch against the plain one-call-per-product loop) unless
+--no-batch is given.
 """
 from __future__ import annotations
 
@@ -22,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import nttanafis_amd as ntt  # noqa: E402
+from nttanafis_amd.workloads import batch_operands, batch_shapes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -68,13 +73,28 @@ def kernel_bytes(name: str, primes: int, T: int, la: int, lb: int, out_len: int,
         return (16.0 if square else 24.0) * primes * T
     if name == "inv_strided_canon":
         return 16.0 * primes * T
-    if name == "crt_carry":
+    if name == "crt_product":
         return 8.0 * primes * min(T, out_len * 64) + 8.0 * out_len
     if name == "crt_disjoint":
         return 8.0 * primes * (la + lb - 1) + 8.0 * out_len
     if name == "fused_single":
         return 8.0 * (la + lb) + 8.0 * primes * T
     return 0.0
+
+
+def measured_traffic(workload: str) -> dict:
+    """HBM bytes per launch of each kernel kind, from the committed rocprofv3
+    counter summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled
+    by what the counter reads for the plain copy kernel). Lines of the form
+    `TRAFFIC <kind> <bytes>`; empty when no summary is committed."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_counters.txt")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "TRAFFIC":
+                out[parts[1]] = int(float(parts[2]))
+    return out
 
 
 def cpu_baseline(wl, sample_seconds: float):
@@ -118,6 +138,94 @@ def cpu_baseline(wl, sample_seconds: float):
             "sample": f"{runs} convolutions of {len(a)} coefficients, 1 thread"}
 
 
+def mod_q61(limbs) -> int:
+    q = (1 << 61) - 1
+    return int.from_bytes(np.ascontiguousarray(limbs, dtype="<u8").tobytes(), "little") % q
+
+
+def bench_batch(steps: int, warmup: int, plain_steps: int) -> dict:
+    """4,096 products, len_a and len_b log-uniform in [192, 65,536], 20%
+    squares, operands resident in HBM. A step is one graph launch computing
+    every product; the comparison is the plain per-call loop (one
+    MulWorkspace device call per product) over the same operands."""
+    la, lb, sq = batch_shapes()
+    a, b = batch_operands(la, lb, sq)
+    n = len(la)
+    batch = ntt.Batch()
+    assert batch.plan(la, lb, sq), ntt.last_error()
+    parts, pa, pb, ofs = [], [], [], 0
+    for i in range(n):
+        parts.append(a[i])
+        pa.append(ofs)
+        ofs += la[i]
+        if sq[i]:
+            pb.append(pa[-1])
+        else:
+            parts.append(b[i])
+            pb.append(ofs)
+            ofs += lb[i]
+    din = ntt.DeviceBuffer.from_host(np.concatenate(parts))
+    out_sizes = [batch.output_len(i) for i in range(n)]
+    out_ofs = np.concatenate([[0], np.cumsum(out_sizes)[:-1]]).astype(np.int64)
+    total_out = int(sum(out_sizes))
+    dout = ntt.DeviceBuffer(8 * total_out)
+    assert batch.bind([din.ptr + 8 * x for x in pa], [din.ptr + 8 * x for x in pb],
+                      [dout.ptr + 8 * int(x) for x in out_ofs]), ntt.last_error()
+    t_tune = time.perf_counter()
+    assert batch.run(), ntt.last_error()  # autotunes and captures the graph
+    assert ntt.synchronize()
+    tune_s = time.perf_counter() - t_tune
+    for _ in range(warmup):
+        assert batch.run()
+    assert ntt.synchronize()
+    # Per-kernel-kind times (the batch runs its launches eagerly while
+    # profiling, with events around each launch).
+    ntt.Profile.reset()
+    ntt.Profile.enable(True)
+    assert batch.run()
+    assert ntt.synchronize()
+    ntt.Profile.enable(False)
+    prof = ntt.Profile.read()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if not batch.run():
+            raise SystemExit("batch run failed: " + ntt.last_error())
+    assert ntt.synchronize()
+    batched_s = (time.perf_counter() - t0) / steps
+    got = dout.download(total_out)
+    # Plain per-call loop over the same operands and outputs.
+    assert batch.run_plain(), ntt.last_error()
+    assert ntt.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(plain_steps):
+        if not batch.run_plain():
+            raise SystemExit("plain loop failed: " + ntt.last_error())
+    assert ntt.synchronize()
+    plain_s = (time.perf_counter() - t1) / plain_steps
+    plain = dout.download(total_out)
+    assert (got == plain).all(), "batched results differ from the plain per-call path"
+    for i in range(0, n, 97):
+        ra = mod_q61(a[i])
+        rb = ra if sq[i] else mod_q61(b[i])
+        seg = got[int(out_ofs[i]):int(out_ofs[i]) + out_sizes[i]]
+        assert mod_q61(seg) == ra * rb % ((1 << 61) - 1), f"batch product {i} failed its modular check"
+    limbs = int(sum(la) + sum(y for y, s in zip(lb, sq) if not s))
+    return {
+        "products": n,
+        "products_per_s": round(n / batched_s, 1),
+        "ms_per_batch": round(1000 * batched_s, 3),
+        "plain_products_per_s": round(n / plain_s, 1),
+        "plain_ms_per_batch": round(1000 * plain_s, 3),
+        "speedup_vs_plain": round(plain_s / batched_s, 2),
+        "operand_limbs": limbs,
+        "Mlimbs_per_s": round(limbs / batched_s / 1e6, 1),
+        "graph_launches": batch.launches(),
+        "first_run_s": round(tune_s, 3),
+        "kernels_ms": {k: round(v[0], 3) for k, v in prof.items()},
+        "squares": int(sum(sq)),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +234,11 @@ def main() -> int:
     ap.add_argument("--workload", default="mul_2p20")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="after the timed region, run the plain copy kernel over this many bytes "
+                         "(the calibration reference for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -135,6 +248,15 @@ def main() -> int:
     if not ntt.device_available():
         print("bench.py: no gfx950 device", file=sys.stderr)
         return 1
+
+    if args.workload == "batch4096":
+        res = bench_batch(args.steps, args.warmup, max(1, args.steps // 5))
+        print(json.dumps({"metric": "batched products/s (4096 uneven products)", "value": res["products_per_s"],
+                          "unit": "products/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": res["ms_per_batch"], "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f64", "data": "synthetic (SplitMix64 limbs)",
+                          "config": {"workload": "batch4096"}, "batched": res}), flush=True)
+        return 0
 
     wl = workload(args.workload)
     a = np.ascontiguousarray(wl["a"])
@@ -194,6 +316,13 @@ def main() -> int:
     assert ntt.synchronize()
     clean = time.perf_counter() - t1
 
+    if args.copy_bytes:
+        n = (args.copy_bytes // 16) * 16
+        src, dst = ntt.DeviceBuffer(n), ntt.DeviceBuffer(n)
+        assert ntt.load().ntt_debug_copy(dst.ptr, src.ptr, n)
+        src.free()
+        dst.free()
+
     # Result check on the last step: the product modulo a 61-bit prime.
     got = dout.download(out_len)
     if kind != "conv":
@@ -214,13 +343,15 @@ def main() -> int:
                             "GBps": round(nbytes / (avg_ms * 1e-3) / 1e9, 1) if nbytes else None}
         if ms > dom_total and nbytes:
             dominant, dom_total = name, ms
+    traffic = measured_traffic(args.workload)
     roofline = None
     if dominant:
         d = per_kernel[dominant]
         nbytes = kernel_bytes(dominant, primes, T, la, lb, out_len, kind == "sqr")
         achieved = nbytes / (d["avg_us"] * 1e-6) / 1e9
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic.get(dominant),
                     "algorithmic_bytes": nbytes}
 
     unit = "Mcoeff/s" if kind == "conv" else "Mlimbs/s"
@@ -244,6 +375,8 @@ def main() -> int:
         "kernels": per_kernel,
         "roofline": roofline,
     }
+    if not args.no_batch:
+        result["batched"] = bench_batch(args.batch_steps, 2, 2)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     print(json.dumps(result), flush=True)

@@ -165,6 +165,59 @@ bool ntt_conv_workspace_convolve_device(ntt_conv_workspace *ws, uint64_t *d_dst,
                                         size_t b_len, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Batched products (no counterpart in the reference, which multiplies one
+ * pair per call: MulWorkspace::mul, src/plan.rs:164-170, is what each batch
+ * entry computes). A batch is planned once for a list of shapes; operands are
+ * bound as device pointers and may be re-bound for every run. A run queues
+ * one hipGraph replay that computes every product of the batch: one launch
+ * per transform pass and length bucket over all (product, prime) items, and
+ * one reconstruction launch per prime count over all (product, block) items.
+ * The first run times candidate pass splits for each transform length
+ * (autotuning, process-wide) and captures the graph. One batch handle must not
+ * be used by two threads at once. */
+
+typedef struct ntt_batch ntt_batch;
+
+ntt_batch *ntt_batch_new(void);
+void ntt_batch_free(ntt_batch *batch);
+
+/* Plans `count` products: product i is a_i * b_i with len_a[i] and len_b[i]
+ * limbs, or a_i squared when square != NULL and square[i] != 0 (len_b may
+ * then be NULL or hold anything at i). Allocates the device arenas. false on
+ * a zero length or a shape beyond the catalog (ntt_last_error says which). */
+bool ntt_batch_plan(ntt_batch *batch, size_t count, const size_t *len_a, const size_t *len_b,
+                    const uint8_t *square);
+
+/* Number of planned products, and the limbs product i writes (len_a+len_b). */
+size_t ntt_batch_count(const ntt_batch *batch);
+size_t ntt_batch_output_len(const ntt_batch *batch, size_t i);
+
+/* Binds device operands and outputs: arrays of `count` device pointers.
+ * d_b may be NULL when every product is a square; d_b[i] is ignored for a
+ * square. Output i receives exactly ntt_batch_output_len(i) limbs and must
+ * not overlap any operand or other output. */
+bool ntt_batch_bind(ntt_batch *batch, const uint64_t *const *d_a, const uint64_t *const *d_b,
+                    uint64_t *const *d_out);
+
+/* Queues every product of the batch on `stream` (NULL: the batch's own
+ * stream) as one graph launch. Synchronize the stream before reading. */
+bool ntt_batch_run(ntt_batch *batch, void *stream);
+
+/* The same products as a plain loop of one ntt_mul_workspace_mul_device (or
+ * _sqr_device) call per product on `stream`: the per-call path a batch
+ * replaces, for comparison. */
+bool ntt_batch_run_plain(ntt_batch *batch, void *stream);
+
+/* Kernel launches in the batch's graph. */
+int ntt_batch_launches(const ntt_batch *batch);
+
+/* Host-operand convenience: copies every a[i] (and b[i] unless a square) to
+ * device staging, binds, runs, and writes each product to out[i] (exactly
+ * ntt_batch_output_len(i) limbs). Synchronous. */
+bool ntt_batch_mul_host(ntt_batch *batch, const uint64_t *const *a, const uint64_t *const *b,
+                        uint64_t *const *out);
+
+/* ------------------------------------------------------------------------ */
 /* Engine utilities (no counterpart in the reference).                       */
 
 /* 1 when a gfx950 device is usable, 0 otherwise. Never initializes more than

@@ -5,6 +5,7 @@ ConvWorkspace: exact big-integer products, squares and dense convolutions,
 computed by hand-written HIP kernels behind the C ABI of include/nttanafis.h.
 """
 from .api import (  # noqa: F401
+    Batch,
     ConvPlan,
     ConvWorkspace,
     DeviceBuffer,
@@ -19,6 +20,7 @@ from .api import (  # noqa: F401
 from ._ffi import LIB_PATH, load  # noqa: F401
 
 __all__ = [
+    "Batch",
     "MulPlan",
     "MulWorkspace",
     "ConvPlan",

@@ -87,6 +87,27 @@ SIGNATURES = {
         ctypes.c_bool,
         [ctypes.c_void_p, ctypes.c_void_p, size_t, ctypes.c_void_p, size_t, ctypes.c_void_p, size_t, ctypes.c_void_p],
     ),
+    "ntt_batch_new": (ctypes.c_void_p, []),
+    "ntt_batch_free": (None, [ctypes.c_void_p]),
+    "ntt_batch_plan": (
+        ctypes.c_bool,
+        [ctypes.c_void_p, size_t, ctypes.POINTER(size_t), ctypes.POINTER(size_t), ctypes.POINTER(ctypes.c_uint8)],
+    ),
+    "ntt_batch_count": (size_t, [ctypes.c_void_p]),
+    "ntt_batch_output_len": (size_t, [ctypes.c_void_p, size_t]),
+    "ntt_batch_bind": (
+        ctypes.c_bool,
+        [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+         ctypes.POINTER(ctypes.c_void_p)],
+    ),
+    "ntt_batch_run": (ctypes.c_bool, [ctypes.c_void_p, ctypes.c_void_p]),
+    "ntt_batch_run_plain": (ctypes.c_bool, [ctypes.c_void_p, ctypes.c_void_p]),
+    "ntt_batch_launches": (ctypes.c_int, [ctypes.c_void_p]),
+    "ntt_batch_mul_host": (
+        ctypes.c_bool,
+        [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+         ctypes.POINTER(ctypes.c_void_p)],
+    ),
     "ntt_device_available": (ctypes.c_int, []),
     "ntt_device_alloc": (ctypes.c_void_p, [size_t]),
     "ntt_device_free": (None, [ctypes.c_void_p]),

@@ -233,6 +233,87 @@ class ConvWorkspace:
         return bool(_lib().ntt_conv_workspace_convolve_device(self._h, d_dst, dst_len, d_a, a_len, d_b, b_len, stream))
 
 
+def _ptr_array(ptrs):
+    arr = (ctypes.c_void_p * max(len(ptrs), 1))()
+    for i, v in enumerate(ptrs):
+        arr[i] = v
+    return arr
+
+
+class Batch:
+    """Many independent products of uneven size computed by one graph launch.
+
+    Each entry is what ``MulWorkspace.mul`` (src/plan.rs:164-170) computes for
+    one pair, or ``sqr`` for a square. ``plan`` fixes the shapes; ``bind``
+    sets device operands (re-bindable); ``run`` queues one graph replay.
+    ``mul_host`` is the synchronous host-array form.
+    """
+
+    def __init__(self):
+        self._h = _lib().ntt_batch_new()
+        if not self._h:
+            raise MemoryError("ntt_batch_new failed")
+        self._count = 0
+        self._square = []
+
+    def close(self):
+        if self._h:
+            _lib().ntt_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def plan(self, len_a, len_b, square=None) -> bool:
+        n = len(len_a)
+        la = (_ffi.size_t * max(n, 1))(*[int(x) for x in len_a])
+        lb = (_ffi.size_t * max(n, 1))(*[int(x) for x in len_b])
+        sq = None
+        if square is not None:
+            sq = (ctypes.c_uint8 * max(n, 1))(*[1 if s else 0 for s in square])
+        ok = bool(_lib().ntt_batch_plan(self._h, n, la, lb, sq))
+        if ok:
+            self._count = n
+            self._square = [bool(s) for s in square] if square is not None else [False] * n
+        return ok
+
+    def count(self) -> int:
+        return int(_lib().ntt_batch_count(self._h))
+
+    def output_len(self, i: int) -> int:
+        return int(_lib().ntt_batch_output_len(self._h, i))
+
+    def launches(self) -> int:
+        return int(_lib().ntt_batch_launches(self._h))
+
+    def bind(self, d_a, d_b, d_out) -> bool:
+        return bool(_lib().ntt_batch_bind(self._h, _ptr_array(d_a), _ptr_array(d_b), _ptr_array(d_out)))
+
+    def run(self, stream: Optional[int] = None) -> bool:
+        return bool(_lib().ntt_batch_run(self._h, stream))
+
+    def run_plain(self, stream: Optional[int] = None) -> bool:
+        return bool(_lib().ntt_batch_run_plain(self._h, stream))
+
+    def mul_host(self, a, b, out) -> bool:
+        """a, b, out: lists of uint64 arrays (b[i] ignored for squares);
+        out[i] must hold output_len(i) limbs."""
+        for i in range(self._count):
+            _limbs(a[i])
+            _limbs(out[i], writable=True)
+            if len(out[i]) < self.output_len(i):
+                return False
+            if not self._square[i]:
+                _limbs(b[i])
+        pa = _ptr_array([x.ctypes.data for x in a])
+        pb = _ptr_array([(x.ctypes.data if x is not None else None) for x in b])
+        po = _ptr_array([x.ctypes.data for x in out])
+        return bool(_lib().ntt_batch_mul_host(self._h, pa, pb, po))
+
+
 class DeviceBuffer:
     """Device memory owned by the engine's allocator (hipMalloc)."""
 

@@ -55,6 +55,34 @@ struct Source {
     uint32_t pad;
 };
 
+// One product of a batch, as the batched kernels read it from device memory.
+struct BatchProduct {
+    Source src_a, src_b;
+    double* spec_a;          // primes rows of T doubles
+    double* spec_b;          // unused for a square
+    uint64_t* out;           // result limbs
+    uint64_t out_len;
+    uint64_t coefficients;   // convolution coefficients reconstructed
+    uint64_t row_stride;     // T
+    uint32_t primes;
+    uint32_t square;
+    uint32_t width;
+    uint32_t width_magic;
+    uint32_t value_bits;
+    uint32_t blocks;         // carry blocks of the output
+    uint32_t* carry;         // 4 * blocks: base, thresh, first thresh, ripple end
+};
+
+// Work items of a batched launch: product index and prime (passes) or
+// product index and carry block (reconstruction).
+struct BatchItem {
+    uint32_t product;
+    uint32_t sub;
+};
+
+// Which rows of a batched product a pass works on.
+enum BatchOperand : int32_t { kOperandA = 0, kOperandB = 1, kOperandProduct = 2 };
+
 // Arguments of every transform pass kernel.
 struct PassArgs {
     double* data;            // rows of one operand, prime y at data + y * row_stride
@@ -73,6 +101,13 @@ struct PassArgs {
     Field field;
     double tinv[kMaxPrimes];                 // centered T^-1 mod p
     double powers[kMaxPrimes][kMaxParts];    // centered 2^(49k) mod p
+    // Batched launches: items[blockIdx.x / tiles_per_item] names the product
+    // and prime of a workgroup; batch == nullptr for a single operation.
+    const BatchProduct* batch;
+    const BatchItem* items;
+    uint64_t tiles_per_item;
+    int32_t operand;                         // BatchOperand
+    int32_t pad1;
 };
 
 // Arguments of the CRT and carry kernels.
@@ -100,6 +135,10 @@ struct CrtArgs {
     uint32_t* ripple_end;    // last limb index (relative) reached by a ripple
     uint32_t* carry_in;      // carry into each block, from the scan
     uint32_t blocks;
+    uint32_t pad;
+    // Batched reconstruction: items[blockIdx.x] names product and block.
+    const BatchProduct* batch;
+    const BatchItem* items;
 };
 
 // Root-table generation arguments.

@@ -142,25 +142,24 @@ static std::atomic<int> g_prof_on{0};
 static std::vector<ProfEntry> g_prof_pending;
 static std::vector<std::pair<std::string, std::pair<double, uint64_t>>> g_prof_totals;
 
+static thread_local const char* t_kind = "kernel";
+
+// Names the launches of a scope for per-launch timing (the events themselves
+// are recorded around each dispatch by the launchers in kernels.hip).
 struct Timed {
-    const char* name;
-    hipStream_t s;
-    hipEvent_t a = nullptr, b = nullptr;
-    bool on;
-    Timed(const char* n, hipStream_t st) : name(n), s(st), on(g_prof_on.load() != 0) {
-        if (on && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
-            hipEventRecord(a, s);
-        } else {
-            on = false;
-        }
-    }
-    ~Timed() {
-        if (!on) return;
-        hipEventRecord(b, s);
-        std::lock_guard<std::mutex> g(g_prof_lock);
-        g_prof_pending.push_back({name, a, b});
-    }
+    const char* saved;
+    Timed(const char* n, hipStream_t) : saved(t_kind) { t_kind = n; }
+    ~Timed() { t_kind = saved; }
 };
+
+static thread_local bool t_capturing = false;
+
+bool prof_enabled() { return g_prof_on.load() != 0 && !t_capturing; }
+const char* prof_kind() { return t_kind; }
+void prof_record(const char* kind, hipEvent_t start, hipEvent_t stop) {
+    std::lock_guard<std::mutex> g(g_prof_lock);
+    g_prof_pending.push_back({kind, start, stop});
+}
 
 static void prof_drain() {
     std::lock_guard<std::mutex> g(g_prof_lock);
@@ -752,6 +751,472 @@ static bool conv_host(Executor& ex, const ntt_conv_plan* plan, uint64_t* dst, si
 }  // namespace nttamd
 
 namespace nttamd {
+
+// ---------------------------------------------------------------------------
+// Batched products: many uneven products in one graph launch.
+//
+// Products are bucketed by transform length. Every pass of a bucket is one
+// launch over all (product, prime) items of the bucket, and the
+// reconstruction is one launch per prime count over all (product, block)
+// items. Pointers and shapes of the products live in a device array that the
+// kernels read, so the captured graph serves any operands bound later. The
+// split of each bucket's transform into passes comes from an autotuner that
+// times the candidate splits on the batch's own data the first time a
+// transform length is run.
+
+static std::mutex g_tune_lock;
+static int g_tuned_row_log[kMaxLog + 1] = {0};  // 0: not tuned yet
+
+static Split split_with_rows(int K, int rows) {
+    Split s{};
+    if (K <= kSinglePassMaxLog) {
+        s.n = 1;
+        s.logs[0] = K;
+        return s;
+    }
+    if (K <= 23) {
+        s.n = 2;
+        s.logs[0] = K - rows;
+        s.logs[1] = rows;
+        return s;
+    }
+    const int k1 = (K - rows + 1) / 2;
+    s.n = 3;
+    s.logs[0] = k1;
+    s.logs[1] = K - rows - k1;
+    s.logs[2] = rows;
+    return s;
+}
+
+// Row-pass sizes whose strided passes stay within 5..11 levels, largest
+// first (at most four).
+static std::vector<int> row_candidates(int K) {
+    std::vector<int> c;
+    if (K <= kSinglePassMaxLog) return c;
+    for (int r = 12; r >= 5 && c.size() < 4; --r) {
+        const int rest = K - r;
+        if (K <= 23) {
+            if (rest < 5 || rest > 11) continue;
+        } else {
+            const int k1 = (rest + 1) / 2;
+            if (k1 > 11 || rest - k1 < 5) continue;
+        }
+        c.push_back(r);
+    }
+    return c;
+}
+
+static Split tuned_split(int K) {
+    std::lock_guard<std::mutex> g(g_tune_lock);
+    const int rows = g_tuned_row_log[K];
+    return rows ? split_with_rows(K, rows) : split_of(K);
+}
+
+struct BatchLaunch {
+    enum Kind { kFwdStrided, kFwdRows, kFused, kInvStrided, kCrt } kind;
+    int logl;
+    bool flag;  // pack (forward strided), canonical output (inverse strided), square (fused)
+    bool pack;  // fused single pass: pack the operands
+    int units;
+    const char* name;
+    PassArgs pass;
+    CrtArgs crt;
+};
+
+struct BatchBucket {
+    int K;
+    uint32_t all, n_all;  // every (product, prime) item of this length
+    uint32_t ns, n_ns;    // items of products with two operands
+    uint32_t sq, n_sq;    // items of squares
+};
+
+struct Batch {
+    Executor ex;
+    size_t count = 0;
+    std::vector<uint64_t> la, lb;
+    std::vector<uint8_t> square;
+    std::vector<Geometry> geo;
+    std::vector<BatchProduct> host;
+    std::vector<BatchBucket> buckets;
+    uint32_t crt_begin[kMaxPrimes + 1] = {0}, crt_count[kMaxPrimes + 1] = {0};
+    DevBuf products, items, arena, carry;
+    std::vector<BatchLaunch> program;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool bound = false;
+    bool tuned = false;
+    const double *fwd = nullptr, *inv = nullptr;
+    uint64_t tstride = 0;
+
+    ~Batch() { drop_graph(); }
+    void drop_graph() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+static bool issue_launch(const BatchLaunch& L, hipStream_t s) {
+    Timed t(L.name, s);
+    switch (L.kind) {
+        case BatchLaunch::kFwdStrided: return launch_fwd_strided(L.logl, L.pass, L.units, L.flag, s);
+        case BatchLaunch::kFwdRows: return launch_fwd_rows(L.logl, L.pass, L.units, s);
+        case BatchLaunch::kFused: return launch_fused(L.logl, L.pass, L.units, L.pack, L.flag, s);
+        case BatchLaunch::kInvStrided: return launch_inv_strided(L.logl, L.pass, L.units, L.flag, s);
+        case BatchLaunch::kCrt: return launch_crt_product(L.crt, s);
+    }
+    return false;
+}
+
+static bool issue_program(const std::vector<BatchLaunch>& prog, hipStream_t s) {
+    for (const BatchLaunch& L : prog)
+        if (!issue_launch(L, s)) return false;
+    return true;
+}
+
+// Appends the transform launches of one bucket for the given split: forward
+// passes of every first operand, forward passes of every second operand, the
+// fused row pass (forward, pointwise product, inverse) and the inverse
+// strided passes, the last of which leaves canonical residues.
+static void bucket_program(std::vector<BatchLaunch>& prog, const PassArgs& base, int K, const Split& sp,
+                           const BatchItem* I, const BatchBucket& r, bool forward_a_only) {
+    auto pass = [&](BatchLaunch::Kind kind, int logl, bool flag, bool pack, int level, uint32_t first,
+                    uint32_t units, int operand, const char* name) {
+        if (units == 0) return;
+        BatchLaunch L;
+        memset(&L, 0, sizeof L);
+        L.kind = kind;
+        L.logl = logl;
+        L.flag = flag;
+        L.pack = pack;
+        L.units = (int)units;
+        L.name = name;
+        L.pass = base;
+        L.pass.items = I + first;
+        L.pass.operand = operand;
+        L.pass.level_start = level;
+        L.pass.line_stride = kind == BatchLaunch::kFwdRows || kind == BatchLaunch::kFused
+                                 ? 1
+                                 : (uint64_t)1 << (K - level - logl);
+        prog.push_back(L);
+    };
+    if (sp.n == 1) {
+        if (forward_a_only) return;
+        pass(BatchLaunch::kFused, K, false, true, 0, r.ns, r.n_ns, kOperandProduct, "batch_fused_single");
+        pass(BatchLaunch::kFused, K, true, true, 0, r.sq, r.n_sq, kOperandProduct, "batch_fused_single");
+        return;
+    }
+    const int row_log = sp.logs[sp.n - 1];
+    const int row_level = K - row_log;
+    int level = 0;
+    for (int i = 0; i + 1 < sp.n; ++i) {
+        pass(BatchLaunch::kFwdStrided, sp.logs[i], i == 0, false, level, r.all, r.n_all, kOperandA,
+             i == 0 ? "batch_fwd_pack_strided" : "batch_fwd_strided");
+        level += sp.logs[i];
+    }
+    pass(BatchLaunch::kFwdRows, row_log, false, false, row_level, r.ns, r.n_ns, kOperandA, "batch_fwd_rows");
+    if (forward_a_only) return;
+    level = 0;
+    for (int i = 0; i + 1 < sp.n; ++i) {
+        pass(BatchLaunch::kFwdStrided, sp.logs[i], i == 0, false, level, r.ns, r.n_ns, kOperandB,
+             i == 0 ? "batch_fwd_pack_strided" : "batch_fwd_strided");
+        level += sp.logs[i];
+    }
+    pass(BatchLaunch::kFused, row_log, false, false, row_level, r.ns, r.n_ns, kOperandProduct, "batch_fused_rows");
+    pass(BatchLaunch::kFused, row_log, true, false, row_level, r.sq, r.n_sq, kOperandProduct, "batch_fused_rows");
+    level = row_level;
+    for (int i = sp.n - 2; i >= 0; --i) {
+        level -= sp.logs[i];
+        pass(BatchLaunch::kInvStrided, sp.logs[i], i == 0, false, level, r.all, r.n_all, kOperandProduct,
+             i == 0 ? "batch_inv_strided_canon" : "batch_inv_strided");
+    }
+}
+
+static PassArgs bucket_base(const Batch& B, int K) {
+    Geometry gK;
+    gK.log_t = K;
+    gK.primes = kMaxPrimes;
+    PassArgs base = base_args(gK, nullptr, B.fwd, B.inv, B.tstride);
+    base.batch = B.products.as<BatchProduct>();
+    return base;
+}
+
+static void build_program(Batch& B) {
+    B.program.clear();
+    const BatchItem* I = B.items.as<BatchItem>();
+    for (const BatchBucket& r : B.buckets)
+        bucket_program(B.program, bucket_base(B, r.K), r.K, tuned_split(r.K), I, r, false);
+    for (int np = 1; np <= kMaxPrimes; ++np) {
+        if (!B.crt_count[np]) continue;
+        BatchLaunch L;
+        memset(&L, 0, sizeof L);
+        L.kind = BatchLaunch::kCrt;
+        L.name = "batch_crt";
+        Geometry gN;
+        gN.primes = np;
+        L.crt = crt_args(gN, nullptr, nullptr, 0);
+        L.crt.batch = B.products.as<BatchProduct>();
+        L.crt.items = I + B.crt_begin[np];
+        L.crt.blocks = B.crt_count[np];
+        B.program.push_back(L);
+    }
+}
+
+static bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, const uint8_t* sq) {
+    B.drop_graph();
+    B.bound = false;
+    B.tuned = false;
+    B.count = 0;
+    B.program.clear();
+    if (!B.ex.init()) {
+        set_error("batch: stream creation failed");
+        return false;
+    }
+    B.la.assign(count, 0);
+    B.lb.assign(count, 0);
+    B.square.assign(count, 0);
+    B.geo.assign(count, Geometry());
+    B.host.assign(count, BatchProduct());
+    uint64_t arena = 0, carry = 0, max_half = 0;
+    std::vector<uint64_t> off_a(count), off_b(count), off_c(count);
+    for (size_t i = 0; i < count; ++i) {
+        const bool square = sq && sq[i];
+        B.square[i] = square ? 1 : 0;
+        B.la[i] = la[i];
+        B.lb[i] = square ? la[i] : lb[i];
+        if (B.la[i] == 0 || B.lb[i] == 0 || !select_product(B.la[i], B.lb[i], &B.geo[i])) {
+            set_error("batch: product " + std::to_string(i) + " has an unsupported shape");
+            return false;
+        }
+        const Geometry& g = B.geo[i];
+        const uint64_t T = (uint64_t)1 << g.log_t;
+        max_half = std::max(max_half, T / 2);
+        off_a[i] = arena;
+        arena += (uint64_t)g.primes * T;
+        off_b[i] = arena;
+        if (!square) arena += (uint64_t)g.primes * T;
+        const uint64_t blocks = (B.la[i] + B.lb[i] + kCrtBlockLimbs - 1) / kCrtBlockLimbs;
+        off_c[i] = carry;
+        carry += 4 * blocks;
+    }
+    if (!ensure_tables(max_half, B.ex.stream, &B.fwd, &B.inv, &B.tstride)) {
+        set_error("batch: root table allocation failed");
+        return false;
+    }
+    if (!B.arena.ensure(std::max<uint64_t>(arena, 1) * 8) || !B.carry.ensure(std::max<uint64_t>(carry, 1) * 4) ||
+        !B.products.ensure(std::max<size_t>(count, 1) * sizeof(BatchProduct))) {
+        set_error("batch: device allocation failed");
+        return false;
+    }
+    double* A = B.arena.as<double>();
+    uint32_t* C = B.carry.as<uint32_t>();
+    for (size_t i = 0; i < count; ++i) {
+        const Geometry& g = B.geo[i];
+        const bool square = B.square[i] != 0;
+        BatchProduct& P = B.host[i];
+        memset(&P, 0, sizeof P);
+        P.src_a = source_of(nullptr, B.la[i], g.digits_a, g.width, g.value_bits_a, false);
+        P.src_b = source_of(nullptr, B.lb[i], g.digits_b, g.width, g.value_bits_b, false);
+        P.spec_a = A + off_a[i];
+        P.spec_b = square ? P.spec_a : A + off_b[i];
+        P.out_len = B.la[i] + B.lb[i];
+        P.coefficients = std::min<uint64_t>(g.conv_len, (P.out_len * 64 + g.width - 1) / g.width);
+        P.row_stride = (uint64_t)1 << g.log_t;
+        P.primes = g.primes;
+        P.square = square ? 1 : 0;
+        P.width = g.width;
+        P.width_magic = (uint32_t)(((1ull << 32) + g.width - 1) / g.width);
+        P.value_bits = g.bound_bits;
+        P.blocks = (uint32_t)((P.out_len + kCrtBlockLimbs - 1) / kCrtBlockLimbs);
+        P.carry = C + off_c[i];
+    }
+    // Work items: per transform length every (product, prime) pair, then the
+    // pairs of products with two operands and of squares; per prime count
+    // every (product, output block) pair.
+    std::vector<BatchItem> items;
+    B.buckets.clear();
+    auto add = [&](int K, int which) {
+        for (size_t i = 0; i < count; ++i) {
+            if (B.geo[i].log_t != K) continue;
+            if (which == 1 && B.square[i]) continue;
+            if (which == 2 && !B.square[i]) continue;
+            for (uint32_t y = 0; y < B.geo[i].primes; ++y) items.push_back({(uint32_t)i, y});
+        }
+    };
+    for (int K = kMinLog; K <= kMaxLog; ++K) {
+        BatchBucket r;
+        r.K = K;
+        r.all = (uint32_t)items.size();
+        add(K, 0);
+        r.n_all = (uint32_t)items.size() - r.all;
+        if (!r.n_all) continue;
+        r.ns = (uint32_t)items.size();
+        add(K, 1);
+        r.n_ns = (uint32_t)items.size() - r.ns;
+        r.sq = (uint32_t)items.size();
+        add(K, 2);
+        r.n_sq = (uint32_t)items.size() - r.sq;
+        B.buckets.push_back(r);
+    }
+    for (int np = 1; np <= kMaxPrimes; ++np) {
+        B.crt_begin[np] = (uint32_t)items.size();
+        for (size_t i = 0; i < count; ++i)
+            if ((int)B.geo[i].primes == np)
+                for (uint32_t blk = 0; blk < B.host[i].blocks; ++blk) items.push_back({(uint32_t)i, blk});
+        B.crt_count[np] = (uint32_t)items.size() - B.crt_begin[np];
+    }
+    if (!B.items.ensure(std::max<size_t>(items.size(), 1) * sizeof(BatchItem)) ||
+        (!items.empty() && hipMemcpy(B.items.ptr, items.data(), items.size() * sizeof(BatchItem),
+                                     hipMemcpyHostToDevice) != hipSuccess)) {
+        set_error("batch: work item upload failed");
+        return false;
+    }
+    B.count = count;
+    build_program(B);
+    return true;
+}
+
+static bool batch_bind(Batch& B, const uint64_t* const* d_a, const uint64_t* const* d_b, uint64_t* const* d_out) {
+    for (size_t i = 0; i < B.count; ++i) {
+        const uint64_t* pb = B.square[i] ? d_a[i] : d_b[i];
+        if (!d_a[i] || !pb || !d_out[i]) {
+            set_error("batch: null operand or output pointer at product " + std::to_string(i));
+            return false;
+        }
+        B.host[i].src_a.limbs = d_a[i];
+        B.host[i].src_b.limbs = pb;
+        B.host[i].out = d_out[i];
+    }
+    if (B.count && hipMemcpyAsync(B.products.ptr, B.host.data(), B.count * sizeof(BatchProduct),
+                                  hipMemcpyHostToDevice, B.ex.stream) != hipSuccess) {
+        set_error("batch: descriptor upload failed");
+        return false;
+    }
+    if (hipStreamSynchronize(B.ex.stream) != hipSuccess) {
+        set_error("batch: descriptor upload failed");
+        return false;
+    }
+    B.bound = true;
+    return true;
+}
+
+static float time_launches(const std::vector<BatchLaunch>& prog, hipStream_t s, int reps) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 1e30f;
+    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess && issue_program(prog, s)) {
+        hipEventRecord(e0, s);
+        bool ok = true;
+        for (int r = 0; r < reps && ok; ++r) ok = issue_program(prog, s);
+        hipEventRecord(e1, s);
+        if (ok && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    return ms;
+}
+
+// Times the forward passes of every first operand of each untuned bucket
+// under each candidate split and keeps the fastest. The operands must be
+// bound: the timed passes read them.
+static bool batch_autotune(Batch& B) {
+    const BatchItem* I = B.items.as<BatchItem>();
+    bool changed = false;
+    for (const BatchBucket& r : B.buckets) {
+        {
+            std::lock_guard<std::mutex> g(g_tune_lock);
+            if (g_tuned_row_log[r.K]) continue;
+        }
+        const std::vector<int> cand = row_candidates(r.K);
+        int best = cand.empty() ? r.K : cand[0];
+        if (cand.size() > 1) {
+            float best_ms = 1e30f;
+            for (int rows : cand) {
+                std::vector<BatchLaunch> fwd;
+                BatchBucket all = r;
+                all.ns = r.all;  // time the row pass over every item
+                all.n_ns = r.n_all;
+                bucket_program(fwd, bucket_base(B, r.K), r.K, split_with_rows(r.K, rows), I, all, true);
+                const float ms = time_launches(fwd, B.ex.stream, 3);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = rows;
+                }
+            }
+            if (hipStreamSynchronize(B.ex.stream) != hipSuccess) {
+                set_error("batch: autotuning run failed");
+                return false;
+            }
+        }
+        std::lock_guard<std::mutex> g(g_tune_lock);
+        if (!g_tuned_row_log[r.K]) {
+            g_tuned_row_log[r.K] = best;
+            changed = true;
+        }
+    }
+    B.tuned = true;
+    if (changed) {
+        B.drop_graph();
+        build_program(B);
+    }
+    return true;
+}
+
+static bool batch_capture(Batch& B) {
+    const hipStream_t s = B.ex.stream;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        set_error("batch: graph capture failed to start");
+        return false;
+    }
+    t_capturing = true;
+    const bool ok = issue_program(B.program, s);
+    t_capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (!ok || e != hipSuccess || !g) {
+        if (g) hipGraphDestroy(g);
+        set_error("batch: graph capture failed");
+        return false;
+    }
+    hipGraphExec_t x = nullptr;
+    if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+        hipGraphDestroy(g);
+        set_error("batch: graph instantiation failed");
+        return false;
+    }
+    B.graph = g;
+    B.exec = x;
+    return true;
+}
+
+// Queues the whole batch on `stream` (the batch's own stream when null):
+// autotunes on the first run, captures the program once, then replays it.
+static bool batch_run(Batch& B, hipStream_t stream, bool use_graph) {
+    if (!B.bound) {
+        set_error("batch: operands are not bound");
+        return false;
+    }
+    if (B.count == 0) return true;
+    if (!B.tuned && !batch_autotune(B)) return false;
+    const hipStream_t s = stream ? stream : B.ex.stream;
+    if (!use_graph || prof_enabled()) {
+        if (!issue_program(B.program, s)) {
+            set_error("batch: kernel launch failed");
+            return false;
+        }
+        return true;
+    }
+    if (!B.exec && !batch_capture(B)) return false;
+    if (hipGraphLaunch(B.exec, s) != hipSuccess) {
+        set_error("batch: graph launch failed");
+        return false;
+    }
+    return true;
+}
+
+}  // namespace nttamd
+
+namespace nttamd {
 // Host self-test of the arithmetic the kernels share with the host
 // (mulmod, reduce, packing, Garner, carry functions) against exact integer
 // arithmetic. Returns the number of failed checks.
@@ -844,6 +1309,14 @@ struct ntt_conv_workspace {
     Executor ex;
     ntt_conv_plan plan;
     bool has_plan = false;
+};
+
+struct ntt_batch {
+    Batch batch;
+    std::vector<const uint64_t*> d_a, d_b;
+    std::vector<uint64_t*> d_out;
+    DevBuf staging_in, staging_out;  // host-operand convenience call
+    ntt_mul_workspace* plain = nullptr;
 };
 
 extern "C" {
@@ -1057,6 +1530,139 @@ bool ntt_conv_workspace_convolve_device(ntt_conv_workspace* ws, uint64_t* d_dst,
     const bool ok = run_conv(ws->ex, &ctx, d_a, d_b, d_dst);
     ws->ex.stream = saved;
     return ok;
+}
+
+// Batched products.
+
+ntt_batch* ntt_batch_new(void) { return new (std::nothrow) ntt_batch(); }
+
+void ntt_batch_free(ntt_batch* b) {
+    if (!b) return;
+    if (b->plain) ntt_mul_workspace_free(b->plain);
+    delete b;
+}
+
+bool ntt_batch_plan(ntt_batch* b, size_t count, const size_t* len_a, const size_t* len_b, const uint8_t* square) {
+    t_error.clear();
+    if (!b || (count && (!len_a || (!len_b && !square)))) return false;
+    if (!len_b)
+        for (size_t i = 0; i < count; ++i)
+            if (!square[i]) return false;
+    std::vector<size_t> lb(count);
+    for (size_t i = 0; i < count; ++i) lb[i] = len_b ? len_b[i] : len_a[i];
+    std::lock_guard<std::mutex> guard(b->batch.ex.lock);
+    b->d_a.clear();
+    return batch_plan(b->batch, count, len_a, lb.data(), square);
+}
+
+size_t ntt_batch_count(const ntt_batch* b) { return b ? b->batch.count : 0; }
+
+size_t ntt_batch_output_len(const ntt_batch* b, size_t i) {
+    if (!b || i >= b->batch.count) return 0;
+    return b->batch.la[i] + b->batch.lb[i];
+}
+
+bool ntt_batch_bind(ntt_batch* b, const uint64_t* const* d_a, const uint64_t* const* d_b, uint64_t* const* d_out) {
+    t_error.clear();
+    if (!b || (b->batch.count && (!d_a || !d_out))) return false;
+    std::lock_guard<std::mutex> guard(b->batch.ex.lock);
+    const size_t n = b->batch.count;
+    std::vector<const uint64_t*> bb(n);
+    for (size_t i = 0; i < n; ++i) {
+        bb[i] = b->batch.square[i] ? d_a[i] : (d_b ? d_b[i] : nullptr);
+        if (!bb[i]) {
+            set_error("batch: missing second operand at product " + std::to_string(i));
+            return false;
+        }
+    }
+    if (!batch_bind(b->batch, d_a, bb.data(), d_out)) return false;
+    b->d_a.assign(d_a, d_a + n);
+    b->d_b = bb;
+    b->d_out.assign(d_out, d_out + n);
+    return true;
+}
+
+bool ntt_batch_run(ntt_batch* b, void* stream) {
+    t_error.clear();
+    if (!b) return false;
+    std::lock_guard<std::mutex> guard(b->batch.ex.lock);
+    return batch_run(b->batch, (hipStream_t)stream, true);
+}
+
+bool ntt_batch_run_plain(ntt_batch* b, void* stream) {
+    t_error.clear();
+    if (!b || !b->batch.bound) return false;
+    if (!b->plain) b->plain = ntt_mul_workspace_new();
+    if (!b->plain) return false;
+    for (size_t i = 0; i < b->d_a.size(); ++i) {
+        const uint64_t la = b->batch.la[i], lb = b->batch.lb[i];
+        const bool ok = b->batch.square[i]
+                            ? ntt_mul_workspace_sqr_device(b->plain, b->d_out[i], la + lb, b->d_a[i], la, stream)
+                            : ntt_mul_workspace_mul_device(b->plain, b->d_out[i], la + lb, b->d_a[i], la,
+                                                           b->d_b[i], lb, stream);
+        if (!ok) return false;
+    }
+    return true;
+}
+
+int ntt_batch_launches(const ntt_batch* b) { return b ? (int)b->batch.program.size() : 0; }
+
+bool ntt_batch_mul_host(ntt_batch* b, const uint64_t* const* a, const uint64_t* const* bops, uint64_t* const* out) {
+    t_error.clear();
+    if (!b) return false;
+    const size_t n = b->batch.count;
+    if (n == 0) return true;
+    if (!a || !out) return false;
+    uint64_t in_limbs = 0, out_limbs = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (!a[i] || !out[i] || (!b->batch.square[i] && (!bops || !bops[i]))) return false;
+        in_limbs += b->batch.la[i] + (b->batch.square[i] ? 0 : b->batch.lb[i]);
+        out_limbs += b->batch.la[i] + b->batch.lb[i];
+    }
+    if (!b->staging_in.ensure(in_limbs * 8) || !b->staging_out.ensure(out_limbs * 8)) {
+        set_error("batch: staging allocation failed");
+        return false;
+    }
+    std::vector<const uint64_t*> da(n), db(n);
+    std::vector<uint64_t*> dout(n);
+    std::vector<uint64_t> host_in(in_limbs);
+    uint64_t* din = b->staging_in.as<uint64_t>();
+    uint64_t* dst = b->staging_out.as<uint64_t>();
+    uint64_t ofs = 0, oofs = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t la = b->batch.la[i], lb = b->batch.lb[i];
+        memcpy(&host_in[ofs], a[i], la * 8);
+        da[i] = din + ofs;
+        ofs += la;
+        if (b->batch.square[i]) {
+            db[i] = da[i];
+        } else {
+            memcpy(&host_in[ofs], bops[i], lb * 8);
+            db[i] = din + ofs;
+            ofs += lb;
+        }
+        dout[i] = dst + oofs;
+        oofs += la + lb;
+    }
+    if (hipMemcpy(din, host_in.data(), in_limbs * 8, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("batch: operand upload failed");
+        return false;
+    }
+    if (!ntt_batch_bind(b, da.data(), db.data(), dout.data())) return false;
+    if (!ntt_batch_run(b, nullptr)) return false;
+    std::vector<uint64_t> host_out(out_limbs);
+    if (hipStreamSynchronize(b->batch.ex.stream) != hipSuccess ||
+        hipMemcpy(host_out.data(), dst, out_limbs * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("batch: device execution failed");
+        return false;
+    }
+    oofs = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t len = b->batch.la[i] + b->batch.lb[i];
+        memcpy(out[i], &host_out[oofs], len * 8);
+        oofs += len;
+    }
+    return true;
 }
 
 // Utilities.

@@ -17,7 +17,29 @@
 // instead of reading a full-length table.
 #include "arith.h"
 
+#include <hip/hip_ext.h>
+
 namespace nttamd {
+
+// Per-launch timing hooks (engine.hip). While profiling is enabled every
+// launch goes through hipExtLaunchKernelGGL, whose start and stop events
+// bracket exactly that dispatch on its stream.
+bool prof_enabled();
+const char* prof_kind();
+void prof_record(const char* kind, hipEvent_t start, hipEvent_t stop);
+
+#define NTT_LAUNCH(KIND, KERNEL, GRID, BLOCK, STREAM, ...)                                 \
+    do {                                                                                   \
+        hipEvent_t e0_ = nullptr, e1_ = nullptr;                                           \
+        if (prof_enabled() && hipEventCreate(&e0_) == hipSuccess &&                        \
+            hipEventCreate(&e1_) == hipSuccess) {                                          \
+            hipExtLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, STREAM, e0_, e1_, 0, __VA_ARGS__); \
+            const char* k_ = (KIND);                                                       \
+            prof_record(k_ ? k_ : prof_kind(), e0_, e1_);                                  \
+        } else {                                                                           \
+            hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, STREAM, __VA_ARGS__);              \
+        }                                                                                  \
+    } while (0)
 
 __device__ __forceinline__ int pidx(int e) { return e + (e >> 5); }
 
@@ -305,13 +327,24 @@ __device__ __forceinline__ void store_rows(const double* lds, double* g) {
 }
 
 // ---------------------------------------------------------------------------
-// Strided passes: the tile is LINES consecutive lines lo0..lo0+LINES-1 of
-// first node H; element m of line lo sits at H*(T >> level_start) + m*S + lo.
+// Work resolution. A single operation takes its prime from blockIdx.y and its
+// tile from blockIdx.x; a batched launch reads the product and prime of its
+// workgroup from the item table, so one launch covers every product of a
+// size bucket.
 
-// Workgroup b of a packing pass runs on XCD b % 8 (round-robin dispatch).
-// Tiles are dealt to XCDs so that the workgroups of all primes for one tile
-// are consecutive on one XCD: they read the same limbs, which the first one
-// brings into that XCD's L2.
+struct Job {
+    double* data;          // rows of the operand this pass transforms
+    const double* other;   // fused passes: the other operand's spectra
+    Source src_a;          // packing sources, held by value so that the
+    Source src_b;          // packing loops never reload them
+    int y;                 // prime
+    uint64_t tile;
+};
+
+// Workgroup b of a single packing pass runs on XCD b % 8 (round-robin
+// dispatch). Tiles are dealt to XCDs so that the workgroups of all primes for
+// one tile are consecutive on one XCD: they read the same limbs, which the
+// first one brings into that XCD's L2.
 __device__ __forceinline__ bool xcd_tile(int primes, uint64_t tiles, uint64_t* tile, int* y) {
     const uint64_t b = blockIdx.x;
     const uint64_t slot = b >> 3;
@@ -321,25 +354,57 @@ __device__ __forceinline__ bool xcd_tile(int primes, uint64_t tiles, uint64_t* t
     return t < tiles;
 }
 
-template <int LOGL, int LINES, int NT, bool PACK>
+template <bool BATCH>
+__device__ __forceinline__ bool resolve(const PassArgs& a, uint64_t tiles, bool xcd, Job* j) {
+    if constexpr (!BATCH) {
+        j->data = a.data;
+        j->other = a.other;
+        j->src_a = a.src_a;
+        j->src_b = a.src_b;
+        if (xcd) return xcd_tile(a.primes, tiles, &j->tile, &j->y);
+        j->tile = blockIdx.x;
+        j->y = blockIdx.y;
+        return true;
+    }
+    const BatchItem it = a.items[blockIdx.x / a.tiles_per_item];
+    const BatchProduct& P = a.batch[it.product];
+    j->tile = blockIdx.x % a.tiles_per_item;
+    j->y = (int)it.sub;
+    j->src_a = P.src_a;
+    j->src_b = P.src_b;
+    j->other = P.spec_a;
+    if (a.operand == kOperandA) {
+        j->data = P.spec_a;
+    } else if (a.operand == kOperandB) {
+        j->data = P.spec_b;
+        j->src_a = P.src_b;
+    } else {
+        j->data = P.square ? P.spec_a : P.spec_b;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Strided passes: the tile is LINES consecutive lines lo0..lo0+LINES-1 of
+// first node H; element m of line lo sits at H*(T >> level_start) + m*S + lo.
+
+template <int LOGL, int LINES, int NT, bool PACK, bool BATCH>
 __global__ __launch_bounds__(NT) void k_fwd_strided(PassArgs a) {
     constexpr int LS = Lay<LOGL>::STRIDE;
     __shared__ double lds[LINES * LS];
     const uint64_t S = a.line_stride;
     const uint64_t tiles_per_node = S / LINES;
-    uint64_t tile = blockIdx.x;
-    int y = blockIdx.y;
-    if constexpr (PACK) {
-        if (!xcd_tile(a.primes, tiles_per_node << a.level_start, &tile, &y)) return;
-    }
+    Job j;
+    if (!resolve<BATCH>(a, tiles_per_node << a.level_start, PACK, &j)) return;
+    const int y = j.y;
     const double p = a.field.p[y], pinv = a.field.pinv[y];
-    const uint64_t H = tile / tiles_per_node;
-    const uint64_t lo0 = (tile % tiles_per_node) * LINES;
+    const uint64_t H = j.tile / tiles_per_node;
+    const uint64_t lo0 = (j.tile % tiles_per_node) * LINES;
     const uint64_t span = (uint64_t)1 << (a.log_t - a.level_start);
     const uint64_t base = H * span + lo0;
-    double* row = a.data + (uint64_t)y * a.row_stride;
+    double* row = j.data + (uint64_t)y * a.row_stride;
     if constexpr (PACK) {
-        pack_strided<LOGL, LINES, NT>(lds, a.src_a, base, S, a.powers[y], p, pinv);
+        pack_strided<LOGL, LINES, NT>(lds, j.src_a, base, S, a.powers[y], p, pinv);
     } else {
         load_strided<LOGL, LINES, NT>(lds, row + base, S);
     }
@@ -351,19 +416,21 @@ __global__ __launch_bounds__(NT) void k_fwd_strided(PassArgs a) {
 
 // Inverse strided pass; CANON (level_start == 0) multiplies by T^-1 and stores
 // canonical residues in [0, p).
-template <int LOGL, int LINES, int NT, bool CANON>
+template <int LOGL, int LINES, int NT, bool CANON, bool BATCH>
 __global__ __launch_bounds__(NT) void k_inv_strided(PassArgs a) {
     constexpr int LS = Lay<LOGL>::STRIDE;
     __shared__ double lds[LINES * LS];
-    const int y = blockIdx.y;
-    const double p = a.field.p[y], pinv = a.field.pinv[y];
     const uint64_t S = a.line_stride;
     const uint64_t tiles_per_node = S / LINES;
-    const uint64_t H = blockIdx.x / tiles_per_node;
-    const uint64_t lo0 = (blockIdx.x % tiles_per_node) * LINES;
+    Job j;
+    resolve<BATCH>(a, 0, false, &j);
+    const int y = j.y;
+    const double p = a.field.p[y], pinv = a.field.pinv[y];
+    const uint64_t H = j.tile / tiles_per_node;
+    const uint64_t lo0 = (j.tile % tiles_per_node) * LINES;
     const uint64_t span = (uint64_t)1 << (a.log_t - a.level_start);
     const uint64_t base = H * span + lo0;
-    double* row = a.data + (uint64_t)y * a.row_stride;
+    double* row = j.data + (uint64_t)y * a.row_stride;
     load_strided<LOGL, LINES, NT>(lds, row + base, S);
     __syncthreads();
     const Tw tw{a.inv + (uint64_t)y * a.table_stride, H, 0, 64, p, pinv};
@@ -378,15 +445,17 @@ __host__ __device__ constexpr int row_split(int logl) { return logl > 8 ? logl -
 // Row passes: rows H0..H0+LINES-1 of 2^LOGL contiguous points, the last
 // levels of the transform.
 
-template <int LOGL, int LINES, int NT>
+template <int LOGL, int LINES, int NT, bool BATCH>
 __global__ __launch_bounds__(NT) void k_fwd_rows(PassArgs a) {
     constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
     __shared__ double lds[LINES * LS];
-    const int y = blockIdx.y;
+    Job j;
+    resolve<BATCH>(a, 0, false, &j);
+    const int y = j.y;
     const double p = a.field.p[y], pinv = a.field.pinv[y];
-    const uint64_t H0 = (uint64_t)blockIdx.x * LINES;
-    double* row = a.data + (uint64_t)y * a.row_stride + H0 * L;
+    const uint64_t H0 = j.tile * LINES;
+    double* row = j.data + (uint64_t)y * a.row_stride + H0 * L;
     load_rows<LOGL, LINES, NT>(lds, row);
     __syncthreads();
     const Tw tw{a.fwd + (uint64_t)y * a.table_stride, H0, 1, row_split(LOGL), p, pinv};
@@ -398,7 +467,7 @@ __global__ __launch_bounds__(NT) void k_fwd_rows(PassArgs a) {
 // operand's spectrum (or a square), inverse levels, store. With PACK the row
 // is the whole transform (level_start == 0): both operands are packed from
 // limbs into LDS and the result is stored as canonical residues times T^-1.
-template <int LOGL, int NT, bool PACK, bool SQUARE>
+template <int LOGL, int NT, bool PACK, bool SQUARE, bool BATCH>
 __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
@@ -406,13 +475,15 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     constexpr int R = (L + NT - 1) / NT;
     constexpr bool FULL = L % NT == 0;
     __shared__ double lds[TILES * LS];
-    const int y = blockIdx.y;
+    Job j;
+    resolve<BATCH>(a, 0, false, &j);
+    const int y = j.y;
     const double p = a.field.p[y], pinv = a.field.pinv[y];
-    const uint64_t H = blockIdx.x;
-    double* row = a.data + (uint64_t)y * a.row_stride + H * L;
+    const uint64_t H = j.tile;
+    double* row = j.data + (uint64_t)y * a.row_stride + H * L;
     if constexpr (PACK) {
-        pack_strided<LOGL, 1, NT>(lds, a.src_a, 0, 1, a.powers[y], p, pinv);
-        if constexpr (!SQUARE) pack_strided<LOGL, 1, NT>(lds + LS, a.src_b, 0, 1, a.powers[y], p, pinv);
+        pack_strided<LOGL, 1, NT>(lds, j.src_a, 0, 1, a.powers[y], p, pinv);
+        if constexpr (!SQUARE) pack_strided<LOGL, 1, NT>(lds + LS, j.src_b, 0, 1, a.powers[y], p, pinv);
     } else {
         load_rows<LOGL, 1, NT>(lds, row);
     }
@@ -437,7 +508,7 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
             lds[pidx(m)] = mulmod(lds[pidx(m)], lds[LS + pidx(m)], p, pinv);
         }
     } else {
-        const double* other = a.other + (uint64_t)y * a.row_stride + H * L;
+        const double* other = j.other + (uint64_t)y * a.row_stride + H * L;
         double o[R];
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -480,6 +551,54 @@ __device__ __forceinline__ CarryFn shfl_up_fn(CarryFn f, int d) {
     return r;
 }
 
+// Reconstruction parameters of one workgroup: the single operation's, or
+// those of the batched product its item names.
+struct CrtJob {
+    const double* residues;
+    uint64_t row_stride, coefficients, out_len;
+    uint64_t* out;
+    uint32_t width, magic, value_bits, block, blocks;
+    uint32_t *agg_base, *agg_thresh, *first_thresh, *ripple_end;
+};
+
+__device__ __forceinline__ CrtJob crt_job(const CrtArgs& a) {
+    CrtJob j;
+    if (a.batch == nullptr) {
+        j.residues = a.residues;
+        j.row_stride = a.row_stride;
+        j.coefficients = a.coefficients;
+        j.out_len = a.out_len;
+        j.out = a.out;
+        j.width = a.width;
+        j.magic = a.width_magic;
+        j.value_bits = a.value_bits;
+        j.block = blockIdx.x;
+        j.blocks = a.blocks;
+        j.agg_base = a.agg_base;
+        j.agg_thresh = a.agg_thresh;
+        j.first_thresh = a.first_thresh;
+        j.ripple_end = a.ripple_end;
+        return j;
+    }
+    const BatchItem it = a.items[blockIdx.x];
+    const BatchProduct& P = a.batch[it.product];
+    j.residues = P.square ? P.spec_a : P.spec_b;
+    j.row_stride = P.row_stride;
+    j.coefficients = P.coefficients;
+    j.out_len = P.out_len;
+    j.out = P.out;
+    j.width = P.width;
+    j.magic = P.width_magic;
+    j.value_bits = P.value_bits;
+    j.block = it.sub;
+    j.blocks = P.blocks;
+    j.agg_base = P.carry;
+    j.agg_thresh = P.carry + P.blocks;
+    j.first_thresh = P.carry + 2 * (uint64_t)P.blocks;
+    j.ripple_end = P.carry + 3 * (uint64_t)P.blocks;
+    return j;
+}
+
 template <int NP>
 __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
     constexpr int NW = CrtWords<NP>::value;
@@ -488,13 +607,14 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
     __shared__ uint64_t xs[kCrtLdsWords];
     __shared__ CarryFn wave_total[WAVES];
     __shared__ uint32_t ripple;
-    const uint64_t L0 = (uint64_t)blockIdx.x * kCrtBlockLimbs;
-    const uint64_t w = a.width;
-    const uint64_t xb = a.value_bits;
+    const CrtJob J = crt_job(a);
+    const uint64_t L0 = (uint64_t)J.block * kCrtBlockLimbs;
+    const uint64_t w = J.width;
+    const uint64_t xb = J.value_bits;
     const uint64_t lo_bit = L0 * 64;
     const uint64_t i_lo = lo_bit > xb ? (lo_bit - xb) / w : 0;
     uint64_t i_hi = (lo_bit + 64 * (uint64_t)kCrtBlockLimbs - 1) / w + 1;
-    if (i_hi > a.coefficients) i_hi = a.coefficients;
+    if (i_hi > J.coefficients) i_hi = J.coefficients;
     const int count = i_hi > i_lo ? (int)(i_hi - i_lo) : 0;
     // Residues of this thread's coefficients, every load issued first.
     double r[RMAX][NP];
@@ -503,7 +623,7 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
         const int idx = threadIdx.x + q * kCrtThreads;
 #pragma unroll
         for (int k = 0; k < NP; ++k)
-            r[q][k] = idx < count ? a.residues[(uint64_t)k * a.row_stride + i_lo + idx] : 0.0;
+            r[q][k] = idx < count ? J.residues[(uint64_t)k * J.row_stride + i_lo + idx] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < RMAX; ++q) {
@@ -526,14 +646,14 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
     const uint32_t wr = (uint32_t)w;
     const uint32_t rel0 = (uint32_t)(lo_bit - i_lo * w);  // bit of limb L0 from coefficient i_lo
     const uint32_t xb32 = (uint32_t)xb;
-    const uint32_t magic = a.width_magic;
+    const uint32_t magic = J.magic;
 #pragma unroll
     for (int q = 0; q < kCrtLimbsPerThread; ++q) {
         const uint32_t jr = rel0 + 64u * (threadIdx.x * kCrtLimbsPerThread + q);  // bit of limb j from i_lo
         const uint64_t j = L0 + (uint64_t)threadIdx.x * kCrtLimbsPerThread + q;
         uint64_t sum = 0;
         uint32_t carry = 0;
-        if (j < a.out_len && count > 0) {
+        if (j < J.out_len && count > 0) {
             const uint32_t ia = jr + 1 > xb32 ? __umulhi(jr + 1 - xb32 + wr - 1, magic) : 0u;
             uint32_t ib = __umulhi(jr + 63, magic);
             if (ib >= (uint32_t)count) ib = (uint32_t)count - 1;
@@ -599,15 +719,15 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
 #pragma unroll
     for (int q = 0; q < kCrtLimbsPerThread; ++q) {
         const uint64_t j = L0 + threadIdx.x + q * kCrtThreads;
-        if (j < a.out_len) a.out[j] = xs[threadIdx.x + q * kCrtThreads];
+        if (j < J.out_len) J.out[j] = xs[threadIdx.x + q * kCrtThreads];
     }
     if (threadIdx.x == kCrtThreads - 1) {
-        a.agg_base[blockIdx.x] = g.base;
-        a.agg_thresh[blockIdx.x] = g.thresh;
+        J.agg_base[J.block] = g.base;
+        J.agg_thresh[J.block] = g.thresh;
     }
     if (threadIdx.x == 0) {
-        a.first_thresh[blockIdx.x] = first_t;
-        a.ripple_end[blockIdx.x] = ripple;
+        J.first_thresh[J.block] = first_t;
+        J.ripple_end[J.block] = ripple;
     }
 }
 
@@ -617,10 +737,13 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
 // look-back stops at the nearest such block and applies the functions after
 // it. Then the first limb takes the carry, and the limbs up to the recorded
 // ripple end take one more when the carry reaches the first limb's threshold.
-__global__ __launch_bounds__(kCrtThreads) void k_carry_apply(CrtArgs a) {
+constexpr int kApplyThreads = 64;
+
+__global__ __launch_bounds__(kApplyThreads) void k_carry_apply(CrtArgs a) {
     __shared__ uint32_t found;
     __shared__ uint32_t cin_s;
-    const int b = blockIdx.x;
+    const CrtJob J = crt_job(a);
+    const int b = (int)J.block;
     if (b == 0) return;
     if (threadIdx.x == 0) found = 0xFFFFFFFFu;
     __syncthreads();
@@ -629,9 +752,9 @@ __global__ __launch_bounds__(kCrtThreads) void k_carry_apply(CrtArgs a) {
     int start = 0;
     bool resolved = false;
     while (!resolved) {
-        const int first = end - kCrtThreads;
+        const int first = end - kApplyThreads;
         const int k = end - 1 - (int)threadIdx.x;
-        if (k >= 0 && k >= first && a.agg_thresh[k] == kInf) atomicMin(&found, (uint32_t)threadIdx.x);
+        if (k >= 0 && k >= first && J.agg_thresh[k] == kInf) atomicMin(&found, (uint32_t)threadIdx.x);
         __syncthreads();
         const uint32_t t = found;
         __syncthreads();
@@ -648,20 +771,20 @@ __global__ __launch_bounds__(kCrtThreads) void k_carry_apply(CrtArgs a) {
         }
     }
     if (threadIdx.x == 0) {
-        uint32_t cc = start >= 0 ? a.agg_base[start] : 0u;
-        for (int k = start + 1; k < b; ++k) cc = a.agg_base[k] + (cc >= a.agg_thresh[k] ? 1u : 0u);
+        uint32_t cc = start >= 0 ? J.agg_base[start] : 0u;
+        for (int k = start + 1; k < b; ++k) cc = J.agg_base[k] + (cc >= J.agg_thresh[k] ? 1u : 0u);
         cin_s = cc;
     }
     __syncthreads();
     c = cin_s;
     if (c == 0) return;
     const uint64_t L0 = (uint64_t)b * kCrtBlockLimbs;
-    if (threadIdx.x == 0 && L0 < a.out_len) a.out[L0] += c;
-    if (c >= a.first_thresh[b]) {
-        const uint32_t rend = a.ripple_end[b];
-        for (uint32_t rel = 1 + threadIdx.x; rel <= rend; rel += kCrtThreads) {
+    if (threadIdx.x == 0 && L0 < J.out_len) J.out[L0] += c;
+    if (c >= J.first_thresh[b]) {
+        const uint32_t rend = J.ripple_end[b];
+        for (uint32_t rel = 1 + threadIdx.x; rel <= rend; rel += kApplyThreads) {
             const uint64_t j = L0 + rel;
-            if (j < a.out_len) a.out[j] += 1;
+            if (j < J.out_len) J.out[j] += 1;
         }
     }
 }
@@ -693,60 +816,99 @@ static constexpr int strided_threads() {
     return strided_lines<LOGL>() * (1 << LOGL) / 16 >= 512 ? 512 : 256;
 }
 
+// In a batched launch (a.batch set) `units` counts work items (product,
+// prime); otherwise it counts primes.
 template <int LOGL>
-static bool launch_fwd_strided_t(const PassArgs& a, int primes, bool pack, hipStream_t s) {
+static bool launch_fwd_strided_t(const PassArgs& a, int units, bool pack, hipStream_t s) {
     constexpr int LINES = strided_lines<LOGL>();
+    constexpr int NT = strided_threads<LOGL>();
     const uint64_t nodes = (uint64_t)1 << a.level_start;
     const uint64_t tiles = nodes * (a.line_stride / LINES);
-    dim3 grid((unsigned)tiles, (unsigned)primes);
-    if (pack) {
-        PassArgs b = a;
-        b.primes = primes;
-        const dim3 flat((unsigned)(((tiles + 7) / 8) * 8 * primes), 1);
-        hipLaunchKernelGGL((k_fwd_strided<LOGL, LINES, strided_threads<LOGL>(), true>), flat, dim3(strided_threads<LOGL>()), 0, s, b);
+    PassArgs b = a;
+    b.primes = units;
+    b.tiles_per_item = tiles;
+    dim3 grid((unsigned)tiles, (unsigned)units);
+    if (a.batch) grid = dim3((unsigned)(tiles * units), 1);
+    else if (pack) grid = dim3((unsigned)(((tiles + 7) / 8) * 8 * units), 1);
+    if (a.batch) {
+        if (pack)
+            NTT_LAUNCH(nullptr, (k_fwd_strided<LOGL, LINES, NT, true, true>), grid, dim3(NT), s, b);
+        else
+            NTT_LAUNCH(nullptr, (k_fwd_strided<LOGL, LINES, NT, false, true>), grid, dim3(NT), s, b);
     } else {
-        hipLaunchKernelGGL((k_fwd_strided<LOGL, LINES, strided_threads<LOGL>(), false>), grid, dim3(strided_threads<LOGL>()), 0, s, a);
+        if (pack)
+            NTT_LAUNCH(nullptr, (k_fwd_strided<LOGL, LINES, NT, true, false>), grid, dim3(NT), s, b);
+        else
+            NTT_LAUNCH(nullptr, (k_fwd_strided<LOGL, LINES, NT, false, false>), grid, dim3(NT), s, b);
     }
     return NTT_CHECK_LAUNCH();
 }
 
 template <int LOGL>
-static bool launch_inv_strided_t(const PassArgs& a, int primes, bool canon, hipStream_t s) {
+static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, hipStream_t s) {
     constexpr int LINES = strided_lines<LOGL>();
+    constexpr int NT = strided_threads<LOGL>();
     const uint64_t nodes = (uint64_t)1 << a.level_start;
     const uint64_t tiles = nodes * (a.line_stride / LINES);
-    dim3 grid((unsigned)tiles, (unsigned)primes);
-    if (canon)
-        hipLaunchKernelGGL((k_inv_strided<LOGL, LINES, strided_threads<LOGL>(), true>), grid, dim3(strided_threads<LOGL>()), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_inv_strided<LOGL, LINES, strided_threads<LOGL>(), false>), grid, dim3(strided_threads<LOGL>()), 0, s, a);
-    return NTT_CHECK_LAUNCH();
-}
-
-template <int LOGL>
-static bool launch_fwd_rows_t(const PassArgs& a, int primes, hipStream_t s) {
-    constexpr int LINES = 1;
-    const uint64_t rows = (uint64_t)1 << a.level_start;
-    dim3 grid((unsigned)(rows / LINES), (unsigned)primes);
-    hipLaunchKernelGGL((k_fwd_rows<LOGL, LINES, 256>), grid, dim3(256), 0, s, a);
-    return NTT_CHECK_LAUNCH();
-}
-
-template <int LOGL>
-static bool launch_fused_t(const PassArgs& a, int primes, bool pack, bool square, hipStream_t s) {
-    const uint64_t rows = (uint64_t)1 << a.level_start;
-    dim3 grid((unsigned)rows, (unsigned)primes);
-    constexpr int NT = LOGL <= 8 ? 64 : 256;
-    if (pack) {
-        if (square)
-            hipLaunchKernelGGL((k_fused<LOGL, NT, true, true>), grid, dim3(NT), 0, s, a);
+    PassArgs b = a;
+    b.tiles_per_item = tiles;
+    dim3 grid((unsigned)tiles, (unsigned)units);
+    if (a.batch) grid = dim3((unsigned)(tiles * units), 1);
+    if (a.batch) {
+        if (canon)
+            NTT_LAUNCH(nullptr, (k_inv_strided<LOGL, LINES, NT, true, true>), grid, dim3(NT), s, b);
         else
-            hipLaunchKernelGGL((k_fused<LOGL, NT, true, false>), grid, dim3(NT), 0, s, a);
+            NTT_LAUNCH(nullptr, (k_inv_strided<LOGL, LINES, NT, false, true>), grid, dim3(NT), s, b);
     } else {
-        if (square)
-            hipLaunchKernelGGL((k_fused<LOGL, NT, false, true>), grid, dim3(NT), 0, s, a);
+        if (canon)
+            NTT_LAUNCH(nullptr, (k_inv_strided<LOGL, LINES, NT, true, false>), grid, dim3(NT), s, b);
         else
-            hipLaunchKernelGGL((k_fused<LOGL, NT, false, false>), grid, dim3(NT), 0, s, a);
+            NTT_LAUNCH(nullptr, (k_inv_strided<LOGL, LINES, NT, false, false>), grid, dim3(NT), s, b);
+    }
+    return NTT_CHECK_LAUNCH();
+}
+
+template <int LOGL>
+static bool launch_fwd_rows_t(const PassArgs& a, int units, hipStream_t s) {
+    constexpr int LINES = 1;
+    const uint64_t rows = ((uint64_t)1 << a.level_start) / LINES;
+    PassArgs b = a;
+    b.tiles_per_item = rows;
+    dim3 grid((unsigned)rows, (unsigned)units);
+    if (a.batch) grid = dim3((unsigned)(rows * units), 1);
+    if (a.batch)
+        NTT_LAUNCH(nullptr, (k_fwd_rows<LOGL, LINES, 256, true>), grid, dim3(256), s, b);
+    else
+        NTT_LAUNCH(nullptr, (k_fwd_rows<LOGL, LINES, 256, false>), grid, dim3(256), s, b);
+    return NTT_CHECK_LAUNCH();
+}
+
+template <int LOGL>
+static bool launch_fused_t(const PassArgs& a, int units, bool pack, bool square, hipStream_t s) {
+    const uint64_t rows = (uint64_t)1 << a.level_start;
+    PassArgs b = a;
+    b.tiles_per_item = rows;
+    dim3 grid((unsigned)rows, (unsigned)units);
+    if (a.batch) grid = dim3((unsigned)(rows * units), 1);
+    constexpr int NT = LOGL <= 8 ? 64 : 256;
+    if (a.batch) {
+        if (pack && square)
+            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, true, true, true>), grid, dim3(NT), s, b);
+        else if (pack)
+            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, true, false, true>), grid, dim3(NT), s, b);
+        else if (square)
+            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, false, true, true>), grid, dim3(NT), s, b);
+        else
+            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, false, false, true>), grid, dim3(NT), s, b);
+    } else {
+        if (pack && square)
+            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, true, true, false>), grid, dim3(NT), s, b);
+        else if (pack)
+            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, true, false, false>), grid, dim3(NT), s, b);
+        else if (square)
+            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, false, true, false>), grid, dim3(NT), s, b);
+        else
+            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, false, false, false>), grid, dim3(NT), s, b);
     }
     return NTT_CHECK_LAUNCH();
 }
@@ -807,35 +969,36 @@ bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool squar
 
 bool launch_tables(const TableArgs& a, int primes, hipStream_t s) {
     const unsigned blocks = (unsigned)((a.len + 255) / 256 < 4096 ? (a.len + 255) / 256 : 4096);
-    hipLaunchKernelGGL(k_tables, dim3(blocks, primes), dim3(256), 0, s, a);
+    NTT_LAUNCH(nullptr, k_tables, dim3(blocks, primes), dim3(256), s, a);
     return NTT_CHECK_LAUNCH();
 }
 
+// In a batched launch a.blocks counts (product, block) items.
 bool launch_crt_product(const CrtArgs& a, hipStream_t s) {
     dim3 grid(a.blocks);
     switch (a.primes) {
-        case 1: hipLaunchKernelGGL(k_crt_product<1>, grid, dim3(kCrtThreads), 0, s, a); break;
-        case 2: hipLaunchKernelGGL(k_crt_product<2>, grid, dim3(kCrtThreads), 0, s, a); break;
-        case 3: hipLaunchKernelGGL(k_crt_product<3>, grid, dim3(kCrtThreads), 0, s, a); break;
-        case 4: hipLaunchKernelGGL(k_crt_product<4>, grid, dim3(kCrtThreads), 0, s, a); break;
-        case 5: hipLaunchKernelGGL(k_crt_product<5>, grid, dim3(kCrtThreads), 0, s, a); break;
-        case 6: hipLaunchKernelGGL(k_crt_product<6>, grid, dim3(kCrtThreads), 0, s, a); break;
-        case 7: hipLaunchKernelGGL(k_crt_product<7>, grid, dim3(kCrtThreads), 0, s, a); break;
-        case 8: hipLaunchKernelGGL(k_crt_product<8>, grid, dim3(kCrtThreads), 0, s, a); break;
+        case 1: NTT_LAUNCH("crt_product", k_crt_product<1>, grid, dim3(kCrtThreads), s, a); break;
+        case 2: NTT_LAUNCH("crt_product", k_crt_product<2>, grid, dim3(kCrtThreads), s, a); break;
+        case 3: NTT_LAUNCH("crt_product", k_crt_product<3>, grid, dim3(kCrtThreads), s, a); break;
+        case 4: NTT_LAUNCH("crt_product", k_crt_product<4>, grid, dim3(kCrtThreads), s, a); break;
+        case 5: NTT_LAUNCH("crt_product", k_crt_product<5>, grid, dim3(kCrtThreads), s, a); break;
+        case 6: NTT_LAUNCH("crt_product", k_crt_product<6>, grid, dim3(kCrtThreads), s, a); break;
+        case 7: NTT_LAUNCH("crt_product", k_crt_product<7>, grid, dim3(kCrtThreads), s, a); break;
+        case 8: NTT_LAUNCH("crt_product", k_crt_product<8>, grid, dim3(kCrtThreads), s, a); break;
         default: return false;
     }
     if (!NTT_CHECK_LAUNCH()) return false;
-    if (a.blocks > 1) hipLaunchKernelGGL(k_carry_apply, dim3(a.blocks), dim3(kCrtThreads), 0, s, a);
+    if (a.blocks > 1 || a.batch) NTT_LAUNCH("carry_apply", k_carry_apply, dim3(a.blocks), dim3(kApplyThreads), s, a);
     return NTT_CHECK_LAUNCH();
 }
 
 bool launch_crt_disjoint(const CrtArgs& a, hipStream_t s) {
     dim3 grid((unsigned)((a.coefficients + 255) / 256));
     switch (a.primes) {
-        case 1: hipLaunchKernelGGL(k_crt_disjoint<1>, grid, dim3(256), 0, s, a); break;
-        case 2: hipLaunchKernelGGL(k_crt_disjoint<2>, grid, dim3(256), 0, s, a); break;
-        case 3: hipLaunchKernelGGL(k_crt_disjoint<3>, grid, dim3(256), 0, s, a); break;
-        case 4: hipLaunchKernelGGL(k_crt_disjoint<4>, grid, dim3(256), 0, s, a); break;
+        case 1: NTT_LAUNCH(nullptr, k_crt_disjoint<1>, grid, dim3(256), s, a); break;
+        case 2: NTT_LAUNCH(nullptr, k_crt_disjoint<2>, grid, dim3(256), s, a); break;
+        case 3: NTT_LAUNCH(nullptr, k_crt_disjoint<3>, grid, dim3(256), s, a); break;
+        case 4: NTT_LAUNCH(nullptr, k_crt_disjoint<4>, grid, dim3(256), s, a); break;
         default: return false;
     }
     return NTT_CHECK_LAUNCH();
@@ -886,14 +1049,14 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* src, uint4* dst, uint
 
 bool launch_copy(const void* src, void* dst, uint64_t bytes, hipStream_t s) {
     const uint64_t n = bytes / 16;
-    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
+    NTT_LAUNCH(nullptr, k_copy, dim3(4096), dim3(256), s, (const uint4*)src, (uint4*)dst, n);
     return hipGetLastError() == hipSuccess;
 }
 
 bool launch_conv_columns(const uint64_t* a, uint64_t la, const uint64_t* b, uint64_t lb, uint64_t* out,
                          uint32_t stride, uint32_t is_signed, hipStream_t s) {
     const uint64_t n = la + lb - 1;
-    hipLaunchKernelGGL(k_conv_columns, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, la, b,
+    NTT_LAUNCH(nullptr, k_conv_columns, dim3((unsigned)((n + 255) / 256)), dim3(256), s, a, la, b,
                        lb, out, stride, is_signed);
     return hipGetLastError() == hipSuccess;
 }
