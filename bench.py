@@ -26,7 +26,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import nttanafis_amd as ntt  # noqa: E402
-from nttanafis_amd.workloads import batch_operands, batch_shapes  # noqa: E402
+from nttanafis_amd.workloads import (CONV, SQR, batch_operands, batch_shapes, operand_pools,  # noqa: E402
+                                     output_len, request_operands, served_stream)
 
 HBM_PEAK_GBS = 8000.0
 
@@ -226,6 +227,121 @@ def bench_batch(steps: int, warmup: int, plain_steps: int) -> dict:
     }
 
 
+def _pct(xs, q):
+    return float(np.percentile(np.asarray(xs), q)) if xs else None
+
+
+def bench_served(count: int, load: float, hi: int = 4194304) -> dict:
+    """Served request stream: Poisson arrivals of mul / sqr / 16-bit convolve
+    requests (equal shares), sizes log-uniform in [256, hi], 10% of operands
+    sparse or all-ones, host operands in pinned memory. The plain per-call
+    path (one MulPlan / ConvPlan call at a time, in arrival order) and the
+    server (continuous batching, admission by device-memory budget) serve the
+    same stream at the same offered load, `load` times the plain path's
+    capacity. Latency runs from a request's arrival to its result in host
+    memory."""
+    base = served_stream(count, 1.0, hi=hi)
+    pools = operand_pools(hi, alloc=ntt.pinned_array)
+    outs_len = [output_len(r) for r in base]
+    out_pool = ntt.pinned_array(int(sum(outs_len)))
+    ofs = np.concatenate([[0], np.cumsum(outs_len)[:-1]]).astype(np.int64)
+    ops = [request_operands(r, pools) for r in base]
+    dsts = [out_pool[int(o):int(o) + n] for o, n in zip(ofs, outs_len)]
+    plans = []
+    for r, (a, b) in zip(base, ops):
+        if r["kind"] == CONV:
+            plans.append(ntt.ConvPlan.new(len(a), len(b), 0xFFFF, 0xFFFF, False))
+        else:
+            plans.append(ntt.MulPlan.new(len(a), len(b)))
+
+    def plain_call(i):
+        r, (a, b), dst, plan = base[i], ops[i], dsts[i], plans[i]
+        if r["kind"] == CONV:
+            ok = plan.convolve(dst, a, b)
+        elif r["kind"] == SQR:
+            ok = plan.sqr(dst, a)
+        else:
+            ok = plan.mul(dst, a, b)
+        if not ok:
+            raise SystemExit(f"plain request {i} failed: {ntt.last_error()}")
+
+    # Warm both paths (tables, arenas, autotuning) on the whole stream once.
+    for i in range(count):
+        plain_call(i)
+    # Plain capacity: the stream back to back.
+    t0 = ntt.clock()
+    for i in range(count):
+        plain_call(i)
+    plain_busy = ntt.clock() - t0
+    capacity = count / plain_busy
+    digests = [hash(d.tobytes()) for d in dsts]
+    rate = load * capacity
+    arrivals = [r["arrival"] / rate for r in base]
+
+    # Plain per-call path at the offered load.
+    lat_plain = []
+    t0 = ntt.clock()
+    for i in range(count):
+        while ntt.clock() - t0 < arrivals[i]:
+            pass
+        plain_call(i)
+        lat_plain.append(ntt.clock() - t0 - arrivals[i])
+    plain_span = ntt.clock() - t0
+
+    server = ntt.Server(budget_bytes=0, slots=4)
+
+    def serve(arr):
+        out_pool[:] = 0
+        t0 = ntt.clock()
+        tickets = []
+        for i in range(count):
+            while ntt.clock() - t0 < arr[i]:
+                pass
+            r, (a, b), dst = base[i], ops[i], dsts[i]
+            if r["kind"] == CONV:
+                t = server.submit_conv(a, b, 0xFFFF, 0xFFFF, False, dst)
+            elif r["kind"] == SQR:
+                t = server.submit_sqr(a, dst)
+            else:
+                t = server.submit_mul(a, b, dst)
+            if t < 0:
+                raise SystemExit(f"served request {i} refused: {ntt.last_error()}")
+            tickets.append(t)
+        if not server.drain():
+            raise SystemExit("a served request failed")
+        lat, last = [], 0.0
+        for i, t in enumerate(tickets):
+            _, _, done = server.times(t)
+            lat.append(done - t0 - arr[i])
+            last = max(last, done - t0)
+            server.release(t)
+        bad = [i for i in range(count) if hash(dsts[i].tobytes()) != digests[i]]
+        if bad:
+            raise SystemExit(f"served results differ from the plain path for requests {bad[:5]}")
+        return lat, last
+
+    serve([0.0] * count)  # warm the server's slots and groups
+    lat_served, served_span = serve(arrivals)
+    _, burst_span = serve([0.0] * count)
+    stats = server.stats()
+    server.close()
+    ms = lambda x: round(1000 * x, 3)  # noqa: E731
+    return {
+        "requests": count,
+        "offered_load_per_s": round(rate, 1),
+        "load_vs_plain_capacity": load,
+        "plain_capacity_per_s": round(capacity, 1),
+        "served_capacity_per_s": round(count / burst_span, 1),
+        "capacity_ratio": round(plain_busy / burst_span, 2),
+        "served": {"products_per_s": round(count / served_span, 1), "p50_ms": ms(_pct(lat_served, 50)),
+                   "p99_ms": ms(_pct(lat_served, 99)), "mean_ms": ms(float(np.mean(lat_served)))},
+        "plain": {"products_per_s": round(count / plain_span, 1), "p50_ms": ms(_pct(lat_plain, 50)),
+                  "p99_ms": ms(_pct(lat_plain, 99)), "mean_ms": ms(float(np.mean(lat_plain)))},
+        "server_stats": stats,
+        "mix": {k: sum(1 for r in base if r["kind"] == k) for k in ("mul", "sqr", "conv")},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +352,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=240)
+    ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="after the timed region, run the plain copy kernel over this many bytes "
                          "(the calibration reference for hardware byte counters)")
@@ -249,6 +368,14 @@ def main() -> int:
         print("bench.py: no gfx950 device", file=sys.stderr)
         return 1
 
+    if args.workload == "served":
+        res = bench_served(args.serve_requests, args.serve_load)
+        print(json.dumps({"metric": "served products/s and p99 latency", "value": res["served"]["products_per_s"],
+                          "unit": "requests/s", "n_gpus": 1, "steps": args.serve_requests, "warmup": 1,
+                          "ms_per_step": res["served"]["p99_ms"], "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f64", "data": "synthetic (SplitMix64 limbs)",
+                          "config": {"workload": "served"}, "served": res}), flush=True)
+        return 0
     if args.workload == "batch4096":
         res = bench_batch(args.steps, args.warmup, max(1, args.steps // 5))
         print(json.dumps({"metric": "batched products/s (4096 uneven products)", "value": res["products_per_s"],
@@ -377,6 +504,8 @@ def main() -> int:
     }
     if not args.no_batch:
         result["batched"] = bench_batch(args.batch_steps, 2, 2)
+    if not args.no_serve:
+        result["served"] = bench_served(args.serve_requests, args.serve_load)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     print(json.dumps(result), flush=True)

@@ -218,6 +218,95 @@ bool ntt_batch_mul_host(ntt_batch *batch, const uint64_t *const *a, const uint64
                         uint64_t *const *out);
 
 /* ------------------------------------------------------------------------ */
+/* Request server (no counterpart in the reference). Serves a stream of
+ * products, squares and convolutions from host threads with continuous
+ * batching: worker slots with their own streams take work whenever they are
+ * free, small products are grouped into one batched launch sequence, and
+ * every other request is admitted in arrival order once its device working
+ * set fits in the memory budget. Root tables stay resident. Each request
+ * computes what MulWorkspace::mul / sqr (src/plan.rs:164-179) or
+ * ConvPlan::convolve (src/convolve.rs:148-162) computes for it alone. */
+
+enum {
+    NTT_REQ_MUL = 0,  /* dst = a * b (len_a + len_b limbs) */
+    NTT_REQ_SQR = 1,  /* dst = a * a (2 len_a limbs); b ignored */
+    NTT_REQ_CONV = 2, /* dst = ConvPlan(len_a, len_b, max_a, max_b, signed).convolve(a, b) */
+};
+
+enum {
+    NTT_REQ_SIGNED = 1, /* convolution of two's complement coefficients */
+    NTT_REQ_DEVICE = 2, /* a, b and dst are device addresses */
+};
+
+typedef struct ntt_request {
+    uint32_t kind;
+    uint32_t flags;
+    const uint64_t *a;
+    uint64_t len_a;
+    const uint64_t *b;
+    uint64_t len_b;
+    uint64_t max_a; /* convolution coefficient bounds */
+    uint64_t max_b;
+    uint64_t *dst;
+    uint64_t dst_len;
+} ntt_request;
+
+typedef struct ntt_server_stats {
+    uint64_t submitted;
+    uint64_t completed;
+    uint64_t failed;
+    uint64_t groups;           /* batched groups of small products run */
+    uint64_t grouped;          /* requests that ran inside groups */
+    uint64_t budget_bytes;     /* device memory budget of large requests */
+    uint64_t peak_bytes;       /* most of it in use at once */
+    uint64_t waits_for_memory; /* admissions deferred for lack of memory */
+} ntt_server_stats;
+
+typedef struct ntt_server ntt_server;
+
+/* Starts a server on the current device with `slots` worker streams (0: 4)
+ * and a budget of `hbm_budget_bytes` of device memory for large requests (0:
+ * half of the free memory), allocated once at start. Each slot also keeps up
+ * to about 0.3 GiB for its groups of small products. */
+ntt_server *ntt_server_new(uint64_t hbm_budget_bytes, uint32_t slots);
+
+/* Finishes every submitted request, then stops the server. */
+void ntt_server_free(ntt_server *server);
+
+/* Queues a request and returns its ticket, or -1 when the request is
+ * malformed (lengths, bounds, dst_len) or can never fit the budget. Buffers
+ * must stay valid and unchanged until the request completes; dst is written
+ * only by a request that completes successfully. Thread-safe. */
+int64_t ntt_server_submit(ntt_server *server, const ntt_request *request);
+
+/* 0 while queued or running, 1 when done, -1 when failed or unknown. */
+int ntt_server_poll(ntt_server *server, int64_t ticket);
+
+/* Blocks until the request ends; returns 1 (done) or -1 (failed). */
+int ntt_server_wait(ntt_server *server, int64_t ticket);
+
+/* Blocks until every submitted request has ended; true when none failed. */
+bool ntt_server_drain(ntt_server *server);
+
+/* Submission, start and completion times of a request on ntt_clock_s's
+ * clock. */
+bool ntt_server_times(ntt_server *server, int64_t ticket, double *submit_s, double *start_s,
+                      double *done_s);
+
+/* Forgets an ended request (its ticket becomes unknown). */
+void ntt_server_release(ntt_server *server, int64_t ticket);
+
+void ntt_server_get_stats(ntt_server *server, ntt_server_stats *out);
+
+/* Monotonic clock of the server's time stamps, in seconds. */
+double ntt_clock_s(void);
+
+/* Pinned host memory (hipHostMalloc) for request buffers: copies from it run
+ * at full link speed. */
+void *ntt_host_alloc(size_t bytes);
+void ntt_host_free(void *ptr);
+
+/* ------------------------------------------------------------------------ */
 /* Engine utilities (no counterpart in the reference).                       */
 
 /* 1 when a gfx950 device is usable, 0 otherwise. Never initializes more than

@@ -12,7 +12,10 @@ from .api import (  # noqa: F401
     MulPlan,
     MulWorkspace,
     Profile,
+    Server,
+    clock,
     device_available,
+    pinned_array,
     last_error,
     mul_geometry,
     synchronize,
@@ -26,6 +29,9 @@ __all__ = [
     "ConvPlan",
     "ConvWorkspace",
     "DeviceBuffer",
+    "Server",
+    "clock",
+    "pinned_array",
     "Profile",
     "device_available",
     "last_error",

@@ -48,6 +48,37 @@ class ConvPlanC(ctypes.Structure):
     ]
 
 
+class RequestC(ctypes.Structure):
+    _fields_ = [
+        ("kind", u32),
+        ("flags", u32),
+        ("a", ctypes.c_void_p),
+        ("len_a", u64),
+        ("b", ctypes.c_void_p),
+        ("len_b", u64),
+        ("max_a", u64),
+        ("max_b", u64),
+        ("dst", ctypes.c_void_p),
+        ("dst_len", u64),
+    ]
+
+
+class ServerStatsC(ctypes.Structure):
+    _fields_ = [
+        ("submitted", u64),
+        ("completed", u64),
+        ("failed", u64),
+        ("groups", u64),
+        ("grouped", u64),
+        ("budget_bytes", u64),
+        ("peak_bytes", u64),
+        ("waits_for_memory", u64),
+    ]
+
+
+REQ_MUL, REQ_SQR, REQ_CONV = 0, 1, 2
+REQ_SIGNED, REQ_DEVICE = 1, 2
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "ntt_mul_plan_new": (ctypes.c_bool, [size_t, size_t, ctypes.POINTER(MulPlanC)]),
@@ -108,6 +139,22 @@ SIGNATURES = {
         [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
          ctypes.POINTER(ctypes.c_void_p)],
     ),
+    "ntt_server_new": (ctypes.c_void_p, [u64, u32]),
+    "ntt_server_free": (None, [ctypes.c_void_p]),
+    "ntt_server_submit": (ctypes.c_int64, [ctypes.c_void_p, ctypes.POINTER(RequestC)]),
+    "ntt_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "ntt_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "ntt_server_drain": (ctypes.c_bool, [ctypes.c_void_p]),
+    "ntt_server_times": (
+        ctypes.c_bool,
+        [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+         ctypes.POINTER(ctypes.c_double)],
+    ),
+    "ntt_server_release": (None, [ctypes.c_void_p, ctypes.c_int64]),
+    "ntt_server_get_stats": (None, [ctypes.c_void_p, ctypes.POINTER(ServerStatsC)]),
+    "ntt_clock_s": (ctypes.c_double, []),
+    "ntt_host_alloc": (ctypes.c_void_p, [size_t]),
+    "ntt_host_free": (None, [ctypes.c_void_p]),
     "ntt_device_available": (ctypes.c_int, []),
     "ntt_device_alloc": (ctypes.c_void_p, [size_t]),
     "ntt_device_free": (None, [ctypes.c_void_p]),

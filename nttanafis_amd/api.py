@@ -314,6 +314,127 @@ class Batch:
         return bool(_lib().ntt_batch_mul_host(self._h, pa, pb, po))
 
 
+class _Pinned:
+    """Owner of one pinned allocation; freed when the last view is gone."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = _lib().ntt_host_alloc(nbytes)
+        if not self.ptr:
+            raise MemoryError(f"ntt_host_alloc({nbytes})")
+
+    def __del__(self):
+        if self.ptr:
+            _lib().ntt_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_array(count: int) -> np.ndarray:
+    """A numpy uint64 array of ``count`` entries in pinned host memory
+    (hipHostMalloc): copies between it and the device run at full link speed.
+    The memory lives as long as the array or any view of it."""
+    count = int(count)
+    owner = _Pinned(8 * max(count, 1))
+    raw = (ctypes.c_uint64 * max(count, 1)).from_address(owner.ptr)
+    raw._owner = owner  # the ctypes array is the numpy base and keeps the owner
+    return np.frombuffer(raw, dtype=np.uint64)[:count]
+
+
+class Server:
+    """Continuous-batching request server (include/nttanafis.h, ntt_server_*).
+
+    Each request computes what ``MulWorkspace.mul``/``sqr`` or
+    ``ConvPlan.convolve`` computes for it alone (src/plan.rs:164-179,
+    src/convolve.rs:148-162). Arrays passed to ``submit_*`` are kept alive by
+    the server object until the request is waited for or released.
+    """
+
+    def __init__(self, budget_bytes: int = 0, slots: int = 0):
+        self._h = _lib().ntt_server_new(int(budget_bytes), int(slots))
+        if not self._h:
+            raise RuntimeError("ntt_server_new failed: " + last_error())
+        self._keep = {}
+
+    def close(self):
+        if self._h:
+            _lib().ntt_server_free(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _submit(self, kind, a, b, dst, max_a=0, max_b=0, flags=0, keep=None) -> int:
+        """Low-level submit: a, b, dst are (address, length) pairs."""
+        r = _ffi.RequestC()
+        r.kind, r.flags = kind, flags
+        r.a, r.len_a = a
+        r.b, r.len_b = b if b is not None else (None, 0)
+        r.max_a, r.max_b = int(max_a), int(max_b)
+        r.dst, r.dst_len = dst
+        t = int(_lib().ntt_server_submit(self._h, ctypes.byref(r)))
+        if t >= 0 and keep is not None:
+            self._keep[t] = keep
+        return t
+
+    @staticmethod
+    def _host(arr):
+        _limbs(arr)
+        return arr.ctypes.data, len(arr)
+
+    def submit_mul(self, a: np.ndarray, b: np.ndarray, dst: np.ndarray) -> int:
+        _limbs(dst, writable=True)
+        return self._submit(_ffi.REQ_MUL, self._host(a), self._host(b), self._host(dst), keep=(a, b, dst))
+
+    def submit_sqr(self, a: np.ndarray, dst: np.ndarray) -> int:
+        _limbs(dst, writable=True)
+        return self._submit(_ffi.REQ_SQR, self._host(a), None, self._host(dst), keep=(a, dst))
+
+    def submit_conv(self, a: np.ndarray, b: np.ndarray, max_a: int, max_b: int, signed: bool,
+                    dst: np.ndarray) -> int:
+        _limbs(dst, writable=True)
+        flags = _ffi.REQ_SIGNED if signed else 0
+        return self._submit(_ffi.REQ_CONV, self._host(a), self._host(b), self._host(dst), max_a, max_b, flags,
+                            keep=(a, b, dst))
+
+    def submit_device(self, kind: int, d_a: int, len_a: int, d_b: int, len_b: int, d_dst: int, dst_len: int,
+                      max_a: int = 0, max_b: int = 0, signed: bool = False) -> int:
+        flags = _ffi.REQ_DEVICE | (_ffi.REQ_SIGNED if signed else 0)
+        return self._submit(kind, (d_a, len_a), (d_b, len_b), (d_dst, dst_len), max_a, max_b, flags)
+
+    def poll(self, ticket: int) -> int:
+        return int(_lib().ntt_server_poll(self._h, ticket))
+
+    def wait(self, ticket: int) -> int:
+        st = int(_lib().ntt_server_wait(self._h, ticket))
+        return st
+
+    def drain(self) -> bool:
+        return bool(_lib().ntt_server_drain(self._h))
+
+    def times(self, ticket: int):
+        s, b, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        if not _lib().ntt_server_times(self._h, ticket, ctypes.byref(s), ctypes.byref(b), ctypes.byref(d)):
+            return None
+        return s.value, b.value, d.value
+
+    def release(self, ticket: int) -> None:
+        _lib().ntt_server_release(self._h, ticket)
+        self._keep.pop(ticket, None)
+
+    def stats(self) -> dict:
+        st = _ffi.ServerStatsC()
+        _lib().ntt_server_get_stats(self._h, ctypes.byref(st))
+        return {name: int(getattr(st, name)) for name, _ in _ffi.ServerStatsC._fields_}
+
+
+def clock() -> float:
+    """The server's monotonic clock, in seconds."""
+    return float(_lib().ntt_clock_s())
+
+
 class DeviceBuffer:
     """Device memory owned by the engine's allocator (hipMalloc)."""
 

@@ -10,12 +10,13 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnttanafis_amd.so")
-SOURCES = ["kernels.hip", "engine.hip"]
+SOURCES = ["kernels.hip", "engine.hip", "server.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -23,7 +24,6 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # Exactness of the residue arithmetic depends on every product and sum
     # rounding separately; only explicit fma() calls may fuse.
     "-ffp-contract=off",
@@ -34,15 +34,33 @@ FLAGS = [
 
 
 def build_library(verbose: bool = False) -> str:
+    """Compiles every source in parallel, then links the shared library."""
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    proc = subprocess.run(cmd, capture_output=True, text=True)
-    if proc.returncode != 0:
-        sys.stderr.write(proc.stdout + proc.stderr)
-        raise RuntimeError("hipcc failed to build libnttanafis_amd.so")
+    with tempfile.TemporaryDirectory(prefix="nttamd_build_") as tmpdir:
+        objs, procs = [], []
+        for src in srcs:
+            obj = os.path.join(tmpdir, os.path.basename(src) + ".o")
+            cmd = [HIPCC, *FLAGS, "-c", "-o", obj, src]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+            objs.append(obj)
+        failed = False
+        for src, proc in procs:
+            out, _ = proc.communicate()
+            if proc.returncode != 0:
+                sys.stderr.write(out)
+                failed = True
+        if failed:
+            raise RuntimeError("hipcc failed to build libnttanafis_amd.so")
+        tmp = LIB + ".tmp"
+        cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        proc = subprocess.run(cmd, capture_output=True, text=True)
+        if proc.returncode != 0:
+            sys.stderr.write(proc.stdout + proc.stderr)
+            raise RuntimeError("hipcc failed to link libnttanafis_amd.so")
     os.replace(tmp, LIB)
     return LIB
 

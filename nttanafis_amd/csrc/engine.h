@@ -1,0 +1,188 @@
+// Internal interface of the engine's host side, shared by the product and
+// convolution pipelines (engine.hip) and the request server (server.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nttanafis.h"
+#include "arith.h"
+
+namespace nttamd {
+
+void set_error(const std::string& e);
+void clear_error();
+
+// Names the kernel launches of a scope for per-launch timing.
+struct Timed {
+    const char* saved;
+    Timed(const char* n, hipStream_t s);
+    ~Timed();
+};
+
+// ---------------------------------------------------------------------------
+// Geometry of one device operation.
+
+struct Geometry {
+    uint32_t primes = 0;
+    uint32_t width = 0;         // bits between operand coefficients
+    uint32_t value_bits_a = 0;  // significant bits per coefficient
+    uint32_t value_bits_b = 0;
+    uint64_t digits_a = 0, digits_b = 0;
+    uint64_t conv_len = 0;
+    int log_t = 0;
+    uint32_t bound_bits = 0;  // bits of every reconstructed coefficient
+    bool disjoint = false;
+    bool is_signed = false;
+    uint32_t stride = 0;  // limbs per result coefficient (convolution)
+};
+
+bool select_product(uint64_t la, uint64_t lb, Geometry* out);
+bool conv_bound_bits(uint64_t terms, uint64_t max_a, uint64_t max_b, uint64_t* bits);
+bool product_geometry(const ntt_mul_plan* plan, Geometry* g);
+bool conv_geometry(const ntt_conv_plan* p, Geometry* g);
+// ConvPlan::admits, src/convolve.rs:169-184.
+bool conv_admits(const ntt_conv_plan* p, const uint64_t* a, size_t a_len, const uint64_t* b, size_t b_len);
+
+// Root tables shared by every caller (grow-only).
+bool ensure_tables(uint64_t need, hipStream_t s, const double** fwd, const double** inv, uint64_t* stride);
+
+// ---------------------------------------------------------------------------
+// Device memory.
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    bool ensure(size_t n) {
+        if (n <= bytes) return true;
+        if (ptr) hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        size_t want = std::max<size_t>(n, 4096);
+        if (hipMalloc(&ptr, want) != hipSuccess) {
+            ptr = nullptr;
+            return false;
+        }
+        bytes = want;
+        return true;
+    }
+    void release() {
+        if (ptr) hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+struct Executor {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    DevBuf spec_a, spec_b, in_a, in_b, out, carry;
+    std::mutex lock;
+
+    bool init() {
+        if (stream) return true;
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+            stream = nullptr;
+            return false;
+        }
+        own_stream = true;
+        return true;
+    }
+    ~Executor() {
+        spec_a.release();
+        spec_b.release();
+        in_a.release();
+        in_b.release();
+        out.release();
+        carry.release();
+        if (own_stream && stream) hipStreamDestroy(stream);
+    }
+};
+
+Executor* plan_executor();
+
+// Working memory of one operation (see op_spectra_bytes, op_carry_bytes).
+struct OpBuffers {
+    double* spec_a;
+    double* spec_b;  // unused for a square
+    uint32_t* carry;
+};
+uint64_t op_spectra_bytes(const Geometry& g);
+uint64_t op_carry_bytes(uint64_t out_len);
+
+// Whole operations on device operands, queued on stream s.
+bool product_on(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint64_t la, const uint64_t* d_b,
+                uint64_t lb, bool square, const OpBuffers& buf, uint64_t* d_out);
+bool convolution_on(hipStream_t s, const Geometry& g, const ntt_conv_plan* p, const uint64_t* d_a,
+                    const uint64_t* d_b, bool square, const OpBuffers& buf, uint64_t* d_out);
+
+// ---------------------------------------------------------------------------
+// Transform splits and batched products.
+
+struct Split {
+    int n;
+    int logs[3];  // strided passes, then the row pass (last)
+};
+
+Split split_of(int K);
+
+struct BatchLaunch {
+    enum Kind { kFwdStrided, kFwdRows, kFused, kInvStrided, kCrt } kind;
+    int logl;
+    bool flag;  // pack (forward strided), canonical output (inverse strided), square (fused)
+    bool pack;  // fused single pass: pack the operands
+    int units;
+    const char* name;
+    PassArgs pass;
+    CrtArgs crt;
+};
+
+struct BatchBucket {
+    int K;
+    uint32_t all, n_all;  // every (product, prime) item of this length
+    uint32_t ns, n_ns;    // items of products with two operands
+    uint32_t sq, n_sq;    // items of squares
+};
+
+struct Batch {
+    Executor ex;
+    size_t count = 0;
+    std::vector<uint64_t> la, lb;
+    std::vector<uint8_t> square;
+    std::vector<Geometry> geo;
+    std::vector<BatchProduct> host;
+    std::vector<BatchBucket> buckets;
+    uint32_t crt_begin[kMaxPrimes + 1] = {0}, crt_count[kMaxPrimes + 1] = {0};
+    DevBuf products, items, arena, carry;
+    std::vector<BatchLaunch> program;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool bound = false;
+    bool tuned = false;
+    const double *fwd = nullptr, *inv = nullptr;
+    uint64_t tstride = 0;
+
+    ~Batch() { drop_graph(); }
+    void drop_graph() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, const uint8_t* sq);
+bool batch_bind(Batch& B, const uint64_t* const* d_a, const uint64_t* const* d_b, uint64_t* const* d_out);
+// Queues the batch on `stream` (the batch's own stream when null), as one
+// graph replay when use_graph is set, else as plain launches.
+bool batch_run(Batch& B, hipStream_t stream, bool use_graph);
+
+}  // namespace nttamd

@@ -1,7 +1,7 @@
 // Host side of the gfx950 multi-prime NTT engine: catalog constants, device
 // geometry, device context, the product/convolution pipelines and the C ABI
 // declared in include/nttanafis.h.
-#include "arith.h"
+#include "engine.h"
 
 #include <hip/hip_runtime.h>
 
@@ -122,7 +122,7 @@ static void init_catalog() {
     }
 }
 
-static const Catalog& cat() {
+const Catalog& cat() {
     std::call_once(g_cat_once, init_catalog);
     return g_cat;
 }
@@ -131,7 +131,8 @@ static const Catalog& cat() {
 // Errors and profiling.
 
 static thread_local std::string t_error;
-static void set_error(const std::string& e) { t_error = e; }
+void set_error(const std::string& e) { t_error = e; }
+void clear_error() { t_error.clear(); }
 
 struct ProfEntry {
     const char* name;
@@ -146,11 +147,8 @@ static thread_local const char* t_kind = "kernel";
 
 // Names the launches of a scope for per-launch timing (the events themselves
 // are recorded around each dispatch by the launchers in kernels.hip).
-struct Timed {
-    const char* saved;
-    Timed(const char* n, hipStream_t) : saved(t_kind) { t_kind = n; }
-    ~Timed() { t_kind = saved; }
-};
+Timed::Timed(const char* n, hipStream_t) : saved(t_kind) { t_kind = n; }
+Timed::~Timed() { t_kind = saved; }
 
 static thread_local bool t_capturing = false;
 
@@ -186,19 +184,6 @@ static void prof_drain() {
 // ---------------------------------------------------------------------------
 // Geometry of one device operation.
 
-struct Geometry {
-    uint32_t primes = 0;
-    uint32_t width = 0;         // bits between operand coefficients
-    uint32_t value_bits_a = 0;  // significant bits per coefficient
-    uint32_t value_bits_b = 0;
-    uint64_t digits_a = 0, digits_b = 0;
-    uint64_t conv_len = 0;
-    int log_t = 0;
-    uint32_t bound_bits = 0;  // bits of every reconstructed coefficient
-    bool disjoint = false;
-    bool is_signed = false;
-    uint32_t stride = 0;  // limbs per result coefficient (convolution)
-};
 
 static int bitlen(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
 
@@ -218,7 +203,7 @@ static int passes_of(int K) { return K <= kSinglePassMaxLog ? 1 : (K <= 23 ? 2 :
 // Product geometry: for every prefix, the widest coefficient whose products
 // fit its capacity (the narrowing of src/profile/planner.rs:168-183), priced
 // by device work: HBM passes and transform levels over np*T points.
-static bool select_product(uint64_t la, uint64_t lb, Geometry* out) {
+bool select_product(uint64_t la, uint64_t lb, Geometry* out) {
     const Catalog& c = cat();
     if (la == 0 || lb == 0 || la > (UINT64_MAX >> 7) || lb > (UINT64_MAX >> 7)) return false;
     const uint64_t bits_a = la * 64, bits_b = lb * 64;
@@ -272,7 +257,7 @@ static bool select_product(uint64_t la, uint64_t lb, Geometry* out) {
 }
 
 // bound_bits of src/convolve.rs:358-369.
-static bool conv_bound_bits(uint64_t terms, uint64_t max_a, uint64_t max_b, uint64_t* bits) {
+bool conv_bound_bits(uint64_t terms, uint64_t max_a, uint64_t max_b, uint64_t* bits) {
     const u128 product = (u128)max_a * max_b;
     const uint64_t hi = (uint64_t)(product >> 64), lo = (uint64_t)product;
     const u128 t_lo = (u128)lo * terms;
@@ -296,7 +281,7 @@ static double* g_inv = nullptr;
 static uint64_t g_table_len = 0;  // entries per prime
 static std::vector<void*> g_retired;
 
-static bool ensure_tables(uint64_t need, hipStream_t s, const double** fwd, const double** inv,
+bool ensure_tables(uint64_t need, hipStream_t s, const double** fwd, const double** inv,
                           uint64_t* stride) {
     std::lock_guard<std::mutex> g(g_table_lock);
     if (need > g_table_len) {
@@ -335,60 +320,9 @@ static bool ensure_tables(uint64_t need, hipStream_t s, const double** fwd, cons
     return true;
 }
 
-struct DevBuf {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-    bool ensure(size_t n) {
-        if (n <= bytes) return true;
-        if (ptr) hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-        size_t want = std::max<size_t>(n, 4096);
-        if (hipMalloc(&ptr, want) != hipSuccess) {
-            ptr = nullptr;
-            return false;
-        }
-        bytes = want;
-        return true;
-    }
-    void release() {
-        if (ptr) hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-    }
-    template <class T>
-    T* as() const {
-        return static_cast<T*>(ptr);
-    }
-};
 
-struct Executor {
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    DevBuf spec_a, spec_b, in_a, in_b, out, carry;
-    std::mutex lock;
 
-    bool init() {
-        if (stream) return true;
-        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
-            stream = nullptr;
-            return false;
-        }
-        own_stream = true;
-        return true;
-    }
-    ~Executor() {
-        spec_a.release();
-        spec_b.release();
-        in_a.release();
-        in_b.release();
-        out.release();
-        carry.release();
-        if (own_stream && stream) hipStreamDestroy(stream);
-    }
-};
-
-static Executor* plan_executor() {
+Executor* plan_executor() {
     static Executor* ex = new Executor();
     return ex;
 }
@@ -396,12 +330,8 @@ static Executor* plan_executor() {
 // ---------------------------------------------------------------------------
 // Pipelines.
 
-struct Split {
-    int n;
-    int logs[3];  // strided passes, then the row pass (last)
-};
 
-static Split split_of(int K) {
+Split split_of(int K) {
     Split s{};
     if (K <= kSinglePassMaxLog) {
         s.n = 1;
@@ -492,11 +422,16 @@ static bool inverse_tail(const Split& sp, const PassArgs& a0, int primes, hipStr
     return true;
 }
 
-// Computes the canonical residues of the convolution into ex.spec_b (or
-// spec_a for a square / single pass) and returns their pointer.
-static bool residues(Executor& ex, const Geometry& g, const uint64_t* d_a, uint64_t la,
-                     const uint64_t* d_b, uint64_t lb, bool square, double** rows_out) {
-    const hipStream_t s = ex.stream;
+// Device working memory of one product or convolution: the spectra rows
+// (primes x T doubles each; spec_b unused for a square) and the carry words
+// of the reconstruction (5 per output block).
+uint64_t op_spectra_bytes(const Geometry& g) { return (uint64_t)g.primes * ((uint64_t)1 << g.log_t) * 8; }
+uint64_t op_carry_bytes(uint64_t out_len) { return ((out_len + kCrtBlockLimbs - 1) / kCrtBlockLimbs) * 5 * 4; }
+
+// Computes the canonical residues of the convolution into buf.spec_b (or
+// spec_a for a square or a single pass) and returns their pointer.
+static bool residues(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint64_t la, const uint64_t* d_b,
+                     uint64_t lb, bool square, const OpBuffers& buf, double** rows_out) {
     const uint64_t T = (uint64_t)1 << g.log_t;
     const int np = (int)g.primes;
     const double *fwd = nullptr, *inv = nullptr;
@@ -505,16 +440,11 @@ static bool residues(Executor& ex, const Geometry& g, const uint64_t* d_a, uint6
         set_error("root table allocation failed");
         return false;
     }
-    const size_t row_bytes = (size_t)np * T * sizeof(double);
-    if (!ex.spec_a.ensure(row_bytes) || (!square && !ex.spec_b.ensure(row_bytes))) {
-        set_error("spectrum allocation failed");
-        return false;
-    }
     const Source sa = source_of(d_a, la, g.digits_a, g.width, g.value_bits_a, g.is_signed);
     const Source sb = source_of(d_b, lb, g.digits_b, g.width, g.value_bits_b, g.is_signed);
     const Split sp = split_of(g.log_t);
-    double* A = ex.spec_a.as<double>();
-    double* B = square ? A : ex.spec_b.as<double>();
+    double* A = buf.spec_a;
+    double* B = square ? A : buf.spec_b;
     if (sp.n == 1) {
         PassArgs a = base_args(g, A, fwd, inv, tstride);
         a.src_a = sa;
@@ -574,42 +504,63 @@ static CrtArgs crt_args(const Geometry& g, const double* rows, uint64_t* out, ui
     return a;
 }
 
+static bool crt_product_from(hipStream_t s, const Geometry& g, const double* rows, uint64_t out_len,
+                             uint32_t* carry, uint64_t* d_out) {
+    CrtArgs a = crt_args(g, rows, d_out, out_len);
+    a.width = g.width;
+    a.coefficients = std::min<uint64_t>(g.conv_len, (out_len * 64 + g.width - 1) / g.width);
+    const uint64_t blocks = (out_len + kCrtBlockLimbs - 1) / kCrtBlockLimbs;
+    a.blocks = (uint32_t)blocks;
+    a.agg_base = carry;
+    a.agg_thresh = carry + blocks;
+    a.first_thresh = carry + 2 * blocks;
+    a.ripple_end = carry + 3 * blocks;
+    a.carry_in = carry + 4 * blocks;
+    return launch_crt_product(a, s);
+}
+
+bool product_on(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint64_t la, const uint64_t* d_b,
+                uint64_t lb, bool square, const OpBuffers& buf, uint64_t* d_out) {
+    double* rows = nullptr;
+    if (!residues(s, g, d_a, la, d_b, lb, square, buf, &rows)) return false;
+    return crt_product_from(s, g, rows, la + lb, buf.carry, d_out);
+}
+
+bool convolution_on(hipStream_t s, const Geometry& g, const ntt_conv_plan* p, const uint64_t* d_a,
+                    const uint64_t* d_b, bool square, const OpBuffers& buf, uint64_t* d_out) {
+    if (p->schoolbook) {
+        Timed t("conv_columns", s);
+        return launch_conv_columns(d_a, p->len_a, d_b, p->len_b, d_out, (uint32_t)p->stride, p->is_signed, s);
+    }
+    double* rows = nullptr;
+    if (!residues(s, g, d_a, p->len_a, d_b, p->len_b, square, buf, &rows)) return false;
+    CrtArgs a = crt_args(g, rows, d_out, (p->len_a + p->len_b - 1) * g.stride);
+    a.coefficients = p->len_a + p->len_b - 1;
+    a.stride = g.stride;
+    a.width = g.stride * 64;
+    Timed t("crt_disjoint", s);
+    return launch_crt_disjoint(a, s);
+}
+
+// Executor forms: grow the executor's own buffers, then run on its stream.
+static bool exec_buffers(Executor& ex, const Geometry& g, bool square, uint64_t out_len, OpBuffers* buf) {
+    const uint64_t sb = op_spectra_bytes(g);
+    if (!ex.spec_a.ensure(sb) || (!square && !ex.spec_b.ensure(sb)) || !ex.carry.ensure(op_carry_bytes(out_len))) {
+        set_error("device working memory allocation failed");
+        return false;
+    }
+    buf->spec_a = ex.spec_a.as<double>();
+    buf->spec_b = square ? buf->spec_a : ex.spec_b.as<double>();
+    buf->carry = ex.carry.as<uint32_t>();
+    return true;
+}
+
 // Product of device operands into d_out (exactly la+lb limbs).
 static bool device_product(Executor& ex, const Geometry& g, const uint64_t* d_a, uint64_t la,
                            const uint64_t* d_b, uint64_t lb, bool square, uint64_t* d_out) {
-    double* rows = nullptr;
-    if (!residues(ex, g, d_a, la, d_b, lb, square, &rows)) return false;
-    const uint64_t out_len = la + lb;
-    CrtArgs a = crt_args(g, rows, d_out, out_len);
-    a.width = g.width;
-    const uint64_t coefficients = std::min<uint64_t>(g.conv_len, (out_len * 64 + g.width - 1) / g.width);
-    a.coefficients = coefficients;
-    const uint64_t blocks = (out_len + kCrtBlockLimbs - 1) / kCrtBlockLimbs;
-    a.blocks = (uint32_t)blocks;
-    if (!ex.carry.ensure(blocks * 5 * sizeof(uint32_t))) {
-        set_error("carry allocation failed");
-        return false;
-    }
-    uint32_t* cb = ex.carry.as<uint32_t>();
-    a.agg_base = cb;
-    a.agg_thresh = cb + blocks;
-    a.first_thresh = cb + 2 * blocks;
-    a.ripple_end = cb + 3 * blocks;
-    a.carry_in = cb + 4 * blocks;
-    Timed t("crt_carry", ex.stream);
-    return launch_crt_product(a, ex.stream);
-}
-
-static bool device_convolution(Executor& ex, const Geometry& g, const uint64_t* d_a, uint64_t la,
-                               const uint64_t* d_b, uint64_t lb, bool square, uint64_t* d_out) {
-    double* rows = nullptr;
-    if (!residues(ex, g, d_a, la, d_b, lb, square, &rows)) return false;
-    CrtArgs a = crt_args(g, rows, d_out, (la + lb - 1) * g.stride);
-    a.coefficients = la + lb - 1;
-    a.stride = g.stride;
-    a.width = g.stride * 64;
-    Timed t("crt_disjoint", ex.stream);
-    return launch_crt_disjoint(a, ex.stream);
+    OpBuffers buf;
+    if (!exec_buffers(ex, g, square, la + lb, &buf)) return false;
+    return product_on(ex.stream, g, d_a, la, d_b, lb, square, buf, d_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -639,13 +590,17 @@ static bool stage_and_run(Executor& ex, uint64_t out_len, const uint64_t* a, uin
         if (t_error.empty()) set_error("kernel launch failed");
         return false;
     }
-    std::vector<uint64_t> host(out_len);
-    if (hipMemcpyAsync(host.data(), dout, out_len * 8, hipMemcpyDeviceToHost, s) != hipSuccess) return false;
+    // Every kernel has completed before dst is touched: a failure up to here
+    // leaves dst unchanged.
     if (hipStreamSynchronize(s) != hipSuccess) {
         set_error("device execution failed");
         return false;
     }
-    memcpy(dst, host.data(), out_len * 8);
+    if (hipMemcpyAsync(dst, dout, out_len * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        set_error("result copy failed");
+        return false;
+    }
     return true;
 }
 
@@ -668,20 +623,17 @@ struct ConvCtx {
 
 static bool run_conv(Executor& ex, void* ctx, const uint64_t* da, const uint64_t* db, uint64_t* dout) {
     ConvCtx* c = static_cast<ConvCtx*>(ctx);
-    const ntt_conv_plan* p = c->plan;
-    if (p->schoolbook) {
-        Timed t("conv_columns", ex.stream);
-        return launch_conv_columns(da, p->len_a, db, p->len_b, dout, (uint32_t)p->stride, p->is_signed, ex.stream);
-    }
-    return device_convolution(ex, *c->g, da, p->len_a, db, p->len_b, c->square, dout);
+    OpBuffers buf{nullptr, nullptr, nullptr};
+    if (!c->plan->schoolbook && !exec_buffers(ex, *c->g, c->square, 0, &buf)) return false;
+    return convolution_on(ex.stream, *c->g, c->plan, da, db, c->square, buf, dout);
 }
 
-static bool product_geometry(const ntt_mul_plan* plan, Geometry* g) {
+bool product_geometry(const ntt_mul_plan* plan, Geometry* g) {
     if (!plan || !plan->valid) return false;
     return select_product(plan->len_a, plan->len_b, g);
 }
 
-static bool conv_geometry(const ntt_conv_plan* p, Geometry* g) {
+bool conv_geometry(const ntt_conv_plan* p, Geometry* g) {
     if (!p || !p->valid) return false;
     g->primes = p->primes;
     g->width = 64;
@@ -711,7 +663,7 @@ static bool mul_host(Executor& ex, const ntt_mul_plan* plan, uint64_t* dst, size
 }
 
 // ConvPlan::admits, src/convolve.rs:169-184.
-static bool conv_admits(const ntt_conv_plan* p, const uint64_t* a, size_t a_len, const uint64_t* b,
+bool conv_admits(const ntt_conv_plan* p, const uint64_t* a, size_t a_len, const uint64_t* b,
                         size_t b_len) {
     if (a_len != p->len_a || b_len != p->len_b || !a || !b) return false;
     const uint64_t* ops[2] = {a, b};
@@ -812,50 +764,8 @@ static Split tuned_split(int K) {
     return rows ? split_with_rows(K, rows) : split_of(K);
 }
 
-struct BatchLaunch {
-    enum Kind { kFwdStrided, kFwdRows, kFused, kInvStrided, kCrt } kind;
-    int logl;
-    bool flag;  // pack (forward strided), canonical output (inverse strided), square (fused)
-    bool pack;  // fused single pass: pack the operands
-    int units;
-    const char* name;
-    PassArgs pass;
-    CrtArgs crt;
-};
 
-struct BatchBucket {
-    int K;
-    uint32_t all, n_all;  // every (product, prime) item of this length
-    uint32_t ns, n_ns;    // items of products with two operands
-    uint32_t sq, n_sq;    // items of squares
-};
 
-struct Batch {
-    Executor ex;
-    size_t count = 0;
-    std::vector<uint64_t> la, lb;
-    std::vector<uint8_t> square;
-    std::vector<Geometry> geo;
-    std::vector<BatchProduct> host;
-    std::vector<BatchBucket> buckets;
-    uint32_t crt_begin[kMaxPrimes + 1] = {0}, crt_count[kMaxPrimes + 1] = {0};
-    DevBuf products, items, arena, carry;
-    std::vector<BatchLaunch> program;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    bool bound = false;
-    bool tuned = false;
-    const double *fwd = nullptr, *inv = nullptr;
-    uint64_t tstride = 0;
-
-    ~Batch() { drop_graph(); }
-    void drop_graph() {
-        if (exec) hipGraphExecDestroy(exec);
-        if (graph) hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
-    }
-};
 
 static bool issue_launch(const BatchLaunch& L, hipStream_t s) {
     Timed t(L.name, s);
@@ -963,7 +873,7 @@ static void build_program(Batch& B) {
     }
 }
 
-static bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, const uint8_t* sq) {
+bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, const uint8_t* sq) {
     B.drop_graph();
     B.bound = false;
     B.tuned = false;
@@ -1077,7 +987,7 @@ static bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* l
     return true;
 }
 
-static bool batch_bind(Batch& B, const uint64_t* const* d_a, const uint64_t* const* d_b, uint64_t* const* d_out) {
+bool batch_bind(Batch& B, const uint64_t* const* d_a, const uint64_t* const* d_b, uint64_t* const* d_out) {
     for (size_t i = 0; i < B.count; ++i) {
         const uint64_t* pb = B.square[i] ? d_a[i] : d_b[i];
         if (!d_a[i] || !pb || !d_out[i]) {
@@ -1191,7 +1101,7 @@ static bool batch_capture(Batch& B) {
 
 // Queues the whole batch on `stream` (the batch's own stream when null):
 // autotunes on the first run, captures the program once, then replays it.
-static bool batch_run(Batch& B, hipStream_t stream, bool use_graph) {
+bool batch_run(Batch& B, hipStream_t stream, bool use_graph) {
     if (!B.bound) {
         set_error("batch: operands are not bound");
         return false;
@@ -1863,8 +1773,10 @@ bool ntt_debug_residues(const uint64_t* a, size_t la, const uint64_t* b, size_t 
     if (!ex->init() || !ex->in_a.ensure(la * 8) || !ex->in_b.ensure(lb * 8)) return false;
     if (hipMemcpy(ex->in_a.ptr, a, la * 8, hipMemcpyHostToDevice) != hipSuccess) return false;
     if (hipMemcpy(ex->in_b.ptr, b, lb * 8, hipMemcpyHostToDevice) != hipSuccess) return false;
+    OpBuffers buf;
+    if (!exec_buffers(*ex, g, square != 0, 0, &buf)) return false;
     double* rows = nullptr;
-    if (!residues(*ex, g, ex->in_a.as<uint64_t>(), la, ex->in_b.as<uint64_t>(), lb, square != 0, &rows))
+    if (!residues(ex->stream, g, ex->in_a.as<uint64_t>(), la, ex->in_b.as<uint64_t>(), lb, square != 0, buf, &rows))
         return false;
     if (hipStreamSynchronize(ex->stream) != hipSuccess) return false;
     return hipMemcpy(out, rows, g.primes * T * 8, hipMemcpyDeviceToHost) == hipSuccess;
@@ -1879,19 +1791,10 @@ bool ntt_debug_crt(size_t la, size_t lb, const double* rows, uint64_t* out) {
     const uint64_t out_len = la + lb;
     if (!ex->init() || !ex->spec_b.ensure(g.primes * T * 8) || !ex->out.ensure(out_len * 8)) return false;
     if (hipMemcpy(ex->spec_b.ptr, rows, g.primes * T * 8, hipMemcpyHostToDevice) != hipSuccess) return false;
-    CrtArgs c = crt_args(g, ex->spec_b.as<double>(), ex->out.as<uint64_t>(), out_len);
-    c.width = g.width;
-    c.coefficients = std::min<uint64_t>(g.conv_len, (out_len * 64 + g.width - 1) / g.width);
-    const uint64_t blocks = (out_len + kCrtBlockLimbs - 1) / kCrtBlockLimbs;
-    c.blocks = (uint32_t)blocks;
-    if (!ex->carry.ensure(blocks * 5 * sizeof(uint32_t))) return false;
-    uint32_t* cb = ex->carry.as<uint32_t>();
-    c.agg_base = cb;
-    c.agg_thresh = cb + blocks;
-    c.first_thresh = cb + 2 * blocks;
-    c.ripple_end = cb + 3 * blocks;
-    c.carry_in = cb + 4 * blocks;
-    if (!launch_crt_product(c, ex->stream)) return false;
+    if (!ex->carry.ensure(op_carry_bytes(out_len))) return false;
+    if (!crt_product_from(ex->stream, g, ex->spec_b.as<double>(), out_len, ex->carry.as<uint32_t>(),
+                          ex->out.as<uint64_t>()))
+        return false;
     if (hipStreamSynchronize(ex->stream) != hipSuccess) return false;
     return hipMemcpy(out, ex->out.ptr, out_len * 8, hipMemcpyDeviceToHost) == hipSuccess;
 }

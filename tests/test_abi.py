@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from conftest import ROOT
@@ -86,3 +87,42 @@ def test_host_selftest_of_shared_arithmetic(seed):
     """Residue products, packing, Garner and carry functions, compiled for
     the host from the same source as the kernels, against exact integers."""
     assert ntt.load().ntt_selftest_host(seed) == 0
+
+
+def test_pinned_array_owns_its_memory(monkeypatch):
+    """pinned_array views stay valid without other references and free the
+    allocation exactly once, after the last view is gone (host logic; libc
+    malloc stands in for hipHostMalloc)."""
+    import ctypes
+    import gc
+
+    import nttanafis_amd.api as api
+
+    libc = ctypes.CDLL(None)
+    libc.malloc.restype = ctypes.c_void_p
+    libc.malloc.argtypes = [ctypes.c_size_t]
+    libc.free.argtypes = [ctypes.c_void_p]
+    freed = []
+
+    class FakeLib:
+        @staticmethod
+        def ntt_host_alloc(n):
+            return libc.malloc(n)
+
+        @staticmethod
+        def ntt_host_free(p):
+            freed.append(p)
+            libc.free(p)
+
+    monkeypatch.setattr(api, "_lib", lambda: FakeLib)
+    arr = api.pinned_array(1000)
+    view = arr[10:20]
+    arr[:] = np.arange(1000, dtype=np.uint64)
+    del arr
+    gc.collect()
+    assert not freed and int(view[0]) == 10
+    del view
+    gc.collect()
+    assert len(freed) == 1
+    empty = api.pinned_array(0)
+    assert len(empty) == 0
