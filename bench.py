@@ -66,9 +66,12 @@ def workload(name: str):
 def kernel_bytes(name: str, primes: int, T: int, la: int, lb: int, out_len: int, square: bool) -> float:
     """Algorithmic HBM bytes of one launch of each kernel kind: every residue
     of a pass read and written once; packing reads the limbs once."""
-    if name == "fwd_pack_strided":
-        return 8.0 * la + 8.0 * primes * T  # per operand launch (la ~ lb here)
-    if name in ("fwd_rows", "fwd_strided", "inv_strided"):
+    ops = 1 if square else 2
+    if name == "fwd_pack_strided":  # one launch packs and transforms both operands
+        return 8.0 * (la if square else la + lb) + 8.0 * ops * primes * T
+    if name == "fwd_strided":
+        return 16.0 * ops * primes * T
+    if name == "inv_strided":
         return 16.0 * primes * T
     if name == "fused_rows":
         return (16.0 if square else 24.0) * primes * T

@@ -32,68 +32,140 @@ __host__ __device__ __forceinline__ uint64_t window64(const uint64_t* limbs, uin
     return (lo >> off) | ((hi << 1) << (63 - off));
 }
 
-__host__ __device__ __forceinline__ double pack_coef(const Source& s, uint64_t n, const double* pw, double p,
-                                                     double pinv) {
+// Parts of a coefficient as doubles: a part below 2^49 converts exactly from
+// its two 32-bit halves.
+__host__ __device__ __forceinline__ double part_to_double(uint64_t v) {
+    return (double)(uint32_t)(v >> 32) * 4294967296.0 + (double)(uint32_t)v;
+}
+
+// One limb per coefficient (width 64): at most two parts, both in the limb;
+// a signed coefficient is two's complement in its value_bits.
+// With CLAMP false the caller guarantees that coefficient n and every limb
+// its window reads lie inside the operand (n < digits), so no index is
+// clamped and no result masked.
+template <bool SIGNED, bool CLAMP = true>
+__host__ __device__ __forceinline__ double pack_w64(const Source& s, uint64_t n, const double* pw, double p,
+                                                    double pinv) {
     const uint32_t vb = s.value_bits;
-    if (s.width == 64) {
-        // One coefficient per limb: at most two parts, both in the limb.
-        const uint64_t last = s.limb_len - 1;
-        const uint64_t limb = s.limbs[n < last ? n : last];
-        const int high_bits = vb > (uint32_t)kPartBits ? (int)vb - kPartBits : (int)vb;
-        double r;
-        if (vb > (uint32_t)kPartBits) {
-            const double d0 = (double)(limb & ((1ull << kPartBits) - 1));
-            const uint64_t masked = (limb >> kPartBits) & ((1ull << high_bits) - 1);
-            const uint64_t flip = s.is_signed ? 1ull << (high_bits - 1) : 0;
-            const double d1 = (double)((int64_t)(masked ^ flip) - (int64_t)flip);
-            r = reduce(d0 + mulmod(d1, pw[1], p, pinv), p, pinv);
-        } else {
-            const uint64_t masked = limb & ((1ull << high_bits) - 1);
-            const uint64_t flip = s.is_signed ? 1ull << (high_bits - 1) : 0;
-            r = (double)((int64_t)(masked ^ flip) - (int64_t)flip);
-        }
-        return n < s.digits ? r : 0.0;
-    }
-    // General width (at most 192 bits): one window of four limbs covers the
-    // coefficient at any bit offset; its parts are cut from registers.
-    const int parts = (int)((vb + kPartBits - 1) / kPartBits);
-    const int high_bits = (int)vb - kPartBits * (parts - 1);
-    const uint64_t bitpos = n * (uint64_t)s.width;
-    const uint64_t idx = bitpos >> 6;
-    const uint32_t off = (uint32_t)(bitpos & 63);
     const uint64_t last = s.limb_len - 1;
-    uint64_t w[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const uint64_t i = idx + q;
-        const uint64_t raw = s.limbs[i < last ? i : last];
-        w[q] = i < s.limb_len ? raw : 0;
+    const uint64_t limb = CLAMP ? s.limbs[n < last ? n : last] : s.limbs[n];
+    double r;
+    if (vb > (uint32_t)kPartBits) {
+        const int high_bits = (int)vb - kPartBits;
+        const double d0 = part_to_double(limb & ((1ull << kPartBits) - 1));
+        const uint64_t masked = (limb >> kPartBits) & ((1ull << high_bits) - 1);
+        const double d1 = SIGNED ? (double)((int64_t)(masked ^ (1ull << (high_bits - 1))) -
+                                            (int64_t)(1ull << (high_bits - 1)))
+                                 : (double)(uint32_t)masked;
+        r = reduce(d0 + mulmod(d1, pw[1], p, pinv), p, pinv);
+    } else {
+        const uint64_t masked = limb & ((1ull << vb) - 1);
+        r = SIGNED ? (double)((int64_t)(masked ^ (1ull << (vb - 1))) - (int64_t)(1ull << (vb - 1)))
+                   : part_to_double(masked);
     }
+    return !CLAMP || n < s.digits ? r : 0.0;
+}
+
+// General width (at most 192 bits, unsigned) with PARTS = ceil(value_bits /
+// 49) fixed at compile time. Coefficient n starts at bit off of limb idx; a
+// window of W limbs covers it, and part k starts in limb (off + 49k) / 64,
+// which is one of two known limbs.
+template <int PARTS>
+__host__ __device__ constexpr int pack_window() {
+    return (63 + PARTS * kPartBits + 63) / 64 < 4 ? (63 + PARTS * kPartBits + 63) / 64 : 4;
+}
+
+// Residue of the coefficient that starts at bit `off` of w[0], from a window
+// w[0..W] of limbs (w[W] = 0); the top part has high_bits bits.
+template <int PARTS>
+__host__ __device__ __forceinline__ double residue_from_window(const uint64_t* w, uint32_t off, int high_bits,
+                                                               const double* pw, double p, double pinv) {
+    constexpr int W = pack_window<PARTS>();
     double sum = 0.0;
 #pragma unroll
-    for (int k = 0; k < kMaxParts; ++k) {
-        if (k >= parts) break;
+    for (int k = 0; k < PARTS; ++k) {
         const uint32_t bit = off + (uint32_t)(kPartBits * k);
-        const uint32_t q = bit >> 6, sh = bit & 63;
-        const uint64_t lo = q == 0 ? w[0] : q == 1 ? w[1] : q == 2 ? w[2] : w[3];
-        const uint64_t hi = q == 0 ? w[1] : q == 1 ? w[2] : q == 2 ? w[3] : 0;
+        const int q0 = (kPartBits * k) >> 6;  // limb of the part when off is 0
+        const bool next = (bit >> 6) != (uint32_t)q0;
+        const uint32_t sh = bit & 63;
+        const uint64_t lo = next ? w[q0 + 1 < W ? q0 + 1 : W] : w[q0];
+        const uint64_t hi = next ? w[q0 + 2 < W ? q0 + 2 : W] : w[q0 + 1 < W ? q0 + 1 : W];
         const uint64_t word = (lo >> sh) | ((hi << 1) << (63 - sh));
-        double d;
-        if (k + 1 == parts) {
-            const uint64_t masked = word & ((1ull << high_bits) - 1);
-            if (s.is_signed) {
-                const uint64_t flip = 1ull << (high_bits - 1);
-                d = (double)((int64_t)(masked ^ flip) - (int64_t)flip);
-            } else {
-                d = (double)masked;
-            }
-        } else {
-            d = (double)(word & ((1ull << kPartBits) - 1));
-        }
+        const uint64_t mask = k + 1 == PARTS ? ((1ull << high_bits) - 1) : ((1ull << kPartBits) - 1);
+        const double d = part_to_double(word & mask);
         sum = k == 0 ? d : sum + mulmod(d, pw[k], p, pinv);
     }
-    const double r = parts > 1 ? reduce(sum, p, pinv) : sum;
-    return n < s.digits ? r : 0.0;
+    return PARTS > 1 ? reduce(sum, p, pinv) : sum;
+}
+
+template <int PARTS, bool CLAMP = true>
+__host__ __device__ __forceinline__ double pack_parts(const Source& s, uint64_t n, const double* pw, double p,
+                                                      double pinv) {
+    constexpr int W = pack_window<PARTS>();
+    const uint64_t bitpos = n * (uint64_t)s.width;
+    const uint64_t idx = bitpos >> 6;
+    const uint32_t off = (uint32_t)bitpos & 63;
+    const uint64_t last = s.limb_len - 1;
+    uint64_t w[W + 1];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        const uint64_t i = idx + q;
+        if (CLAMP) {
+            const uint64_t raw = s.limbs[i < last ? i : last];
+            w[q] = i < s.limb_len ? raw : 0;
+        } else {
+            w[q] = s.limbs[i];
+        }
+    }
+    w[W] = 0;
+    const int high_bits = (int)s.value_bits - kPartBits * (PARTS - 1);
+    const double r = residue_from_window<PARTS>(w, off, high_bits, pw, p, pinv);
+    return !CLAMP || n < s.digits ? r : 0.0;
+}
+
+// Residue of coefficient n of `s` (dispatch over the forms above; callers in
+// hot loops dispatch once with pack_kind and call the form directly).
+enum PackKind { kPackW64, kPackW64Signed, kPack1, kPack2, kPack3, kPack4 };
+
+__host__ __device__ __forceinline__ int pack_kind(const Source& s) {
+    if (s.width == 64) return s.is_signed ? kPackW64Signed : kPackW64;
+    const int parts = (int)((s.value_bits + kPartBits - 1) / kPartBits);
+    return kPack1 + parts - 1;
+}
+
+template <int KIND, bool CLAMP = true>
+__host__ __device__ __forceinline__ double pack_as(const Source& s, uint64_t n, const double* pw, double p,
+                                                   double pinv) {
+    if constexpr (KIND == kPackW64) return pack_w64<false, CLAMP>(s, n, pw, p, pinv);
+    if constexpr (KIND == kPackW64Signed) return pack_w64<true, CLAMP>(s, n, pw, p, pinv);
+    if constexpr (KIND >= kPack1) return pack_parts<KIND - kPack1 + 1, CLAMP>(s, n, pw, p, pinv);
+    return 0.0;
+}
+
+// True when coefficients [0, n_end) and every limb their windows read lie
+// inside the operand, so pack_as<KIND, false> may be used for them.
+template <int KIND>
+__host__ __device__ __forceinline__ bool pack_interior(const Source& s, uint64_t n_end) {
+    if (n_end > s.digits) return false;
+    if constexpr (KIND == kPackW64 || KIND == kPackW64Signed) {
+        return n_end <= s.limb_len;
+    } else {
+        if (n_end == 0) return true;
+        const uint64_t last_idx = (((n_end - 1) * (uint64_t)s.width) >> 6) + pack_window<KIND - kPack1 + 1>() - 1;
+        return last_idx < s.limb_len;
+    }
+}
+
+__host__ __device__ __forceinline__ double pack_coef(const Source& s, uint64_t n, const double* pw, double p,
+                                                     double pinv) {
+    switch (pack_kind(s)) {
+        case kPackW64: return pack_as<kPackW64>(s, n, pw, p, pinv);
+        case kPackW64Signed: return pack_as<kPackW64Signed>(s, n, pw, p, pinv);
+        case kPack1: return pack_as<kPack1>(s, n, pw, p, pinv);
+        case kPack2: return pack_as<kPack2>(s, n, pw, p, pinv);
+        case kPack3: return pack_as<kPack3>(s, n, pw, p, pinv);
+        default: return pack_as<kPack4>(s, n, pw, p, pinv);
+    }
 }
 
 template <int NP>

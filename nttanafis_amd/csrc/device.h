@@ -74,19 +74,23 @@ struct BatchProduct {
 };
 
 // Work items of a batched launch: product index and prime (passes) or
-// product index and carry block (reconstruction).
+// product index and carry block (reconstruction). In a forward pass, the
+// prime carries kItemOperandB when the item transforms the second operand.
 struct BatchItem {
     uint32_t product;
     uint32_t sub;
 };
+constexpr uint32_t kItemOperandB = 0x80000000u;
 
-// Which rows of a batched product a pass works on.
-enum BatchOperand : int32_t { kOperandA = 0, kOperandB = 1, kOperandProduct = 2 };
+// Which rows of a batched product a pass works on: forward passes take the
+// operand each item names; product passes take the product rows.
+enum BatchOperand : int32_t { kOperandForward = 0, kOperandProduct = 2 };
 
 // Arguments of every transform pass kernel.
 struct PassArgs {
     double* data;            // rows of one operand, prime y at data + y * row_stride
-    const double* other;     // fused passes: the other operand's spectra (same layout)
+    double* other;           // rows of operand b in forward passes over both operands
+                             // (grid z = 1); rows of operand a in a fused pass
     const double* fwd;       // forward root table, prime y at fwd + y * table_stride
     const double* inv;       // inverse root table
     uint64_t row_stride;     // T

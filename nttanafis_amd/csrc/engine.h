@@ -135,7 +135,7 @@ struct Split {
 Split split_of(int K);
 
 struct BatchLaunch {
-    enum Kind { kFwdStrided, kFwdRows, kFused, kInvStrided, kCrt } kind;
+    enum Kind { kFwdStrided, kFused, kInvStrided, kCrt } kind;
     int logl;
     bool flag;  // pack (forward strided), canonical output (inverse strided), square (fused)
     bool pack;  // fused single pass: pack the operands
@@ -147,9 +147,9 @@ struct BatchLaunch {
 
 struct BatchBucket {
     int K;
-    uint32_t all, n_all;  // every (product, prime) item of this length
-    uint32_t ns, n_ns;    // items of products with two operands
-    uint32_t sq, n_sq;    // items of squares
+    uint32_t fwd, n_fwd;  // forward items: every first operand, then every second one
+    uint32_t ns, n_ns;    // product items of products with two operands
+    uint32_t sq, n_sq;    // product items of squares (follow ns: ns..sq is every product)
 };
 
 struct Batch {

@@ -19,9 +19,8 @@ namespace nttamd {
 
 typedef unsigned __int128 u128;
 
-bool launch_fwd_strided(int logl, const PassArgs& a, int primes, bool pack, hipStream_t s);
+bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, bool pack, hipStream_t s);
 bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, hipStream_t s);
-bool launch_fwd_rows(int logl, const PassArgs& a, int primes, hipStream_t s);
 bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, hipStream_t s);
 bool launch_crt_product(const CrtArgs& a, hipStream_t s);
 bool launch_crt_disjoint(const CrtArgs& a, hipStream_t s);
@@ -394,16 +393,20 @@ static Source source_of(const uint64_t* limbs, uint64_t len, uint64_t digits, ui
 
 // Forward transform of operand `src` into `rows` through all passes but the
 // row pass; returns the args of the row pass.
-static bool forward_head(const Split& sp, const PassArgs& a0, const Source& src, int primes,
-                         hipStream_t s) {
+// Strided forward passes of the transform (all but the row pass). With
+// operands == 2 every launch covers both operands: a0.data with source sa and
+// a0.other with source sb.
+static bool forward_head(const Split& sp, const PassArgs& a0, const Source& sa, const Source& sb, int operands,
+                         int primes, hipStream_t s) {
     PassArgs a = a0;
-    a.src_a = src;
+    a.src_a = sa;
+    a.src_b = sb;
     int level = 0;
     for (int i = 0; i + 1 < sp.n; ++i) {
         a.level_start = level;
         a.line_stride = (uint64_t)1 << (a.log_t - level - sp.logs[i]);
         Timed t(i == 0 ? "fwd_pack_strided" : "fwd_strided", s);
-        if (!launch_fwd_strided(sp.logs[i], a, primes, i == 0, s)) return false;
+        if (!launch_fwd_strided(sp.logs[i], a, primes, operands, i == 0, s)) return false;
         level += sp.logs[i];
     }
     return true;
@@ -457,19 +460,11 @@ static bool residues(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint
     }
     const int row_log = sp.logs[sp.n - 1];
     const int row_level = g.log_t - row_log;
-    // Operand a: every forward pass.
+    // Strided forward passes of both operands, then one fused row pass:
+    // forward rows of both, pointwise product, inverse rows.
     PassArgs pa = base_args(g, A, fwd, inv, tstride);
-    if (!forward_head(sp, pa, sa, np, s)) return false;
-    if (!square) {
-        pa.level_start = row_level;
-        pa.line_stride = 1;
-        {
-            Timed t("fwd_rows", s);
-            if (!launch_fwd_rows(row_log, pa, np, s)) return false;
-        }
-        PassArgs pb = base_args(g, B, fwd, inv, tstride);
-        if (!forward_head(sp, pb, sb, np, s)) return false;
-    }
+    pa.other = B;
+    if (!forward_head(sp, pa, sa, sb, square ? 1 : 2, np, s)) return false;
     PassArgs pf = base_args(g, B, fwd, inv, tstride);
     pf.other = A;
     pf.level_start = row_level;
@@ -770,8 +765,7 @@ static Split tuned_split(int K) {
 static bool issue_launch(const BatchLaunch& L, hipStream_t s) {
     Timed t(L.name, s);
     switch (L.kind) {
-        case BatchLaunch::kFwdStrided: return launch_fwd_strided(L.logl, L.pass, L.units, L.flag, s);
-        case BatchLaunch::kFwdRows: return launch_fwd_rows(L.logl, L.pass, L.units, s);
+        case BatchLaunch::kFwdStrided: return launch_fwd_strided(L.logl, L.pass, L.units, 1, L.flag, s);
         case BatchLaunch::kFused: return launch_fused(L.logl, L.pass, L.units, L.pack, L.flag, s);
         case BatchLaunch::kInvStrided: return launch_inv_strided(L.logl, L.pass, L.units, L.flag, s);
         case BatchLaunch::kCrt: return launch_crt_product(L.crt, s);
@@ -785,12 +779,12 @@ static bool issue_program(const std::vector<BatchLaunch>& prog, hipStream_t s) {
     return true;
 }
 
-// Appends the transform launches of one bucket for the given split: forward
-// passes of every first operand, forward passes of every second operand, the
-// fused row pass (forward, pointwise product, inverse) and the inverse
-// strided passes, the last of which leaves canonical residues.
+// Appends the transform launches of one bucket for the given split: the
+// strided forward passes over both operands of every product, the fused row
+// pass (forward rows of both operands, pointwise product, inverse rows) and
+// the inverse strided passes, the last of which leaves canonical residues.
 static void bucket_program(std::vector<BatchLaunch>& prog, const PassArgs& base, int K, const Split& sp,
-                           const BatchItem* I, const BatchBucket& r, bool forward_a_only) {
+                           const BatchItem* I, const BatchBucket& r) {
     auto pass = [&](BatchLaunch::Kind kind, int logl, bool flag, bool pack, int level, uint32_t first,
                     uint32_t units, int operand, const char* name) {
         if (units == 0) return;
@@ -806,13 +800,10 @@ static void bucket_program(std::vector<BatchLaunch>& prog, const PassArgs& base,
         L.pass.items = I + first;
         L.pass.operand = operand;
         L.pass.level_start = level;
-        L.pass.line_stride = kind == BatchLaunch::kFwdRows || kind == BatchLaunch::kFused
-                                 ? 1
-                                 : (uint64_t)1 << (K - level - logl);
+        L.pass.line_stride = kind == BatchLaunch::kFused ? 1 : (uint64_t)1 << (K - level - logl);
         prog.push_back(L);
     };
     if (sp.n == 1) {
-        if (forward_a_only) return;
         pass(BatchLaunch::kFused, K, false, true, 0, r.ns, r.n_ns, kOperandProduct, "batch_fused_single");
         pass(BatchLaunch::kFused, K, true, true, 0, r.sq, r.n_sq, kOperandProduct, "batch_fused_single");
         return;
@@ -821,15 +812,7 @@ static void bucket_program(std::vector<BatchLaunch>& prog, const PassArgs& base,
     const int row_level = K - row_log;
     int level = 0;
     for (int i = 0; i + 1 < sp.n; ++i) {
-        pass(BatchLaunch::kFwdStrided, sp.logs[i], i == 0, false, level, r.all, r.n_all, kOperandA,
-             i == 0 ? "batch_fwd_pack_strided" : "batch_fwd_strided");
-        level += sp.logs[i];
-    }
-    pass(BatchLaunch::kFwdRows, row_log, false, false, row_level, r.ns, r.n_ns, kOperandA, "batch_fwd_rows");
-    if (forward_a_only) return;
-    level = 0;
-    for (int i = 0; i + 1 < sp.n; ++i) {
-        pass(BatchLaunch::kFwdStrided, sp.logs[i], i == 0, false, level, r.ns, r.n_ns, kOperandB,
+        pass(BatchLaunch::kFwdStrided, sp.logs[i], i == 0, false, level, r.fwd, r.n_fwd, kOperandForward,
              i == 0 ? "batch_fwd_pack_strided" : "batch_fwd_strided");
         level += sp.logs[i];
     }
@@ -838,7 +821,7 @@ static void bucket_program(std::vector<BatchLaunch>& prog, const PassArgs& base,
     level = row_level;
     for (int i = sp.n - 2; i >= 0; --i) {
         level -= sp.logs[i];
-        pass(BatchLaunch::kInvStrided, sp.logs[i], i == 0, false, level, r.all, r.n_all, kOperandProduct,
+        pass(BatchLaunch::kInvStrided, sp.logs[i], i == 0, false, level, r.ns, r.n_ns + r.n_sq, kOperandProduct,
              i == 0 ? "batch_inv_strided_canon" : "batch_inv_strided");
     }
 }
@@ -856,7 +839,7 @@ static void build_program(Batch& B) {
     B.program.clear();
     const BatchItem* I = B.items.as<BatchItem>();
     for (const BatchBucket& r : B.buckets)
-        bucket_program(B.program, bucket_base(B, r.K), r.K, tuned_split(r.K), I, r, false);
+        bucket_program(B.program, bucket_base(B, r.K), r.K, tuned_split(r.K), I, r);
     for (int np = 1; np <= kMaxPrimes; ++np) {
         if (!B.crt_count[np]) continue;
         BatchLaunch L;
@@ -946,26 +929,31 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
     // every (product, output block) pair.
     std::vector<BatchItem> items;
     B.buckets.clear();
+    // which: 0 first operands, 1 second operands (forward items); 2 products
+    // with two operands, 3 squares (product items).
     auto add = [&](int K, int which) {
         for (size_t i = 0; i < count; ++i) {
             if (B.geo[i].log_t != K) continue;
-            if (which == 1 && B.square[i]) continue;
-            if (which == 2 && !B.square[i]) continue;
-            for (uint32_t y = 0; y < B.geo[i].primes; ++y) items.push_back({(uint32_t)i, y});
+            const bool sq = B.square[i] != 0;
+            if ((which == 1 || which == 2) && sq) continue;
+            if (which == 3 && !sq) continue;
+            const uint32_t flag = which == 1 ? kItemOperandB : 0u;
+            for (uint32_t y = 0; y < B.geo[i].primes; ++y) items.push_back({(uint32_t)i, y | flag});
         }
     };
     for (int K = kMinLog; K <= kMaxLog; ++K) {
         BatchBucket r;
         r.K = K;
-        r.all = (uint32_t)items.size();
+        r.fwd = (uint32_t)items.size();
         add(K, 0);
-        r.n_all = (uint32_t)items.size() - r.all;
-        if (!r.n_all) continue;
-        r.ns = (uint32_t)items.size();
+        if ((uint32_t)items.size() == r.fwd) continue;
         add(K, 1);
+        r.n_fwd = (uint32_t)items.size() - r.fwd;
+        r.ns = (uint32_t)items.size();
+        add(K, 2);
         r.n_ns = (uint32_t)items.size() - r.ns;
         r.sq = (uint32_t)items.size();
-        add(K, 2);
+        add(K, 3);
         r.n_sq = (uint32_t)items.size() - r.sq;
         B.buckets.push_back(r);
     }
@@ -1026,9 +1014,9 @@ static float time_launches(const std::vector<BatchLaunch>& prog, hipStream_t s, 
     return ms;
 }
 
-// Times the forward passes of every first operand of each untuned bucket
-// under each candidate split and keeps the fastest. The operands must be
-// bound: the timed passes read them.
+// Times the transform work of each untuned bucket under each candidate split
+// and keeps the fastest. The operands must be bound: the timed passes read
+// them.
 static bool batch_autotune(Batch& B) {
     const BatchItem* I = B.items.as<BatchItem>();
     bool changed = false;
@@ -1042,12 +1030,12 @@ static bool batch_autotune(Batch& B) {
         if (cand.size() > 1) {
             float best_ms = 1e30f;
             for (int rows : cand) {
-                std::vector<BatchLaunch> fwd;
-                BatchBucket all = r;
-                all.ns = r.all;  // time the row pass over every item
-                all.n_ns = r.n_all;
-                bucket_program(fwd, bucket_base(B, r.K), r.K, split_with_rows(r.K, rows), I, all, true);
-                const float ms = time_launches(fwd, B.ex.stream, 3);
+                // The bucket's whole transform work under this split (it
+                // recomputes everything from the operands, so repeats are
+                // harmless).
+                std::vector<BatchLaunch> prog;
+                bucket_program(prog, bucket_base(B, r.K), r.K, split_with_rows(r.K, rows), I, r);
+                const float ms = time_launches(prog, B.ex.stream, 3);
                 if (ms < best_ms) {
                     best_ms = ms;
                     best = rows;
@@ -1663,6 +1651,7 @@ int ntt_selftest_host(uint64_t seed) {
         // Packing of every chunk width shape against exact digits.
         static const uint32_t widths[] = {22, 49, 50, 64, 69, 98, 99, 139, 143, 147, 191, 192};
         uint64_t limbs[24];
+        int interior = 0;
         for (uint32_t width : widths) {
             for (int t = 0; t < 3; ++t) {
                 for (auto& l : limbs) l = t == 2 ? ~0ull : rng.next();
@@ -1686,9 +1675,30 @@ int ntt_selftest_host(uint64_t seed) {
                     if (n >= s.digits) r = 0;
                     const double got = pack_coef(s, n, c.powers[y], pf, pinv);
                     if (!check_mod(got, (i128)r, p) || fabs(got) >= pf) ++fails;
+                    // The unclamped form, wherever the tile test admits it.
+                    const int kind = pack_kind(s);
+                    double fast = got;
+                    bool admitted = false;
+                    switch (kind) {
+#define NTT_FAST_CASE(K)                                                  \
+    case K:                                                               \
+        if ((admitted = pack_interior<K>(s, n + 1)))                      \
+            fast = pack_as<K, false>(s, n, c.powers[y], pf, pinv);        \
+        break;
+                        NTT_FAST_CASE(kPackW64)
+                        NTT_FAST_CASE(kPack1)
+                        NTT_FAST_CASE(kPack2)
+                        NTT_FAST_CASE(kPack3)
+                        NTT_FAST_CASE(kPack4)
+#undef NTT_FAST_CASE
+                        default: break;
+                    }
+                    if (fast != got) ++fails;
+                    if (admitted) ++interior;
                 }
             }
         }
+        if (interior == 0) ++fails;  // the unclamped form must have been exercised
         // Signed whole-limb coefficients of 1..64 bits with their sign.
         for (uint32_t vb : {1u, 2u, 16u, 33u, 49u, 50u, 63u, 64u}) {
             for (int t = 0; t < 24; ++t) {
@@ -1708,6 +1718,9 @@ int ntt_selftest_host(uint64_t seed) {
             for (uint64_t n = 0; n < 24; ++n) {
                 const double got = pack_coef(s, n, c.powers[y], pf, pinv);
                 if (!check_mod(got, (i128)(int64_t)limbs[n], p) || fabs(got) >= pf) ++fails;
+                if (pack_interior<kPackW64Signed>(s, n + 1) &&
+                    pack_as<kPackW64Signed, false>(s, n, c.powers[y], pf, pinv) != got)
+                    ++fails;
             }
         }
     }

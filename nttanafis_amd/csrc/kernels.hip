@@ -260,9 +260,9 @@ __device__ __forceinline__ void store_strided(const double* lds, double* g, uint
 
 // Strided tile packed from limbs: element m of line c is coefficient
 // n0 + m*S + c of the operand.
-template <int LOGL, int LINES, int NT>
-__device__ __forceinline__ void pack_strided(double* lds, const Source& src, uint64_t n0, uint64_t S,
-                                             const double* pw, double p, double pinv) {
+template <int LOGL, int LINES, int NT, int KIND, bool CLAMP>
+__device__ __forceinline__ void pack_strided_as(double* lds, const Source& src, uint64_t n0, uint64_t S,
+                                                const double* pw, double p, double pinv) {
     constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
     constexpr int TOTAL = LINES * L;
@@ -275,13 +275,40 @@ __device__ __forceinline__ void pack_strided(double* lds, const Source& src, uin
         for (int r = 0; r < BATCH; ++r) {
             const int f = threadIdx.x + (r0 + r) * NT;
             if (TOTAL % NT == 0 || f < TOTAL)
-                v[r] = pack_coef(src, n0 + (uint64_t)(f / LINES) * S + (f % LINES), pw, p, pinv);
+                v[r] = pack_as<KIND, CLAMP>(src, n0 + (uint64_t)(f / LINES) * S + (f % LINES), pw, p, pinv);
         }
 #pragma unroll
         for (int r = 0; r < BATCH; ++r) {
             const int f = threadIdx.x + (r0 + r) * NT;
             if (TOTAL % NT == 0 || f < TOTAL) lds[(f % LINES) * LS + pidx(f / LINES)] = v[r];
         }
+    }
+}
+
+// Packs a strided tile from limbs: one uniform dispatch on the source's form
+// and on whether the tile lies inside the operand (no clamped loads), then a
+// loop specialised for both.
+template <int LOGL, int LINES, int NT, int KIND>
+__device__ __forceinline__ void pack_tile_as(double* lds, const Source& src, uint64_t n0, uint64_t S,
+                                             const double* pw, double p, double pinv) {
+    constexpr int L = 1 << LOGL;
+    const uint64_t n_end = n0 + (uint64_t)(L - 1) * S + LINES;
+    if (pack_interior<KIND>(src, n_end))
+        pack_strided_as<LOGL, LINES, NT, KIND, false>(lds, src, n0, S, pw, p, pinv);
+    else
+        pack_strided_as<LOGL, LINES, NT, KIND, true>(lds, src, n0, S, pw, p, pinv);
+}
+
+template <int LOGL, int LINES, int NT>
+__device__ __forceinline__ void pack_strided(double* lds, const Source& src, uint64_t n0, uint64_t S,
+                                             const double* pw, double p, double pinv) {
+    switch (pack_kind(src)) {
+        case kPackW64: pack_tile_as<LOGL, LINES, NT, kPackW64>(lds, src, n0, S, pw, p, pinv); break;
+        case kPackW64Signed: pack_tile_as<LOGL, LINES, NT, kPackW64Signed>(lds, src, n0, S, pw, p, pinv); break;
+        case kPack1: pack_tile_as<LOGL, LINES, NT, kPack1>(lds, src, n0, S, pw, p, pinv); break;
+        case kPack2: pack_tile_as<LOGL, LINES, NT, kPack2>(lds, src, n0, S, pw, p, pinv); break;
+        case kPack3: pack_tile_as<LOGL, LINES, NT, kPack3>(lds, src, n0, S, pw, p, pinv); break;
+        default: pack_tile_as<LOGL, LINES, NT, kPack4>(lds, src, n0, S, pw, p, pinv); break;
     }
 }
 
@@ -334,7 +361,7 @@ __device__ __forceinline__ void store_rows(const double* lds, double* g) {
 
 struct Job {
     double* data;          // rows of the operand this pass transforms
-    const double* other;   // fused passes: the other operand's spectra
+    const double* other;   // fused passes: the rows of operand a
     Source src_a;          // packing sources, held by value so that the
     Source src_b;          // packing loops never reload them
     int y;                 // prime
@@ -357,9 +384,12 @@ __device__ __forceinline__ bool xcd_tile(int primes, uint64_t tiles, uint64_t* t
 template <bool BATCH>
 __device__ __forceinline__ bool resolve(const PassArgs& a, uint64_t tiles, bool xcd, Job* j) {
     if constexpr (!BATCH) {
-        j->data = a.data;
+        // Forward passes over both operands in one launch: z = 1 is operand b,
+        // whose rows are a.other and whose source is a.src_b.
+        const bool opb = blockIdx.z != 0;
+        j->data = opb ? a.other : a.data;
         j->other = a.other;
-        j->src_a = a.src_a;
+        j->src_a = opb ? a.src_b : a.src_a;
         j->src_b = a.src_b;
         if (xcd) return xcd_tile(a.primes, tiles, &j->tile, &j->y);
         j->tile = blockIdx.x;
@@ -369,17 +399,17 @@ __device__ __forceinline__ bool resolve(const PassArgs& a, uint64_t tiles, bool 
     const BatchItem it = a.items[blockIdx.x / a.tiles_per_item];
     const BatchProduct& P = a.batch[it.product];
     j->tile = blockIdx.x % a.tiles_per_item;
-    j->y = (int)it.sub;
+    j->y = (int)(it.sub & ~kItemOperandB);
     j->src_a = P.src_a;
     j->src_b = P.src_b;
     j->other = P.spec_a;
-    if (a.operand == kOperandA) {
-        j->data = P.spec_a;
-    } else if (a.operand == kOperandB) {
+    if (a.operand == kOperandProduct) {
+        j->data = P.square ? P.spec_a : P.spec_b;
+    } else if (it.sub & kItemOperandB) {
         j->data = P.spec_b;
         j->src_a = P.src_b;
     } else {
-        j->data = P.square ? P.spec_a : P.spec_b;
+        j->data = P.spec_a;
     }
     return true;
 }
@@ -442,36 +472,20 @@ __global__ __launch_bounds__(NT) void k_inv_strided(PassArgs a) {
 __host__ __device__ constexpr int row_split(int logl) { return logl > 8 ? logl - 4 : 64; }
 
 // ---------------------------------------------------------------------------
-// Row passes: rows H0..H0+LINES-1 of 2^LOGL contiguous points, the last
-// levels of the transform.
-
-template <int LOGL, int LINES, int NT, bool BATCH>
-__global__ __launch_bounds__(NT) void k_fwd_rows(PassArgs a) {
-    constexpr int L = 1 << LOGL;
-    constexpr int LS = Lay<LOGL>::STRIDE;
-    __shared__ double lds[LINES * LS];
-    Job j;
-    resolve<BATCH>(a, 0, false, &j);
-    const int y = j.y;
-    const double p = a.field.p[y], pinv = a.field.pinv[y];
-    const uint64_t H0 = j.tile * LINES;
-    double* row = j.data + (uint64_t)y * a.row_stride + H0 * L;
-    load_rows<LOGL, LINES, NT>(lds, row);
-    __syncthreads();
-    const Tw tw{a.fwd + (uint64_t)y * a.table_stride, H0, 1, row_split(LOGL), p, pinv};
-    fwd_rounds<LOGL, LINES, NT, 0>(lds, tw, p, pinv);
-    store_rows<LOGL, LINES, NT>(lds, row);
-}
-
-// Fused row pass: forward levels of one row, pointwise product with the other
-// operand's spectrum (or a square), inverse levels, store. With PACK the row
-// is the whole transform (level_start == 0): both operands are packed from
-// limbs into LDS and the result is stored as canonical residues times T^-1.
+// Row pass: row H of 2^LOGL contiguous points holds the last levels of the
+// transform.
+//
+// Fused row pass: forward levels of the row of each operand, pointwise
+// product (or a square), inverse levels, store. The rows of b are j.data and
+// those of a are j.other; the product replaces the rows of b. With PACK the
+// row is the whole transform (level_start == 0): both operands are packed
+// from limbs into LDS and the result is stored as canonical residues times
+// T^-1.
 template <int LOGL, int NT, bool PACK, bool SQUARE, bool BATCH>
 __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
-    constexpr int TILES = (PACK && !SQUARE) ? 2 : 1;
+    constexpr int TILES = SQUARE ? 1 : 2;
     constexpr int R = (L + NT - 1) / NT;
     constexpr bool FULL = L % NT == 0;
     __shared__ double lds[TILES * LS];
@@ -486,12 +500,14 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
         if constexpr (!SQUARE) pack_strided<LOGL, 1, NT>(lds + LS, j.src_b, 0, 1, a.powers[y], p, pinv);
     } else {
         load_rows<LOGL, 1, NT>(lds, row);
+        if constexpr (!SQUARE) load_rows<LOGL, 1, NT>(lds + LS, j.other + (uint64_t)y * a.row_stride + H * L);
     }
     __syncthreads();
     const Tw twF{a.fwd + (uint64_t)y * a.table_stride, H, 0, PACK ? 64 : row_split(LOGL), p, pinv};
     const Tw twI{a.inv + (uint64_t)y * a.table_stride, H, 0, PACK ? 64 : row_split(LOGL), p, pinv};
-    fwd_rounds<LOGL, 1, NT, 0>(lds, twF, p, pinv);
-    if constexpr (TILES == 2) fwd_rounds<LOGL, 1, NT, 0>(lds + LS, twF, p, pinv);
+    // Both tiles share the row's roots (hstep 0), so one set of rounds
+    // transforms them together.
+    fwd_rounds<LOGL, TILES, NT, 0>(lds, twF, p, pinv);
     if constexpr (SQUARE) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -500,24 +516,12 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
             const double v = lds[pidx(m)];
             lds[pidx(m)] = mulmod(v, v, p, pinv);
         }
-    } else if constexpr (PACK) {
+    } else {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int m = threadIdx.x + r * NT;
             if (!FULL && m >= L) continue;
             lds[pidx(m)] = mulmod(lds[pidx(m)], lds[LS + pidx(m)], p, pinv);
-        }
-    } else {
-        const double* other = j.other + (uint64_t)y * a.row_stride + H * L;
-        double o[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-            if (FULL || threadIdx.x + r * NT < L) o[r] = other[threadIdx.x + r * NT];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int m = threadIdx.x + r * NT;
-            if (!FULL && m >= L) continue;
-            lds[pidx(m)] = mulmod(lds[pidx(m)], o[r], p, pinv);
         }
     }
     __syncthreads();
@@ -807,8 +811,12 @@ __global__ __launch_bounds__(256) void k_crt_disjoint(CrtArgs a) {
 
 #define NTT_CHECK_LAUNCH() (hipGetLastError() == hipSuccess)
 
+// Lines per strided tile: short lines are taken 64 or 32 at a time, so every
+// point of the tile moves in runs of at least 256 bytes.
 template <int LOGL>
-static constexpr int strided_lines() { return LOGL <= 10 ? 16 : (LOGL == 11 ? 8 : 4); }
+static constexpr int strided_lines() {
+    return LOGL <= 6 ? 64 : LOGL == 7 ? 32 : LOGL <= 10 ? 16 : (LOGL == 11 ? 8 : 4);
+}
 
 // Threads of a strided tile: one radix-16 group per thread and round.
 template <int LOGL>
@@ -819,17 +827,18 @@ static constexpr int strided_threads() {
 // In a batched launch (a.batch set) `units` counts work items (product,
 // prime); otherwise it counts primes.
 template <int LOGL>
-static bool launch_fwd_strided_t(const PassArgs& a, int units, bool pack, hipStream_t s) {
+static bool launch_fwd_strided_t(const PassArgs& a, int units, int operands, bool pack, hipStream_t s) {
     constexpr int LINES = strided_lines<LOGL>();
     constexpr int NT = strided_threads<LOGL>();
+    if (a.line_stride < (uint64_t)LINES) return false;
     const uint64_t nodes = (uint64_t)1 << a.level_start;
     const uint64_t tiles = nodes * (a.line_stride / LINES);
     PassArgs b = a;
     b.primes = units;
     b.tiles_per_item = tiles;
-    dim3 grid((unsigned)tiles, (unsigned)units);
+    dim3 grid((unsigned)tiles, (unsigned)units, (unsigned)operands);
     if (a.batch) grid = dim3((unsigned)(tiles * units), 1);
-    else if (pack) grid = dim3((unsigned)(((tiles + 7) / 8) * 8 * units), 1);
+    else if (pack) grid = dim3((unsigned)(((tiles + 7) / 8) * 8 * units), 1, (unsigned)operands);
     if (a.batch) {
         if (pack)
             NTT_LAUNCH(nullptr, (k_fwd_strided<LOGL, LINES, NT, true, true>), grid, dim3(NT), s, b);
@@ -848,6 +857,7 @@ template <int LOGL>
 static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, hipStream_t s) {
     constexpr int LINES = strided_lines<LOGL>();
     constexpr int NT = strided_threads<LOGL>();
+    if (a.line_stride < (uint64_t)LINES) return false;
     const uint64_t nodes = (uint64_t)1 << a.level_start;
     const uint64_t tiles = nodes * (a.line_stride / LINES);
     PassArgs b = a;
@@ -868,20 +878,6 @@ static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, hipSt
     return NTT_CHECK_LAUNCH();
 }
 
-template <int LOGL>
-static bool launch_fwd_rows_t(const PassArgs& a, int units, hipStream_t s) {
-    constexpr int LINES = 1;
-    const uint64_t rows = ((uint64_t)1 << a.level_start) / LINES;
-    PassArgs b = a;
-    b.tiles_per_item = rows;
-    dim3 grid((unsigned)rows, (unsigned)units);
-    if (a.batch) grid = dim3((unsigned)(rows * units), 1);
-    if (a.batch)
-        NTT_LAUNCH(nullptr, (k_fwd_rows<LOGL, LINES, 256, true>), grid, dim3(256), s, b);
-    else
-        NTT_LAUNCH(nullptr, (k_fwd_rows<LOGL, LINES, 256, false>), grid, dim3(256), s, b);
-    return NTT_CHECK_LAUNCH();
-}
 
 template <int LOGL>
 static bool launch_fused_t(const PassArgs& a, int units, bool pack, bool square, hipStream_t s) {
@@ -890,38 +886,39 @@ static bool launch_fused_t(const PassArgs& a, int units, bool pack, bool square,
     b.tiles_per_item = rows;
     dim3 grid((unsigned)rows, (unsigned)units);
     if (a.batch) grid = dim3((unsigned)(rows * units), 1);
-    constexpr int NT = LOGL <= 8 ? 64 : 256;
+    constexpr int NT1 = LOGL <= 8 ? 64 : 256;  // one tile (square)
+    constexpr int NT = 2 * NT1;                  // two tiles
     if (a.batch) {
         if (pack && square)
-            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, true, true, true>), grid, dim3(NT), s, b);
+            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT1, true, true, true>), grid, dim3(NT1), s, b);
         else if (pack)
             NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, true, false, true>), grid, dim3(NT), s, b);
         else if (square)
-            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, false, true, true>), grid, dim3(NT), s, b);
+            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT1, false, true, true>), grid, dim3(NT1), s, b);
         else
             NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, false, false, true>), grid, dim3(NT), s, b);
     } else {
         if (pack && square)
-            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, true, true, false>), grid, dim3(NT), s, b);
+            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT1, true, true, false>), grid, dim3(NT1), s, b);
         else if (pack)
             NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, true, false, false>), grid, dim3(NT), s, b);
         else if (square)
-            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, false, true, false>), grid, dim3(NT), s, b);
+            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT1, false, true, false>), grid, dim3(NT1), s, b);
         else
             NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, false, false, false>), grid, dim3(NT), s, b);
     }
     return NTT_CHECK_LAUNCH();
 }
 
-bool launch_fwd_strided(int logl, const PassArgs& a, int primes, bool pack, hipStream_t s) {
+bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, bool pack, hipStream_t s) {
     switch (logl) {
-        case 5: return launch_fwd_strided_t<5>(a, primes, pack, s);
-        case 6: return launch_fwd_strided_t<6>(a, primes, pack, s);
-        case 7: return launch_fwd_strided_t<7>(a, primes, pack, s);
-        case 8: return launch_fwd_strided_t<8>(a, primes, pack, s);
-        case 9: return launch_fwd_strided_t<9>(a, primes, pack, s);
-        case 10: return launch_fwd_strided_t<10>(a, primes, pack, s);
-        case 11: return launch_fwd_strided_t<11>(a, primes, pack, s);
+        case 5: return launch_fwd_strided_t<5>(a, primes, operands, pack, s);
+        case 6: return launch_fwd_strided_t<6>(a, primes, operands, pack, s);
+        case 7: return launch_fwd_strided_t<7>(a, primes, operands, pack, s);
+        case 8: return launch_fwd_strided_t<8>(a, primes, operands, pack, s);
+        case 9: return launch_fwd_strided_t<9>(a, primes, operands, pack, s);
+        case 10: return launch_fwd_strided_t<10>(a, primes, operands, pack, s);
+        case 11: return launch_fwd_strided_t<11>(a, primes, operands, pack, s);
         default: return false;
     }
 }
@@ -939,19 +936,6 @@ bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, hip
     }
 }
 
-bool launch_fwd_rows(int logl, const PassArgs& a, int primes, hipStream_t s) {
-    switch (logl) {
-        case 5: return launch_fwd_rows_t<5>(a, primes, s);
-        case 6: return launch_fwd_rows_t<6>(a, primes, s);
-        case 7: return launch_fwd_rows_t<7>(a, primes, s);
-        case 8: return launch_fwd_rows_t<8>(a, primes, s);
-        case 9: return launch_fwd_rows_t<9>(a, primes, s);
-        case 10: return launch_fwd_rows_t<10>(a, primes, s);
-        case 11: return launch_fwd_rows_t<11>(a, primes, s);
-        case 12: return launch_fwd_rows_t<12>(a, primes, s);
-        default: return false;
-    }
-}
 
 bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, hipStream_t s) {
     switch (logl) {
