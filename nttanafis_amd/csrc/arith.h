@@ -142,18 +142,27 @@ __host__ __device__ __forceinline__ double pack_as(const Source& s, uint64_t n, 
     return 0.0;
 }
 
-// True when coefficients [0, n_end) and every limb their windows read lie
-// inside the operand, so pack_as<KIND, false> may be used for them.
+// Number of leading coefficients whose window lies inside the operand and
+// which are below `digits`: pack_as<KIND, false> is exact for every n below
+// it.
+template <int KIND>
+__host__ __device__ __forceinline__ uint64_t pack_safe(const Source& s) {
+    if constexpr (KIND == kPackW64 || KIND == kPackW64Signed) {
+        return s.digits < s.limb_len ? s.digits : s.limb_len;
+    } else {
+        constexpr uint64_t W = pack_window<KIND - kPack1 + 1>();
+        if (s.limb_len < W) return 0;
+        // idx + W - 1 < limb_len with idx = floor(n * width / 64).
+        const uint64_t n = ((s.limb_len - W + 1) * 64 + s.width - 1) / s.width;
+        return n < s.digits ? n : s.digits;
+    }
+}
+
+// True when coefficients [0, n_end) may all be packed with
+// pack_as<KIND, false>.
 template <int KIND>
 __host__ __device__ __forceinline__ bool pack_interior(const Source& s, uint64_t n_end) {
-    if (n_end > s.digits) return false;
-    if constexpr (KIND == kPackW64 || KIND == kPackW64Signed) {
-        return n_end <= s.limb_len;
-    } else {
-        if (n_end == 0) return true;
-        const uint64_t last_idx = (((n_end - 1) * (uint64_t)s.width) >> 6) + pack_window<KIND - kPack1 + 1>() - 1;
-        return last_idx < s.limb_len;
-    }
+    return n_end <= pack_safe<KIND>(s);
 }
 
 __host__ __device__ __forceinline__ double pack_coef(const Source& s, uint64_t n, const double* pw, double p,
@@ -173,6 +182,47 @@ struct CrtWords {
     static constexpr int value = NP == 1 ? 1 : NP == 2 ? 2 : NP == 3 ? 3 : NP <= 5 ? 4 : NP == 6 ? 5 : NP == 7 ? 6 : 7;
 };
 
+// Mixed radix to binary for canonical digits y (X = y[0] + p0*(y[1] + p1*(...)))
+// in 32-bit words, so every product is one 32x32->64 multiply-add. After the
+// step for prime k, X < p_k * ... * p_{NP-1} < 2^(50*(NP-k)), which bounds the
+// words each step touches. p = plo + phi*2^32 with phi < 2^18: word i of
+// X*p + y takes x_i*plo from the first carry chain and x_{i-1}*phi from the
+// second; neither multiply-add can exceed 64 bits (the first chain's carry
+// stays below 2^32 + 2^18, the second's below 2^19).
+template <int NP>
+__host__ __device__ __forceinline__ void horner32(const CrtArgs& a, const double* y, uint64_t* X) {
+    constexpr int NW = CrtWords<NP>::value;
+    constexpr int N32 = 2 * NW;
+    uint32_t x[N32];
+    const uint64_t top = (uint64_t)y[NP - 1];
+    x[0] = (uint32_t)top;
+    x[1] = (uint32_t)(top >> 32);
+#pragma unroll
+    for (int i = 2; i < N32; ++i) x[i] = 0;
+#pragma unroll
+    for (int k = NP - 2; k >= 0; --k) {
+        const int words = (kPartBits + 1) * (NP - k) / 32 + 1 < N32 ? (kPartBits + 1) * (NP - k) / 32 + 1 : N32;
+        const uint64_t p = a.pint[k];
+        const uint32_t plo = (uint32_t)p, phi = (uint32_t)(p >> 32);
+        const uint64_t yk = (uint64_t)y[k];
+        uint64_t c1 = (uint32_t)yk, c2 = 0;
+        uint32_t prev = 0;
+#pragma unroll
+        for (int i = 0; i < N32; ++i) {
+            if (i >= words) break;
+            const uint32_t xo = x[i];
+            const uint64_t t1 = (uint64_t)xo * plo + c1;
+            c1 = (t1 >> 32) + (i == 0 ? (yk >> 32) : 0);
+            const uint64_t t2 = (uint64_t)prev * phi + ((t1 & 0xFFFFFFFFull) + c2);
+            x[i] = (uint32_t)t2;
+            c2 = t2 >> 32;
+            prev = xo;
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < NW; ++w) X[w] = (uint64_t)x[2 * w] | ((uint64_t)x[2 * w + 1] << 32);
+}
+
 template <int NP>
 __host__ __device__ __forceinline__ void garner_r(const CrtArgs& a, const double* r, uint64_t* X, bool balanced) {
     constexpr int NW = CrtWords<NP>::value;
@@ -186,8 +236,12 @@ __host__ __device__ __forceinline__ void garner_r(const CrtArgs& a, const double
         for (int j = k - 2; j >= 0; --j) u = reduce(y[j] + mulmod(u, a.pmod[j][k], pk, pinvk), pk, pinvk);
         y[k] = canonical(mulmod(r[k] - u, a.ginv[k], pk, pinvk), pk);
     }
+    if (!balanced) {
+        horner32<NP>(a, y, X);
+        return;
+    }
     int64_t top = (int64_t)y[NP - 1];
-    if (balanced && 2 * (uint64_t)top > a.pint[NP - 1]) top -= (int64_t)a.pint[NP - 1];
+    if (2 * (uint64_t)top > a.pint[NP - 1]) top -= (int64_t)a.pint[NP - 1];
     X[0] = (uint64_t)top;
 #pragma unroll
     for (int w = 1; w < NW; ++w) X[w] = top < 0 ? ~0ull : 0ull;

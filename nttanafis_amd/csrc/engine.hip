@@ -1142,6 +1142,41 @@ bool check_mod(double r, i128 expect, uint64_t p) {
     return d == 0;
 }
 
+// X = P - 1 (every residue p_k - 1), the largest value the recombination
+// can meet, against P computed by exact multiword products.
+template <int NP>
+int garner_extreme(const Catalog& c) {
+    constexpr int NW = CrtWords<NP>::value;
+    uint64_t P[kMaxWords + 2] = {1};
+    for (int k = 0; k < NP; ++k) {
+        unsigned __int128 carry = 0;
+        for (int w = 0; w < kMaxWords + 2; ++w) {
+            const unsigned __int128 t = (unsigned __int128)P[w] * c.p[k] + carry;
+            P[w] = (uint64_t)t;
+            carry = t >> 64;
+        }
+    }
+    for (int w = 0; w < kMaxWords + 2; ++w) {  // P - 1
+        if (P[w]--) break;
+    }
+    double residues[NP];
+    for (int k = 0; k < NP; ++k) residues[k] = (double)(c.p[k] - 1);
+    CrtArgs a;
+    memset(&a, 0, sizeof a);
+    a.residues = residues;
+    a.row_stride = 1;
+    a.field = c.field;
+    memcpy(a.pmod, c.pmod, sizeof a.pmod);
+    memcpy(a.ginv, c.ginv, sizeof a.ginv);
+    memcpy(a.pint, c.p, sizeof a.pint);
+    uint64_t got[NW];
+    garner<NP>(a, 0, got, false);
+    int fails = 0;
+    for (int w = 0; w < NW; ++w) fails += got[w] != P[w];
+    for (int w = NW; w < kMaxWords + 2; ++w) fails += P[w] != 0;  // P - 1 fits the words
+    return fails ? 1 : 0;
+}
+
 template <int NP>
 int garner_case(const Catalog& c, Rng& rng, bool balanced) {
     constexpr int NW = CrtWords<NP>::value;
@@ -1724,6 +1759,8 @@ int ntt_selftest_host(uint64_t seed) {
             }
         }
     }
+    fails += garner_extreme<1>(c) + garner_extreme<2>(c) + garner_extreme<3>(c) + garner_extreme<4>(c) +
+             garner_extreme<5>(c) + garner_extreme<6>(c) + garner_extreme<7>(c) + garner_extreme<8>(c);
     for (int t = 0; t < 200; ++t) {
         fails += garner_case<1>(c, rng, false) + garner_case<2>(c, rng, false) + garner_case<3>(c, rng, false) +
                  garner_case<4>(c, rng, false) + garner_case<5>(c, rng, false) + garner_case<6>(c, rng, false) +

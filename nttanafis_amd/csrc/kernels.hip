@@ -260,22 +260,40 @@ __device__ __forceinline__ void store_strided(const double* lds, double* g, uint
 
 // Strided tile packed from limbs: element m of line c is coefficient
 // n0 + m*S + c of the operand.
-template <int LOGL, int LINES, int NT, int KIND, bool CLAMP>
-__device__ __forceinline__ void pack_strided_as(double* lds, const Source& src, uint64_t n0, uint64_t S,
-                                                const double* pw, double p, double pinv) {
+// Packs the tile's coefficients n0 + m*S + c (m < L, c < LINES) into
+// lds[c*LS + pidx(m)]. Each thread takes its points in groups of BATCH whose
+// coefficient indices increase; a group entirely at or beyond `digits` is
+// zero padding and costs nothing, a group entirely below `safe` loads
+// without clamping, and the few groups in between take the clamped form.
+template <int LOGL, int LINES, int NT, int KIND>
+__device__ __forceinline__ void pack_tile_as(double* lds, const Source& src, uint64_t n0, uint64_t S,
+                                             const double* pw, double p, double pinv) {
     constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
     constexpr int TOTAL = LINES * L;
     constexpr int R = (TOTAL + NT - 1) / NT;
     constexpr int BATCH = (R >= 4 && R % 4 == 0) ? 4 : 1;
+    const uint64_t safe = pack_safe<KIND>(src);
+    const uint64_t digits = src.digits;
 #pragma unroll
     for (int r0 = 0; r0 < R; r0 += BATCH) {
         double v[BATCH];
+        uint64_t n[BATCH];
 #pragma unroll
         for (int r = 0; r < BATCH; ++r) {
             const int f = threadIdx.x + (r0 + r) * NT;
-            if (TOTAL % NT == 0 || f < TOTAL)
-                v[r] = pack_as<KIND, CLAMP>(src, n0 + (uint64_t)(f / LINES) * S + (f % LINES), pw, p, pinv);
+            n[r] = n0 + (uint64_t)(f / LINES) * S + (f % LINES);
+            v[r] = 0.0;
+        }
+        const bool valid_last = TOTAL % NT == 0 || (int)threadIdx.x + (r0 + BATCH - 1) * NT < TOTAL;
+        if (n[0] < digits) {
+            if (valid_last && n[BATCH - 1] < safe) {
+#pragma unroll
+                for (int r = 0; r < BATCH; ++r) v[r] = pack_as<KIND, false>(src, n[r], pw, p, pinv);
+            } else {
+#pragma unroll
+                for (int r = 0; r < BATCH; ++r) v[r] = pack_as<KIND, true>(src, n[r], pw, p, pinv);
+            }
         }
 #pragma unroll
         for (int r = 0; r < BATCH; ++r) {
@@ -283,20 +301,6 @@ __device__ __forceinline__ void pack_strided_as(double* lds, const Source& src, 
             if (TOTAL % NT == 0 || f < TOTAL) lds[(f % LINES) * LS + pidx(f / LINES)] = v[r];
         }
     }
-}
-
-// Packs a strided tile from limbs: one uniform dispatch on the source's form
-// and on whether the tile lies inside the operand (no clamped loads), then a
-// loop specialised for both.
-template <int LOGL, int LINES, int NT, int KIND>
-__device__ __forceinline__ void pack_tile_as(double* lds, const Source& src, uint64_t n0, uint64_t S,
-                                             const double* pw, double p, double pinv) {
-    constexpr int L = 1 << LOGL;
-    const uint64_t n_end = n0 + (uint64_t)(L - 1) * S + LINES;
-    if (pack_interior<KIND>(src, n_end))
-        pack_strided_as<LOGL, LINES, NT, KIND, false>(lds, src, n0, S, pw, p, pinv);
-    else
-        pack_strided_as<LOGL, LINES, NT, KIND, true>(lds, src, n0, S, pw, p, pinv);
 }
 
 template <int LOGL, int LINES, int NT>
