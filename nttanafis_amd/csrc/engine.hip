@@ -330,31 +330,64 @@ Executor* plan_executor() {
 // Pipelines.
 
 
-Split split_of(int K) {
+static std::mutex g_tune_lock;
+static int g_tuned_row_log[kMaxLog + 1] = {0};  // 0: not tuned yet
+
+static Split split_with_rows(int K, int rows) {
     Split s{};
     if (K <= kSinglePassMaxLog) {
         s.n = 1;
         s.logs[0] = K;
         return s;
     }
-    int rows = 12;
-    if (const char* e = getenv("NTTAMD_ROW_LOG")) rows = atoi(e);
     if (K <= 23) {
-        int r = std::min(rows, K - 5);
-        r = std::max(r, K - 11);
-        r = std::min(r, 12);
         s.n = 2;
-        s.logs[0] = K - r;
-        s.logs[1] = r;
+        s.logs[0] = K - rows;
+        s.logs[1] = rows;
         return s;
     }
-    const int r = 12;
-    const int k1 = (K - r + 1) / 2;
+    const int k1 = (K - rows + 1) / 2;
     s.n = 3;
     s.logs[0] = k1;
-    s.logs[1] = K - r - k1;
-    s.logs[2] = r;
+    s.logs[1] = K - rows - k1;
+    s.logs[2] = rows;
     return s;
+}
+
+// Row-pass sizes whose strided passes stay within 5..11 levels, largest
+// first (at most four).
+static std::vector<int> row_candidates(int K) {
+    std::vector<int> c;
+    if (K <= kSinglePassMaxLog) return c;
+    for (int r = 12; r >= 5 && c.size() < 4; --r) {
+        const int rest = K - r;
+        if (K <= 23) {
+            if (rest < 5 || rest > 11) continue;
+        } else {
+            const int k1 = (rest + 1) / 2;
+            if (k1 > 11 || rest - k1 < 5) continue;
+        }
+        c.push_back(r);
+    }
+    return c;
+}
+
+// Split of a transform of 2^K points: a single pass up to 2^12; otherwise a
+// row pass of 2^rows points after one or two strided passes, with rows from
+// NTTAMD_ROW_LOG when set and admissible, else from the autotuner once it
+// has timed this length, else 12 (as close to 12 as the strided passes
+// allow).
+Split split_of(int K) {
+    if (K <= kSinglePassMaxLog) return split_with_rows(K, K);
+    const std::vector<int> cand = row_candidates(K);
+    int rows = 0;
+    if (const char* e = getenv("NTTAMD_ROW_LOG")) rows = atoi(e);
+    if (!rows) {
+        std::lock_guard<std::mutex> g(g_tune_lock);
+        rows = g_tuned_row_log[K];
+    }
+    if (std::find(cand.begin(), cand.end(), rows) == cand.end()) rows = cand.empty() ? 12 : cand[0];
+    return split_with_rows(K, rows);
 }
 
 static PassArgs base_args(const Geometry& g, double* data, const double* fwd, const double* inv,
@@ -711,53 +744,9 @@ namespace nttamd {
 // times the candidate splits on the batch's own data the first time a
 // transform length is run.
 
-static std::mutex g_tune_lock;
-static int g_tuned_row_log[kMaxLog + 1] = {0};  // 0: not tuned yet
 
-static Split split_with_rows(int K, int rows) {
-    Split s{};
-    if (K <= kSinglePassMaxLog) {
-        s.n = 1;
-        s.logs[0] = K;
-        return s;
-    }
-    if (K <= 23) {
-        s.n = 2;
-        s.logs[0] = K - rows;
-        s.logs[1] = rows;
-        return s;
-    }
-    const int k1 = (K - rows + 1) / 2;
-    s.n = 3;
-    s.logs[0] = k1;
-    s.logs[1] = K - rows - k1;
-    s.logs[2] = rows;
-    return s;
-}
 
-// Row-pass sizes whose strided passes stay within 5..11 levels, largest
-// first (at most four).
-static std::vector<int> row_candidates(int K) {
-    std::vector<int> c;
-    if (K <= kSinglePassMaxLog) return c;
-    for (int r = 12; r >= 5 && c.size() < 4; --r) {
-        const int rest = K - r;
-        if (K <= 23) {
-            if (rest < 5 || rest > 11) continue;
-        } else {
-            const int k1 = (rest + 1) / 2;
-            if (k1 > 11 || rest - k1 < 5) continue;
-        }
-        c.push_back(r);
-    }
-    return c;
-}
 
-static Split tuned_split(int K) {
-    std::lock_guard<std::mutex> g(g_tune_lock);
-    const int rows = g_tuned_row_log[K];
-    return rows ? split_with_rows(K, rows) : split_of(K);
-}
 
 
 
@@ -839,7 +828,7 @@ static void build_program(Batch& B) {
     B.program.clear();
     const BatchItem* I = B.items.as<BatchItem>();
     for (const BatchBucket& r : B.buckets)
-        bucket_program(B.program, bucket_base(B, r.K), r.K, tuned_split(r.K), I, r);
+        bucket_program(B.program, bucket_base(B, r.K), r.K, split_of(r.K), I, r);
     for (int np = 1; np <= kMaxPrimes; ++np) {
         if (!B.crt_count[np]) continue;
         BatchLaunch L;

@@ -62,10 +62,11 @@ struct Tw {
     const double* table;
     uint64_t h0;
     int hstep;  // 1 when every line is its own node (rows), 0 when lines share one
+    int cmask;  // line c is node h0 + (c & cmask) * hstep (several tiles of rows)
     int split;  // first local level whose roots are formed as products
     double p, pinv;
     __device__ __forceinline__ double at(int c, int lvl, int node) const {
-        const uint64_t h = (h0 + (uint64_t)(c * hstep)) << lvl;
+        const uint64_t h = (h0 + (uint64_t)((c & cmask) * hstep)) << lvl;
         if (lvl >= split) return reduce(mulmod(table[node], table[h], p, pinv), p, pinv);
         return table[h + node];
     }
@@ -443,7 +444,7 @@ __global__ __launch_bounds__(NT) void k_fwd_strided(PassArgs a) {
         load_strided<LOGL, LINES, NT>(lds, row + base, S);
     }
     __syncthreads();
-    const Tw tw{a.fwd + (uint64_t)y * a.table_stride, H, 0, 64, p, pinv};
+    const Tw tw{a.fwd + (uint64_t)y * a.table_stride, H, 0, 0, 64, p, pinv};
     fwd_rounds<LOGL, LINES, NT, 0>(lds, tw, p, pinv);
     store_strided<LOGL, LINES, NT, false>(lds, row + base, S, 0.0, p, pinv);
 }
@@ -467,7 +468,7 @@ __global__ __launch_bounds__(NT) void k_inv_strided(PassArgs a) {
     double* row = j.data + (uint64_t)y * a.row_stride;
     load_strided<LOGL, LINES, NT>(lds, row + base, S);
     __syncthreads();
-    const Tw tw{a.inv + (uint64_t)y * a.table_stride, H, 0, 64, p, pinv};
+    const Tw tw{a.inv + (uint64_t)y * a.table_stride, H, 0, 0, 64, p, pinv};
     inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, tw, p, pinv);
     store_strided<LOGL, LINES, NT, CANON>(lds, row + base, S, a.tinv[y], p, pinv);
 }
@@ -479,57 +480,57 @@ __host__ __device__ constexpr int row_split(int logl) { return logl > 8 ? logl -
 // Row pass: row H of 2^LOGL contiguous points holds the last levels of the
 // transform.
 //
-// Fused row pass: forward levels of the row of each operand, pointwise
-// product (or a square), inverse levels, store. The rows of b are j.data and
-// those of a are j.other; the product replaces the rows of b. With PACK the
-// row is the whole transform (level_start == 0): both operands are packed
-// from limbs into LDS and the result is stored as canonical residues times
-// T^-1.
-template <int LOGL, int NT, bool PACK, bool SQUARE, bool BATCH>
+// Fused row pass over LINES consecutive rows H0.. of each operand: forward
+// levels of the rows of both operands, pointwise product (or a square),
+// inverse levels, store. The rows of b are j.data and those of a are
+// j.other; the product replaces the rows of b. Tile t, line c sits at
+// lds + (t*LINES + c)*LS and is node H0 + c. With PACK the row is the whole
+// transform (level_start == 0, LINES == 1): both operands are packed from
+// limbs into LDS and the result is stored as canonical residues times T^-1.
+template <int LOGL, int LINES, int NT, bool PACK, bool SQUARE, bool BATCH>
 __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
+    static_assert(!PACK || LINES == 1, "a packed transform is one row");
     constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
     constexpr int TILES = SQUARE ? 1 : 2;
-    constexpr int R = (L + NT - 1) / NT;
-    constexpr bool FULL = L % NT == 0;
-    __shared__ double lds[TILES * LS];
+    constexpr int PTS = LINES * L;
+    constexpr int R = (PTS + NT - 1) / NT;
+    constexpr bool FULL = PTS % NT == 0;
+    __shared__ double lds[TILES * LINES * LS];
     Job j;
     resolve<BATCH>(a, 0, false, &j);
     const int y = j.y;
     const double p = a.field.p[y], pinv = a.field.pinv[y];
-    const uint64_t H = j.tile;
-    double* row = j.data + (uint64_t)y * a.row_stride + H * L;
+    const uint64_t H0 = j.tile * LINES;
+    double* row = j.data + (uint64_t)y * a.row_stride + H0 * L;
     if constexpr (PACK) {
         pack_strided<LOGL, 1, NT>(lds, j.src_a, 0, 1, a.powers[y], p, pinv);
         if constexpr (!SQUARE) pack_strided<LOGL, 1, NT>(lds + LS, j.src_b, 0, 1, a.powers[y], p, pinv);
     } else {
-        load_rows<LOGL, 1, NT>(lds, row);
-        if constexpr (!SQUARE) load_rows<LOGL, 1, NT>(lds + LS, j.other + (uint64_t)y * a.row_stride + H * L);
+        load_rows<LOGL, LINES, NT>(lds, row);
+        if constexpr (!SQUARE)
+            load_rows<LOGL, LINES, NT>(lds + LINES * LS, j.other + (uint64_t)y * a.row_stride + H0 * L);
     }
     __syncthreads();
-    const Tw twF{a.fwd + (uint64_t)y * a.table_stride, H, 0, PACK ? 64 : row_split(LOGL), p, pinv};
-    const Tw twI{a.inv + (uint64_t)y * a.table_stride, H, 0, PACK ? 64 : row_split(LOGL), p, pinv};
-    // Both tiles share the row's roots (hstep 0), so one set of rounds
-    // transforms them together.
-    fwd_rounds<LOGL, TILES, NT, 0>(lds, twF, p, pinv);
-    if constexpr (SQUARE) {
+    const int split = PACK ? 64 : row_split(LOGL);
+    const Tw twF{a.fwd + (uint64_t)y * a.table_stride, H0, 1, LINES - 1, split, p, pinv};
+    const Tw twI{a.inv + (uint64_t)y * a.table_stride, H0, 1, LINES - 1, split, p, pinv};
+    // The lines of both tiles go through one set of rounds; line c of either
+    // tile is node H0 + (c mod LINES).
+    fwd_rounds<LOGL, TILES * LINES, NT, 0>(lds, twF, p, pinv);
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int m = threadIdx.x + r * NT;
-            if (!FULL && m >= L) continue;
-            const double v = lds[pidx(m)];
-            lds[pidx(m)] = mulmod(v, v, p, pinv);
-        }
-    } else {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int m = threadIdx.x + r * NT;
-            if (!FULL && m >= L) continue;
-            lds[pidx(m)] = mulmod(lds[pidx(m)], lds[LS + pidx(m)], p, pinv);
-        }
+    for (int r = 0; r < R; ++r) {
+        const int e = threadIdx.x + r * NT;
+        if (!FULL && e >= PTS) continue;
+        const int c = e >> LOGL, m = e & (L - 1);
+        double* x = lds + c * LS + pidx(m);
+        if constexpr (SQUARE)
+            *x = mulmod(*x, *x, p, pinv);
+        else
+            *x = mulmod(*x, x[LINES * LS], p, pinv);
     }
     __syncthreads();
-    inv_rounds<LOGL, 1, NT, Lay<LOGL>::ROUNDS - 1>(lds, twI, p, pinv);
+    inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, twI, p, pinv);
     if constexpr (PACK) {
         const double tinv = a.tinv[y];
 #pragma unroll
@@ -539,7 +540,7 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
             row[m] = canonical(mulmod(lds[pidx(m)], tinv, p, pinv), p);
         }
     } else {
-        store_rows<LOGL, 1, NT>(lds, row);
+        store_rows<LOGL, LINES, NT>(lds, row);
     }
 }
 
@@ -883,35 +884,39 @@ static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, hipSt
 }
 
 
-template <int LOGL>
-static bool launch_fused_t(const PassArgs& a, int units, bool pack, bool square, hipStream_t s) {
+// Shapes of the fused row pass: about 256 radix-16 groups per workgroup
+// round (enough rows per workgroup to fill it), one thread per group.
+template <int LOGL, bool PACK, bool SQUARE>
+struct FusedShape {
+    static constexpr int TILES = SQUARE ? 1 : 2;
+    static constexpr int GROUPS = TILES * (1 << LOGL) / 16;  // per row, per round
+    static constexpr int LINES = PACK ? 1 : (GROUPS >= 256 ? 1 : (256 / GROUPS > 32 ? 32 : 256 / GROUPS));
+    static constexpr int NT_RAW = GROUPS * LINES;
+    static constexpr int NT = NT_RAW < 64 ? 64 : (NT_RAW > 512 ? 512 : NT_RAW);
+};
+
+template <int LOGL, bool PACK, bool SQUARE>
+static bool launch_fused_v(const PassArgs& a, int units, hipStream_t s) {
+    using F = FusedShape<LOGL, PACK, SQUARE>;
     const uint64_t rows = (uint64_t)1 << a.level_start;
+    if (rows % F::LINES != 0) return false;
+    const uint64_t tiles = rows / F::LINES;
     PassArgs b = a;
-    b.tiles_per_item = rows;
-    dim3 grid((unsigned)rows, (unsigned)units);
-    if (a.batch) grid = dim3((unsigned)(rows * units), 1);
-    constexpr int NT1 = LOGL <= 8 ? 64 : 256;  // one tile (square)
-    constexpr int NT = 2 * NT1;                  // two tiles
+    b.tiles_per_item = tiles;
+    dim3 grid((unsigned)tiles, (unsigned)units);
     if (a.batch) {
-        if (pack && square)
-            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT1, true, true, true>), grid, dim3(NT1), s, b);
-        else if (pack)
-            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, true, false, true>), grid, dim3(NT), s, b);
-        else if (square)
-            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT1, false, true, true>), grid, dim3(NT1), s, b);
-        else
-            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, false, false, true>), grid, dim3(NT), s, b);
+        grid = dim3((unsigned)(tiles * units), 1);
+        NTT_LAUNCH(nullptr, (k_fused<LOGL, F::LINES, F::NT, PACK, SQUARE, true>), grid, dim3(F::NT), s, b);
     } else {
-        if (pack && square)
-            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT1, true, true, false>), grid, dim3(NT1), s, b);
-        else if (pack)
-            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, true, false, false>), grid, dim3(NT), s, b);
-        else if (square)
-            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT1, false, true, false>), grid, dim3(NT1), s, b);
-        else
-            NTT_LAUNCH(nullptr, (k_fused<LOGL, NT, false, false, false>), grid, dim3(NT), s, b);
+        NTT_LAUNCH(nullptr, (k_fused<LOGL, F::LINES, F::NT, PACK, SQUARE, false>), grid, dim3(F::NT), s, b);
     }
     return NTT_CHECK_LAUNCH();
+}
+
+template <int LOGL>
+static bool launch_fused_t(const PassArgs& a, int units, bool pack, bool square, hipStream_t s) {
+    if (pack) return square ? launch_fused_v<LOGL, true, true>(a, units, s) : launch_fused_v<LOGL, true, false>(a, units, s);
+    return square ? launch_fused_v<LOGL, false, true>(a, units, s) : launch_fused_v<LOGL, false, false>(a, units, s);
 }
 
 bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, bool pack, hipStream_t s) {
