@@ -189,12 +189,19 @@ struct CrtWords {
 // X*p + y takes x_i*plo from the first carry chain and x_{i-1}*phi from the
 // second; neither multiply-add can exceed 64 bits (the first chain's carry
 // stays below 2^32 + 2^18, the second's below 2^19).
+// The integer value of an integer-valued double y in [0, 2^52): y + 2^52 is
+// exact and its low 52 mantissa bits are y (two instructions instead of a
+// general conversion).
+__host__ __device__ __forceinline__ uint64_t exact_u64(double y) {
+    return __builtin_bit_cast(uint64_t, y + 4503599627370496.0) - 0x4330000000000000ull;
+}
+
 template <int NP>
 __host__ __device__ __forceinline__ void horner32(const CrtArgs& a, const double* y, uint64_t* X) {
     constexpr int NW = CrtWords<NP>::value;
     constexpr int N32 = 2 * NW;
     uint32_t x[N32];
-    const uint64_t top = (uint64_t)y[NP - 1];
+    const uint64_t top = exact_u64(y[NP - 1]);
     x[0] = (uint32_t)top;
     x[1] = (uint32_t)(top >> 32);
 #pragma unroll
@@ -204,7 +211,7 @@ __host__ __device__ __forceinline__ void horner32(const CrtArgs& a, const double
         const int words = (kPartBits + 1) * (NP - k) / 32 + 1 < N32 ? (kPartBits + 1) * (NP - k) / 32 + 1 : N32;
         const uint64_t p = a.pint[k];
         const uint32_t plo = (uint32_t)p, phi = (uint32_t)(p >> 32);
-        const uint64_t yk = (uint64_t)y[k];
+        const uint64_t yk = exact_u64(y[k]);
         uint64_t c1 = (uint32_t)yk, c2 = 0;
         uint32_t prev = 0;
 #pragma unroll

@@ -545,12 +545,14 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
 }
 
 // Product reconstruction, one block of kCrtBlockLimbs output limbs: every
-// coefficient X_i touching the block is rebuilt once into LDS, every limb
-// sums its windows of the shifted coefficients, and carries inside the block
-// resolve by a scan of carry functions (per wave with lane shuffles, then
-// across the block's waves). Blocks publish their carry function for
-// k_carry_apply (the block merge of src/limb/
-// reconstruction.rs:211-247).
+// coefficient X_i touching the block is rebuilt once, shifted to its bit
+// offset and added word by word into LDS limb accumulators with atomic adds
+// that return the previous value, so each limb's sum (mod 2^64) and its
+// number of wrap-arounds (the carries it sends on) come out exact in any
+// order. Carries inside the block then resolve by a scan of carry functions
+// (per wave with lane shuffles, then across the block's waves). Blocks
+// publish their carry function for k_carry_apply (the block merge of
+// src/limb/reconstruction.rs:211-247).
 constexpr int kCrtLdsWords = 4096;
 
 __device__ __forceinline__ CarryFn shfl_up_fn(CarryFn f, int d) {
@@ -613,7 +615,8 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
     constexpr int NW = CrtWords<NP>::value;
     constexpr int RMAX = (kCrtLdsWords / NW + kCrtThreads - 1) / kCrtThreads;
     constexpr int WAVES = kCrtThreads / 64;
-    __shared__ uint64_t xs[kCrtLdsWords];
+    __shared__ unsigned long long acc[kCrtBlockLimbs];
+    __shared__ uint32_t wraps[kCrtBlockLimbs];
     __shared__ CarryFn wave_total[WAVES];
     __shared__ uint32_t ripple;
     const CrtJob J = crt_job(a);
@@ -635,55 +638,49 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
             r[q][k] = idx < count ? J.residues[(uint64_t)k * J.row_stride + i_lo + idx] : 0.0;
     }
 #pragma unroll
+    for (int q = 0; q < kCrtLimbsPerThread; ++q) {
+        acc[threadIdx.x + q * kCrtThreads] = 0;
+        wraps[threadIdx.x + q * kCrtThreads] = 0;
+    }
+    if (threadIdx.x == 0) ripple = 0;
+    __syncthreads();
+    // Coefficient i starts at bit i*w - 64*L0 of the block, which lies in
+    // [-xb, 64*kCrtBlockLimbs); offsets from bit i_lo*w keep the arithmetic
+    // in 32 bits. Word m of X_i << s lands in limb qrel + m.
+    const uint32_t wr = (uint32_t)w;
+    const uint32_t rel0 = (uint32_t)(lo_bit - i_lo * w);  // bit of limb L0 from coefficient i_lo
+#pragma unroll
     for (int q = 0; q < RMAX; ++q) {
         const int idx = threadIdx.x + q * kCrtThreads;
         if (idx < count) {
             uint64_t X[NW];
             garner_r<NP>(a, r[q], X, false);
+            const int32_t bit = (int32_t)((uint32_t)idx * wr) - (int32_t)rel0;
+            const int32_t qrel = bit >> 6;  // arithmetic shift: floor division
+            const uint32_t sh = (uint32_t)bit & 63u;
+            uint64_t prev = 0;
 #pragma unroll
-            for (int m = 0; m < NW; ++m) xs[idx * NW + m] = X[m];
-        }
-    }
-    if (threadIdx.x == 0) ripple = 0;
-    __syncthreads();
-
-    // Limb j sums the windows of coefficients i with i*w < 64(j+1) and
-    // i*w + xb > 64j. Offsets are taken from bit i_lo*w of the block, which
-    // keeps them below 2^24, so divisions by w are multiplications.
-    uint64_t v[kCrtLimbsPerThread];
-    CarryFn f[kCrtLimbsPerThread];
-    const uint32_t wr = (uint32_t)w;
-    const uint32_t rel0 = (uint32_t)(lo_bit - i_lo * w);  // bit of limb L0 from coefficient i_lo
-    const uint32_t xb32 = (uint32_t)xb;
-    const uint32_t magic = J.magic;
-#pragma unroll
-    for (int q = 0; q < kCrtLimbsPerThread; ++q) {
-        const uint32_t jr = rel0 + 64u * (threadIdx.x * kCrtLimbsPerThread + q);  // bit of limb j from i_lo
-        const uint64_t j = L0 + (uint64_t)threadIdx.x * kCrtLimbsPerThread + q;
-        uint64_t sum = 0;
-        uint32_t carry = 0;
-        if (j < J.out_len && count > 0) {
-            const uint32_t ia = jr + 1 > xb32 ? __umulhi(jr + 1 - xb32 + wr - 1, magic) : 0u;
-            uint32_t ib = __umulhi(jr + 63, magic);
-            if (ib >= (uint32_t)count) ib = (uint32_t)count - 1;
-            for (uint32_t ii = ia; ii <= ib; ++ii) {
-                const uint64_t* X = xs + ii * NW;
-                const int32_t o = (int32_t)jr - (int32_t)(ii * wr);
-                uint64_t piece;
-                if (o >= 0) {
-                    const int qw = o >> 6, sh = o & 63;
-                    const uint64_t lo = qw < NW ? X[qw] : 0;
-                    const uint64_t hi = qw + 1 < NW ? X[qw + 1] : 0;
-                    piece = sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
-                } else {
-                    piece = X[0] << (-o);
+            for (int m = 0; m <= NW; ++m) {
+                const uint64_t cur = m < NW ? X[m] : 0;
+                const uint64_t word = sh ? (cur << sh) | (prev >> (64 - sh)) : cur;
+                prev = cur;
+                const int32_t j = qrel + m;
+                if (word != 0 && j >= 0 && j < kCrtBlockLimbs) {
+                    const unsigned long long old = atomicAdd(&acc[j], (unsigned long long)word);
+                    if (old + word < old) atomicAdd(&wraps[j], 1u);
                 }
-                sum += piece;
-                carry += sum < piece;
             }
         }
+    }
+    __syncthreads();
+    uint64_t v[kCrtLimbsPerThread];
+    CarryFn f[kCrtLimbsPerThread];
+#pragma unroll
+    for (int q = 0; q < kCrtLimbsPerThread; ++q) {
+        const int jl = threadIdx.x * kCrtLimbsPerThread + q;
+        const uint64_t sum = acc[jl];
         v[q] = sum;
-        f[q] = CarryFn{carry, limb_thresh(sum)};
+        f[q] = CarryFn{wraps[jl], limb_thresh(sum)};
     }
     // Inclusive scan of the threads' carry functions within each wave.
     CarryFn agg = f[0];
@@ -721,14 +718,14 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
         have = true;
     }
     if (my_ripple) atomicMax(&ripple, my_ripple);
-    __syncthreads();  // every xs read is done; reuse it to coalesce the stores
+    __syncthreads();  // every accumulator read is done; reuse them to coalesce the stores
 #pragma unroll
-    for (int q = 0; q < kCrtLimbsPerThread; ++q) xs[threadIdx.x * kCrtLimbsPerThread + q] = outv[q];
+    for (int q = 0; q < kCrtLimbsPerThread; ++q) acc[threadIdx.x * kCrtLimbsPerThread + q] = outv[q];
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kCrtLimbsPerThread; ++q) {
         const uint64_t j = L0 + threadIdx.x + q * kCrtThreads;
-        if (j < J.out_len) J.out[j] = xs[threadIdx.x + q * kCrtThreads];
+        if (j < J.out_len) J.out[j] = acc[threadIdx.x + q * kCrtThreads];
     }
     if (threadIdx.x == kCrtThreads - 1) {
         J.agg_base[J.block] = g.base;
