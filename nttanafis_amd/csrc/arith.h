@@ -75,13 +75,12 @@ __host__ __device__ constexpr int pack_window() {
     return (63 + PARTS * kPartBits + 63) / 64 < 4 ? (63 + PARTS * kPartBits + 63) / 64 : 4;
 }
 
-// Residue of the coefficient that starts at bit `off` of w[0], from a window
-// w[0..W] of limbs (w[W] = 0); the top part has high_bits bits.
+// Parts of the coefficient that starts at bit `off` of w[0], from a window
+// w[0..W] of limbs (w[W] = 0), as doubles; the top part has high_bits bits.
 template <int PARTS>
-__host__ __device__ __forceinline__ double residue_from_window(const uint64_t* w, uint32_t off, int high_bits,
-                                                               const double* pw, double p, double pinv) {
+__host__ __device__ __forceinline__ void parts_from_window(const uint64_t* w, uint32_t off, int high_bits,
+                                                           double* d) {
     constexpr int W = pack_window<PARTS>();
-    double sum = 0.0;
 #pragma unroll
     for (int k = 0; k < PARTS; ++k) {
         const uint32_t bit = off + (uint32_t)(kPartBits * k);
@@ -92,10 +91,51 @@ __host__ __device__ __forceinline__ double residue_from_window(const uint64_t* w
         const uint64_t hi = next ? w[q0 + 2 < W ? q0 + 2 : W] : w[q0 + 1 < W ? q0 + 1 : W];
         const uint64_t word = (lo >> sh) | ((hi << 1) << (63 - sh));
         const uint64_t mask = k + 1 == PARTS ? ((1ull << high_bits) - 1) : ((1ull << kPartBits) - 1);
-        const double d = part_to_double(word & mask);
-        sum = k == 0 ? d : sum + mulmod(d, pw[k], p, pinv);
+        d[k] = part_to_double(word & mask);
     }
+}
+
+// Residue of a coefficient from its parts: part_0 + sum_k part_k (2^(49k) mod p),
+// reduced once.
+template <int PARTS>
+__host__ __device__ __forceinline__ double combine_parts(const double* d, const double* pw, double p,
+                                                         double pinv) {
+    double sum = d[0];
+#pragma unroll
+    for (int k = 1; k < PARTS; ++k) sum = sum + mulmod(d[k], pw[k], p, pinv);
     return PARTS > 1 ? reduce(sum, p, pinv) : sum;
+}
+
+template <int PARTS>
+__host__ __device__ __forceinline__ double residue_from_window(const uint64_t* w, uint32_t off, int high_bits,
+                                                               const double* pw, double p, double pinv) {
+    double d[PARTS];
+    parts_from_window<PARTS>(w, off, high_bits, d);
+    return combine_parts<PARTS>(d, pw, p, pinv);
+}
+
+// Parts of coefficient n of a general-width source (zero at or beyond
+// digits); CLAMP as for pack_parts.
+template <int PARTS, bool CLAMP = true>
+__host__ __device__ __forceinline__ void extract_parts(const Source& s, uint64_t n, double* d) {
+    constexpr int W = pack_window<PARTS>();
+    const uint64_t bitpos = n * (uint64_t)s.width;
+    const uint64_t idx = bitpos >> 6;
+    const uint32_t off = (uint32_t)bitpos & 63;
+    const uint64_t last = s.limb_len - 1;
+    uint64_t w[W + 1];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        const uint64_t i = idx + q;
+        if (CLAMP) {
+            const uint64_t raw = s.limbs[i < last ? i : last];
+            w[q] = i < s.limb_len && n < s.digits ? raw : 0;
+        } else {
+            w[q] = s.limbs[i];
+        }
+    }
+    w[W] = 0;
+    parts_from_window<PARTS>(w, off, (int)s.value_bits - kPartBits * (PARTS - 1), d);
 }
 
 template <int PARTS, bool CLAMP = true>

@@ -97,7 +97,7 @@ struct PassArgs {
     uint64_t table_stride;   // entries per prime in the root tables
     int32_t log_t;           // K = log2(T)
     int32_t level_start;     // global level of this pass's first level
-    int32_t primes;          // primes of the operation (packing passes)
+    int32_t primes;          // work units (primes) of a single-operation strided pass
     int32_t pad0;
     uint64_t line_stride;    // element stride of a strided pass, 1 for rows
     Source src_a;            // packing sources (first forward pass)

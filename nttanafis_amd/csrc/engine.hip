@@ -1719,6 +1719,25 @@ int ntt_selftest_host(uint64_t seed) {
                     }
                     if (fast != got) ++fails;
                     if (admitted) ++interior;
+                    // Parts extracted once and combined per prime (stacked
+                    // packing tiles) give the same residue.
+                    if (kind >= kPack1) {
+                        double d[kMaxParts];
+                        double split = 0.0;
+                        switch (kind) {
+#define NTT_PARTS_CASE(P)                                                  \
+    case kPack1 + P - 1:                                                   \
+        extract_parts<P>(s, n, d);                                         \
+        split = combine_parts<P>(d, c.powers[y], pf, pinv);                \
+        break;
+                            NTT_PARTS_CASE(1)
+                            NTT_PARTS_CASE(2)
+                            NTT_PARTS_CASE(3)
+                            NTT_PARTS_CASE(4)
+#undef NTT_PARTS_CASE
+                        }
+                        if (split != got) ++fails;
+                    }
                 }
             }
         }

@@ -364,6 +364,13 @@ __device__ __forceinline__ void store_rows(const double* lds, double* g) {
 // workgroup from the item table, so one launch covers every product of a
 // size bucket.
 
+// Consecutive tiles kept on one XCD: enough to cover a 128-byte line.
+__host__ __device__ constexpr int xcd_group(int lines) { return lines >= 16 ? 1 : 16 / lines; }
+// Strided passes deal tiles to XCDs when neighbouring tiles share lines or
+// the units of a tile read the same limbs (packing); otherwise the plain
+// (tile, unit) grid is used.
+__host__ __device__ constexpr bool strided_xcd(bool pack, int lines) { return pack || xcd_group(lines) > 1; }
+
 struct Job {
     double* data;          // rows of the operand this pass transforms
     const double* other;   // fused passes: the rows of operand a
@@ -373,20 +380,24 @@ struct Job {
     uint64_t tile;
 };
 
-// Workgroup b of a single packing pass runs on XCD b % 8 (round-robin
-// dispatch). Tiles are dealt to XCDs so that the workgroups of all primes for
-// one tile are consecutive on one XCD: they read the same limbs, which the
-// first one brings into that XCD's L2.
-__device__ __forceinline__ bool xcd_tile(int primes, uint64_t tiles, uint64_t* tile, int* y) {
+// Workgroup b of a single-operation strided pass runs on XCD b % 8
+// (round-robin dispatch). Tiles are dealt to XCDs in groups of G consecutive
+// tiles, and all units (primes) of a tile follow each other on one XCD: tiles
+// of a group share cache lines (G*LINES*8 >= 128 bytes), and the primes of a
+// packing tile read the same limbs, which the first one brings into that
+// XCD's L2.
+template <int G>
+__device__ __forceinline__ bool xcd_tile(int units, uint64_t tiles, uint64_t* tile, int* y) {
     const uint64_t b = blockIdx.x;
     const uint64_t slot = b >> 3;
-    const uint64_t t = (slot / (uint64_t)primes) * 8 + (b & 7);
-    *y = (int)(slot % (uint64_t)primes);
+    const uint64_t kk = slot / (uint64_t)units;
+    *y = (int)(slot % (uint64_t)units);
+    const uint64_t t = (kk / G) * (8 * G) + (b & 7) * G + (kk % G);
     *tile = t;
     return t < tiles;
 }
 
-template <bool BATCH>
+template <bool BATCH, int G = 1>
 __device__ __forceinline__ bool resolve(const PassArgs& a, uint64_t tiles, bool xcd, Job* j) {
     if constexpr (!BATCH) {
         // Forward passes over both operands in one launch: z = 1 is operand b,
@@ -396,7 +407,7 @@ __device__ __forceinline__ bool resolve(const PassArgs& a, uint64_t tiles, bool 
         j->other = a.other;
         j->src_a = opb ? a.src_b : a.src_a;
         j->src_b = a.src_b;
-        if (xcd) return xcd_tile(a.primes, tiles, &j->tile, &j->y);
+        if (xcd) return xcd_tile<G>(a.primes, tiles, &j->tile, &j->y);
         j->tile = blockIdx.x;
         j->y = blockIdx.y;
         return true;
@@ -430,7 +441,8 @@ __global__ __launch_bounds__(NT) void k_fwd_strided(PassArgs a) {
     const uint64_t S = a.line_stride;
     const uint64_t tiles_per_node = S / LINES;
     Job j;
-    if (!resolve<BATCH>(a, tiles_per_node << a.level_start, PACK, &j)) return;
+    if (!resolve<BATCH, xcd_group(LINES)>(a, tiles_per_node << a.level_start, strided_xcd(PACK, LINES), &j))
+        return;
     const int y = j.y;
     const double p = a.field.p[y], pinv = a.field.pinv[y];
     const uint64_t H = j.tile / tiles_per_node;
@@ -458,7 +470,8 @@ __global__ __launch_bounds__(NT) void k_inv_strided(PassArgs a) {
     const uint64_t S = a.line_stride;
     const uint64_t tiles_per_node = S / LINES;
     Job j;
-    resolve<BATCH>(a, 0, false, &j);
+    if (!resolve<BATCH, xcd_group(LINES)>(a, tiles_per_node << a.level_start, strided_xcd(false, LINES), &j))
+        return;
     const int y = j.y;
     const double p = a.field.p[y], pinv = a.field.pinv[y];
     const uint64_t H = j.tile / tiles_per_node;
@@ -817,7 +830,7 @@ __global__ __launch_bounds__(256) void k_crt_disjoint(CrtArgs a) {
 // point of the tile moves in runs of at least 256 bytes.
 template <int LOGL>
 static constexpr int strided_lines() {
-    return LOGL <= 6 ? 64 : LOGL == 7 ? 32 : LOGL <= 10 ? 16 : (LOGL == 11 ? 8 : 4);
+    return LOGL <= 6 ? 64 : LOGL == 7 ? 32 : LOGL <= 9 ? 16 : (LOGL == 10 ? 8 : 4);
 }
 
 // Threads of a strided tile: one radix-16 group per thread and round.
@@ -838,9 +851,10 @@ static bool launch_fwd_strided_t(const PassArgs& a, int units, int operands, boo
     PassArgs b = a;
     b.primes = units;
     b.tiles_per_item = tiles;
+    constexpr uint64_t G8 = 8 * (uint64_t)xcd_group(LINES);
     dim3 grid((unsigned)tiles, (unsigned)units, (unsigned)operands);
+    if (strided_xcd(pack, LINES)) grid = dim3((unsigned)(((tiles + G8 - 1) / G8) * G8 * units), 1, (unsigned)operands);
     if (a.batch) grid = dim3((unsigned)(tiles * units), 1);
-    else if (pack) grid = dim3((unsigned)(((tiles + 7) / 8) * 8 * units), 1, (unsigned)operands);
     if (a.batch) {
         if (pack)
             NTT_LAUNCH(nullptr, (k_fwd_strided<LOGL, LINES, NT, true, true>), grid, dim3(NT), s, b);
@@ -863,8 +877,11 @@ static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, hipSt
     const uint64_t nodes = (uint64_t)1 << a.level_start;
     const uint64_t tiles = nodes * (a.line_stride / LINES);
     PassArgs b = a;
+    b.primes = units;
     b.tiles_per_item = tiles;
+    constexpr uint64_t G8 = 8 * (uint64_t)xcd_group(LINES);
     dim3 grid((unsigned)tiles, (unsigned)units);
+    if (strided_xcd(false, LINES)) grid = dim3((unsigned)(((tiles + G8 - 1) / G8) * G8 * units), 1);
     if (a.batch) grid = dim3((unsigned)(tiles * units), 1);
     if (a.batch) {
         if (canon)
