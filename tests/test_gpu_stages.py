@@ -1,0 +1,111 @@
+"""Stage-level parity: the transform pipeline and the reconstruction, each
+checked on its own against exact integers.
+
+- ntt_debug_residues runs packing, forward passes, pointwise product and
+  inverse passes and returns the canonical residue rows; row y must be the
+  linear convolution of the operands' coefficients (width-bit digits, as
+  src/limb/packing.rs:113-185 cuts them) reduced mod prime y, zero past the
+  convolution length.
+- ntt_debug_crt runs the reconstruction and carry kernels on given residue
+  rows; fed the exact rows it must produce the product limbs
+  (src/limb/reconstruction.rs:77-165).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import to_int
+
+import nttanafis_amd as ntt
+from nttanafis_amd import _ffi
+from nttanafis_amd.workloads import operand
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    ntt.load()
+    assert ntt.device_available(), "no gfx950 device visible"
+    yield
+
+
+def digits(limbs, width, count):
+    x = to_int(limbs)
+    mask = (1 << width) - 1
+    return [(x >> (n * width)) & mask for n in range(count)]
+
+
+def conv_mod(da, db, p):
+    """Linear convolution of two digit lists mod p, exact: residues are cut
+    into 25-bit halves so every int64 convolution sum stays below 2^63."""
+    ra = np.array([d % p for d in da], dtype=object)
+    rb = np.array([d % p for d in db], dtype=object)
+    mask = (1 << 25) - 1
+    ah = np.array([int(v) >> 25 for v in ra], dtype=np.int64)
+    al = np.array([int(v) & mask for v in ra], dtype=np.int64)
+    bh = np.array([int(v) >> 25 for v in rb], dtype=np.int64)
+    bl = np.array([int(v) & mask for v in rb], dtype=np.int64)
+    hh, hl, lh, ll = (np.convolve(x, y) for x, y in ((ah, bh), (ah, bl), (al, bh), (al, bl)))
+    return [((int(hh[k]) << 50) + ((int(hl[k]) + int(lh[k])) << 25) + int(ll[k])) % p for k in range(len(hh))]
+
+
+def geometry(la, lb):
+    geo = (ctypes.c_uint32 * 3)()
+    lib = ntt.load()
+    assert lib.ntt_debug_residues(None, la, None, lb, 0, None, 0, geo)
+    return int(geo[0]), int(geo[1]), int(geo[2])
+
+
+def residue_rows(a, b):
+    np_, width, K = geometry(len(a), len(b))
+    T = 1 << K
+    out = np.zeros(np_ * T, dtype=np.float64)
+    geo = (ctypes.c_uint32 * 3)()
+    lib = ntt.load()
+    ok = lib.ntt_debug_residues(a.ctypes.data_as(_ffi.P64), len(a), b.ctypes.data_as(_ffi.P64), len(b),
+                                1 if a is b else 0, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                out.size, geo)
+    assert ok, ntt.last_error()
+    return np_, width, K, out.reshape(np_, T)
+
+
+def expected_rows(oracle, a, b, np_, width, K):
+    T = 1 << K
+    na = (len(a) * 64 + width - 1) // width
+    nb = (len(b) * 64 + width - 1) // width
+    da, db = digits(a, width, na), digits(b, width, nb)
+    rows = np.zeros((np_, T), dtype=np.float64)
+    for y in range(np_):
+        c = conv_mod(da, db, oracle.prime(y))
+        rows[y, :len(c)] = np.array(c, dtype=np.float64)
+    return rows
+
+
+# Single-pass transforms, two-pass transforms, squares, a ragged operand.
+SHAPES = [(40, 30, False), (300, 300, False), (1000, 777, False), (2000, 2000, True), (4000, 3000, False)]
+
+
+@pytest.mark.parametrize("la,lb,square", SHAPES)
+def test_residue_rows_are_the_convolution(oracle, la, lb, square):
+    a = operand(la, 5)
+    b = a if square else operand(lb, 6)
+    np_, width, K, rows = residue_rows(a, b)
+    want = expected_rows(oracle, a, b, np_, width, K)
+    for y in range(np_):
+        bad = np.nonzero(rows[y] != want[y])[0]
+        assert bad.size == 0, f"prime {y}: {bad.size} residues differ (first at {bad[0]})"
+
+
+@pytest.mark.parametrize("la,lb", [(40, 30), (1000, 777), (4000, 3000)])
+def test_reconstruction_of_exact_rows(oracle, la, lb):
+    a, b = operand(la, 7), operand(lb, 8)
+    np_, width, K = geometry(la, lb)
+    rows = expected_rows(oracle, a, b, np_, width, K)
+    out = np.zeros(la + lb, dtype=np.uint64)
+    lib = ntt.load()
+    ok = lib.ntt_debug_crt(la, lb, np.ascontiguousarray(rows).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                           out.ctypes.data_as(_ffi.P64))
+    assert ok, ntt.last_error()
+    assert (out == oracle.mul(a, b)).all()
