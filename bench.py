@@ -233,6 +233,25 @@ def bench_batch(steps: int, warmup: int, plain_steps: int) -> dict:
     }
 
 
+def bench_host_path(a, b, steps: int) -> dict:
+    """The same product through the host-buffer API (MulPlan::mul with host
+    slices): operands cross PCIe in, the product back out. Pageable numpy
+    arrays and pinned (hipHostMalloc) arrays."""
+    plan = ntt.MulPlan.new(len(a), len(b))
+    res = {}
+    for name, alloc in (("pageable", lambda n: np.empty(n, dtype=np.uint64)), ("pinned", ntt.pinned_array)):
+        ha, hb, dst = alloc(len(a)), alloc(len(b)), alloc(len(a) + len(b))
+        ha[:] = a
+        hb[:] = b
+        assert plan.mul(dst, ha, hb), ntt.last_error()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            assert plan.mul(dst, ha, hb)
+        per = (time.perf_counter() - t0) / steps
+        res[name] = {"ms_per_call": round(1000 * per, 3), "Mlimbs_per_s": round(len(a) / per / 1e6, 1)}
+    return res
+
+
 def _pct(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if xs else None
 
@@ -357,6 +376,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-buffer API timing")
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=240)
@@ -508,6 +528,8 @@ def main() -> int:
         "kernels": per_kernel,
         "roofline": roofline,
     }
+    if kind == "mul" and not args.no_host:
+        result["host_buffers"] = bench_host_path(a, b, max(3, args.steps // 5))
     if not args.no_batch:
         result["batched"] = bench_batch(args.batch_steps, 2, 2)
     if not args.no_serve:

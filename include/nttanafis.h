@@ -275,8 +275,11 @@ void ntt_server_free(ntt_server *server);
 
 /* Queues a request and returns its ticket, or -1 when the request is
  * malformed (lengths, bounds, dst_len) or can never fit the budget. Buffers
- * must stay valid and unchanged until the request completes; dst is written
- * only by a request that completes successfully. Thread-safe. */
+ * must stay valid and unchanged until the request completes. A host dst is
+ * written only by a request that completes successfully; a convolution whose
+ * host coefficients exceed their bounds fails and leaves dst unchanged. With
+ * NTT_REQ_DEVICE the kernels write dst directly and coefficient bounds are
+ * not checked (as for the *_device calls). Thread-safe. */
 int64_t ntt_server_submit(ntt_server *server, const ntt_request *request);
 
 /* 0 while queued or running, 1 when done, -1 when failed or unknown. */
