@@ -9,6 +9,7 @@
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -377,11 +378,14 @@ static std::vector<int> row_candidates(int K) {
 // NTTAMD_ROW_LOG when set and admissible, else from the autotuner once it
 // has timed this length, else 12 (as close to 12 as the strided passes
 // allow).
+static thread_local int t_force_rows = 0;  // set while the autotuner times a candidate
+
 Split split_of(int K) {
     if (K <= kSinglePassMaxLog) return split_with_rows(K, K);
     const std::vector<int> cand = row_candidates(K);
-    int rows = 0;
-    if (const char* e = getenv("NTTAMD_ROW_LOG")) rows = atoi(e);
+    int rows = t_force_rows;
+    if (!rows)
+        if (const char* e = getenv("NTTAMD_ROW_LOG")) rows = atoi(e);
     if (!rows) {
         std::lock_guard<std::mutex> g(g_tune_lock);
         rows = g_tuned_row_log[K];
@@ -589,6 +593,47 @@ static bool device_product(Executor& ex, const Geometry& g, const uint64_t* d_a,
     OpBuffers buf;
     if (!exec_buffers(ex, g, square, la + lb, &buf)) return false;
     return product_on(ex.stream, g, d_a, la, d_b, lb, square, buf, d_out);
+}
+
+// Plan-time autotuning of a single operation's split: the first prepare of a
+// transform length times every admissible row size on the executor's own
+// buffers (operands of the plan's shape, contents irrelevant to the timing)
+// and records the fastest for every later call of that length.
+static void tune_split(Executor& ex, int K, const std::function<bool()>& run) {
+    {
+        std::lock_guard<std::mutex> g(g_tune_lock);
+        if (g_tuned_row_log[K]) return;
+    }
+    const std::vector<int> cand = row_candidates(K);
+    int best = cand.empty() ? K : cand[0];
+    if (cand.size() > 1) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            if (e0) hipEventDestroy(e0);
+            return;
+        }
+        float best_ms = 1e30f;
+        for (int rows : cand) {
+            t_force_rows = rows;
+            bool ok = run();  // warm
+            hipEventRecord(e0, ex.stream);
+            for (int r = 0; r < 3 && ok; ++r) ok = run();
+            hipEventRecord(e1, ex.stream);
+            float ms = 1e30f;
+            if (ok && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
+            if (ok && ms < best_ms) {
+                best_ms = ms;
+                best = rows;
+            }
+        }
+        t_force_rows = 0;
+        hipStreamSynchronize(ex.stream);
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        clear_error();
+    }
+    std::lock_guard<std::mutex> g(g_tune_lock);
+    if (!g_tuned_row_log[K]) g_tuned_row_log[K] = best;
 }
 
 // ---------------------------------------------------------------------------
@@ -1293,6 +1338,17 @@ void ntt_mul_workspace_prepare(ntt_mul_workspace* ws, const ntt_mul_plan* plan) 
     ensure_tables(T / 2, ws->ex.stream, &f, &i, &st);
     ws->ex.spec_a.ensure((size_t)g.primes * T * 8);
     ws->ex.spec_b.ensure((size_t)g.primes * T * 8);
+    // Time the admissible splits of this length on operands of this shape.
+    const uint64_t la = plan->len_a, lb = plan->len_b;
+    Executor& ex = ws->ex;
+    if (g.log_t > kSinglePassMaxLog && ex.in_a.ensure(la * 8) && ex.in_b.ensure(lb * 8) &&
+        ex.out.ensure((la + lb) * 8) && hipMemsetAsync(ex.in_a.ptr, 0x5a, la * 8, ex.stream) == hipSuccess &&
+        hipMemsetAsync(ex.in_b.ptr, 0xa5, lb * 8, ex.stream) == hipSuccess) {
+        tune_split(ex, g.log_t, [&] {
+            return device_product(ex, g, ex.in_a.as<uint64_t>(), la, ex.in_b.as<uint64_t>(), lb, false,
+                                  ex.out.as<uint64_t>());
+        });
+    }
 }
 
 static bool ws_run(ntt_mul_workspace* ws, uint64_t* dst, size_t dst_len, const uint64_t* a, size_t a_len,
@@ -1397,6 +1453,22 @@ void ntt_conv_workspace_prepare(ntt_conv_workspace* ws, const ntt_conv_plan* pla
     ensure_tables(T / 2, ws->ex.stream, &f, &i, &st);
     ws->ex.spec_a.ensure((size_t)plan->primes * T * 8);
     ws->ex.spec_b.ensure((size_t)plan->primes * T * 8);
+    // Time the admissible splits of this length on zero coefficients of this
+    // shape (every coefficient within any bound).
+    Geometry g;
+    Executor& ex = ws->ex;
+    const uint64_t la = plan->len_a, lb = plan->len_b;
+    if ((int)plan->log_len > kSinglePassMaxLog && conv_geometry(plan, &g) && ex.in_a.ensure(la * 8) &&
+        ex.in_b.ensure(lb * 8) && ex.out.ensure(plan->output_len * 8) &&
+        hipMemsetAsync(ex.in_a.ptr, 0, la * 8, ex.stream) == hipSuccess &&
+        hipMemsetAsync(ex.in_b.ptr, 0, lb * 8, ex.stream) == hipSuccess) {
+        OpBuffers buf;
+        if (exec_buffers(ex, g, false, 0, &buf))
+            tune_split(ex, g.log_t, [&] {
+                return convolution_on(ex.stream, g, plan, ex.in_a.as<uint64_t>(), ex.in_b.as<uint64_t>(), false, buf,
+                                      ex.out.as<uint64_t>());
+            });
+    }
 }
 
 bool ntt_conv_workspace_convolve(ntt_conv_workspace* ws, uint64_t* dst, size_t dst_len, const uint64_t* a,
