@@ -6,7 +6,7 @@ limbs, seeds 1 and 2, as the reference benchmarks use), operands and result
 resident in HBM. A step is one MulWorkspace product on the device; the metric
 is operand limbs per second (Mlimbs/s). Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--workload mul_2p20|sqr_2p24|conv16_2p22|conv32s_2p22|batch4096]
+    python bench.py [--steps K] [--warmup W] [--workload mul_2p20|sqr_2p24|conv16_2p22|conv16_5x2p20|conv32s_2p22|batch4096|served]
 
 The line also carries the batched form ("batched": 4,096 uneven products in
 one graph launch against the plain one-call-per-product loop) unless
@@ -52,6 +52,12 @@ def workload(name: str):
         n = 1 << 22
         return dict(kind="conv", a=splitmix(n, 1) & np.uint64(0xFFFF), b=splitmix(n, 2) & np.uint64(0xFFFF),
                     max=65535, signed=False)
+    if name == "conv16_5x2p20":
+        # 5*2^20 coefficients: a convolution of 5*2^21 - 1 terms, just above
+        # 2^23, on a composite row of 5*2^21 points.
+        n = 5 << 20
+        return dict(kind="conv", a=splitmix(n, 1) & np.uint64(0xFFFF), b=splitmix(n, 2) & np.uint64(0xFFFF),
+                    max=65535, signed=False)
     if name == "conv32s_2p22":
         n = 1 << 22
 
@@ -63,27 +69,50 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def kernel_bytes(name: str, primes: int, T: int, la: int, lb: int, out_len: int, square: bool) -> float:
-    """Algorithmic HBM bytes of one launch of each kernel kind: every residue
-    of a pass read and written once; packing reads the limbs once."""
+def row_parts(log_len: int, tail: int):
+    """Transform lengths of the parts of a row: 2^log_len, plus the tails of a
+    composite row (N/4 for tail 1, N/2 for tail 2, both for tail 3)."""
+    n = 1 << log_len
+    return [n] + {0: [], 1: [n // 4], 2: [n // 2], 3: [n // 2, n // 4]}[tail]
+
+
+def passes_of(n: int) -> int:
+    k = n.bit_length() - 1
+    return 1 if k <= 12 else (2 if k <= 23 else 3)
+
+
+def step_bytes(name: str, primes: int, log_len: int, tail: int, la: int, lb: int, out_len: int,
+               square: bool) -> float:
+    """Algorithmic HBM bytes that the launches of one kernel kind move in one
+    step: every residue of a pass read and written once, packing reads the
+    limbs once per transform it feeds, and the recombination of a composite
+    row reads the values it combines and writes the ones it changes."""
     ops = 1 if square else 2
-    if name == "fwd_pack_strided":  # one launch packs and transforms both operands
-        return 8.0 * (la if square else la + lb) + 8.0 * ops * primes * T
-    if name == "fwd_strided":
-        return 16.0 * ops * primes * T
-    if name == "inv_strided":
-        return 16.0 * primes * T
-    if name == "fused_rows":
-        return (16.0 if square else 24.0) * primes * T
-    if name == "inv_strided_canon":
-        return 16.0 * primes * T
+    limbs = 8.0 * (la if square else la + lb)
+    n = 1 << log_len
+    T = n + tail * (n // 4)
+    total = 0.0
+    for m in row_parts(log_len, tail):
+        passes = passes_of(m)
+        if name == "fused_single" and passes == 1:
+            total += limbs + 8.0 * primes * m
+        elif name == "fwd_pack_strided" and passes > 1:
+            total += limbs + 8.0 * ops * primes * m
+        elif name == "fwd_strided" and passes == 3:
+            total += 16.0 * ops * primes * m
+        elif name == "inv_strided" and passes == 3:
+            total += 16.0 * primes * m
+        elif name == "fused_rows" and passes > 1:
+            total += (16.0 if square else 24.0) * primes * m
+        elif name == "inv_strided_canon" and passes > 1:
+            total += 16.0 * primes * m
+    if name == "recombine" and tail:
+        total = 8.0 * primes * {1: 7 * n / 4, 2: 5 * n / 2, 3: 13 * n / 4}[tail]
     if name == "crt_product":
-        return 8.0 * primes * min(T, out_len * 64) + 8.0 * out_len
+        total = 8.0 * primes * min(T, out_len * 64) + 8.0 * out_len
     if name == "crt_disjoint":
-        return 8.0 * primes * (la + lb - 1) + 8.0 * out_len
-    if name == "fused_single":
-        return 8.0 * (la + lb) + 8.0 * primes * T
-    return 0.0
+        total = 8.0 * primes * (la + lb - 1) + 8.0 * out_len
+    return total
 
 
 def measured_traffic(workload: str) -> dict:
@@ -423,7 +452,7 @@ def main() -> int:
         out_len = plan.output_len()
         ws = ntt.ConvWorkspace()
         ws.prepare(plan)
-        primes, log_len = plan.primes, plan.log_len
+        primes, log_len, tail = plan.primes, plan.log_len, plan.row_tail
         chunk = 64
 
         def step():
@@ -434,6 +463,7 @@ def main() -> int:
         ws = ntt.MulWorkspace()
         ws.prepare(plan)
         primes, log_len, chunk = plan.primes, plan.log_len, plan.chunk_bits
+        tail = plan.row_tail
 
         if kind == "sqr":
             def step():
@@ -442,7 +472,7 @@ def main() -> int:
             def step():
                 return ws.mul_device(dout.ptr, out_len, da.ptr, la, db.ptr, lb)
     dout = ntt.DeviceBuffer(out_len * 8)
-    T = 1 << log_len
+    T = ntt.row_len(log_len, tail)
 
     for _ in range(args.warmup):
         assert step(), ntt.last_error()
@@ -491,21 +521,27 @@ def main() -> int:
     dominant, dom_total = None, -1.0
     for name, (ms, count) in prof.items():
         avg_ms = ms / max(count, 1)
-        nbytes = kernel_bytes(name, primes, T, la, lb, out_len, kind == "sqr")
+        # Bytes of one step's launches of this kind over their time in a step.
+        nbytes = step_bytes(name, primes, log_len, tail, la, lb, out_len, kind == "sqr")
+        step_ms = ms / args.steps
         per_kernel[name] = {"avg_us": round(avg_ms * 1000, 2), "launches": int(count),
-                            "GBps": round(nbytes / (avg_ms * 1e-3) / 1e9, 1) if nbytes else None}
+                            "GBps": round(nbytes / (step_ms * 1e-3) / 1e9, 1) if nbytes else None}
         if ms > dom_total and nbytes:
             dominant, dom_total = name, ms
     traffic = measured_traffic(args.workload)
     roofline = None
     if dominant:
-        d = per_kernel[dominant]
-        nbytes = kernel_bytes(dominant, primes, T, la, lb, out_len, kind == "sqr")
-        achieved = nbytes / (d["avg_us"] * 1e-6) / 1e9
+        ms, count = prof[dominant]
+        per_step = count // args.steps
+        nbytes = step_bytes(dominant, primes, log_len, tail, la, lb, out_len, kind == "sqr")
+        achieved = nbytes / (ms / args.steps * 1e-3) / 1e9
+        # A kind launched once per step: bytes and traffic per launch. A
+        # composite row launches some kinds once per part; then the figures
+        # cover the step's launches of that kind together.
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic.get(dominant),
-                    "algorithmic_bytes": nbytes}
+                    "traffic": traffic.get(dominant) if per_step == 1 else None,
+                    "algorithmic_bytes": nbytes, "launches_per_step": per_step}
 
     unit = "Mcoeff/s" if kind == "conv" else "Mlimbs/s"
     value = la / (ms_clean * 1e-3) / 1e6
@@ -524,7 +560,7 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic (SplitMix64 limbs, seeds 1 and 2), operands resident in HBM",
         "config": {"workload": args.workload, "len_a": la, "len_b": lb, "primes": primes,
-                   "chunk_bits": chunk, "transform_len": T},
+                   "chunk_bits": chunk, "transform_len": T, "cyclic_len": 1 << log_len},
         "kernels": per_kernel,
         "roofline": roofline,
     }

@@ -43,8 +43,10 @@ typedef struct ntt_mul_plan {
     uint64_t len_b;
     uint32_t primes;     /* catalog prefix of the device geometry */
     uint32_t chunk_bits; /* bits per coefficient */
-    uint32_t log_len;    /* log2 of the transform length */
+    uint32_t log_len;    /* log2 of the cyclic transform length N */
     uint32_t valid;      /* 1 for a plan from ntt_mul_plan_new */
+    uint32_t row_tail;   /* transform row of N + row_tail * N/4 points (0..3) */
+    uint32_t reserved;
 } ntt_mul_plan;
 
 /* MulPlan::new, src/plan.rs:53-69. false <=> None: a zero length, an
@@ -102,11 +104,13 @@ typedef struct ntt_conv_plan {
     uint32_t is_signed;
     uint32_t schoolbook; /* shorter operand of at most 24 terms: direct sums */
     uint32_t primes;
-    uint32_t log_len;
+    uint32_t log_len;    /* log2 of the cyclic transform length N */
     uint32_t bound_bits; /* bits of t*max_a*max_b, plus a sign bit if signed */
     uint32_t value_bits_a;
     uint32_t value_bits_b;
     uint32_t valid;
+    uint32_t row_tail;   /* transform row of N + row_tail * N/4 points (0..3) */
+    uint32_t reserved;
 } ntt_conv_plan;
 
 /* ConvPlan::new, src/convolve.rs:75-113. false <=> None: an empty operand,
@@ -334,9 +338,19 @@ void ntt_profile_enable(int on);
 int ntt_profile_read(char *names, size_t names_len, double *ms, uint64_t *counts, int cap);
 void ntt_profile_reset(void);
 
-/* Geometry the engine chose for a product shape (for reports and tests). */
+/* Row shape of src/profile/planner.rs:565-591 (row_shape): the shortest
+ * row of 2^k, 5N/4, 3N/2 or 7N/4 points (N = 2^(k-1) holding both operands,
+ * every tail of at least 32 points) for a convolution of conv_len
+ * coefficients whose operands have at most `longest`, with its cyclic
+ * length and root-table length. A single operation runs this composite row
+ * when it prices below the power-of-two row (host-only; needs no GPU). */
+bool ntt_row_shape(uint64_t conv_len, uint64_t longest, uint64_t *row_len, uint64_t *cyclic_len,
+                   uint64_t *table_len);
+
+/* Geometry the engine chose for a product shape (for reports and tests):
+ * the transform row has N + row_tail * N/4 points, N = 2^log_len. */
 bool ntt_mul_geometry(size_t len_a, size_t len_b, uint32_t *primes, uint32_t *chunk_bits,
-                      uint32_t *log_len);
+                      uint32_t *log_len, uint32_t *row_tail);
 
 /* Diagnostics (no counterpart in the reference; used by tests). */
 
@@ -353,9 +367,9 @@ bool ntt_debug_copy(void *d_dst, const void *d_src, uint64_t bytes);
 bool ntt_debug_tables(uint64_t len, double *fwd, double *inv);
 
 /* Runs the transforms of a product and copies the canonical residue rows
- * (primes x transform length) to `out`; `geometry` receives primes, chunk
- * bits and log2 of the transform length. With out == NULL only the geometry
- * is reported. */
+ * (primes x row length) to `out`; `geometry` (4 entries) receives primes,
+ * chunk bits, log2 of the cyclic length N and the row tail (row length
+ * N + tail * N/4). With out == NULL only the geometry is reported. */
 bool ntt_debug_residues(const uint64_t *a, size_t la, const uint64_t *b, size_t lb, int square,
                         double *out, size_t out_len, uint32_t *geometry);
 

@@ -18,6 +18,8 @@ from .api import (  # noqa: F401
     pinned_array,
     last_error,
     mul_geometry,
+    row_len,
+    row_shape,
     synchronize,
 )
 from ._ffi import LIB_PATH, load  # noqa: F401
@@ -36,6 +38,8 @@ __all__ = [
     "device_available",
     "last_error",
     "mul_geometry",
+    "row_len",
+    "row_shape",
     "synchronize",
     "load",
     "LIB_PATH",

@@ -26,6 +26,8 @@ class MulPlanC(ctypes.Structure):
         ("chunk_bits", u32),
         ("log_len", u32),
         ("valid", u32),
+        ("row_tail", u32),
+        ("reserved", u32),
     ]
 
 
@@ -45,6 +47,8 @@ class ConvPlanC(ctypes.Structure):
         ("value_bits_a", u32),
         ("value_bits_b", u32),
         ("valid", u32),
+        ("row_tail", u32),
+        ("reserved", u32),
     ]
 
 
@@ -168,9 +172,10 @@ SIGNATURES = {
         [ctypes.c_char_p, size_t, ctypes.POINTER(ctypes.c_double), P64, ctypes.c_int],
     ),
     "ntt_profile_reset": (None, []),
+    "ntt_row_shape": (ctypes.c_bool, [u64, u64, P64, P64, P64]),
     "ntt_mul_geometry": (
         ctypes.c_bool,
-        [size_t, size_t, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)],
+        [size_t, size_t, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)],
     ),
     "ntt_selftest_host": (ctypes.c_int, [u64]),
     "ntt_debug_copy": (ctypes.c_bool, [ctypes.c_void_p, ctypes.c_void_p, u64]),

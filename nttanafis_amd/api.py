@@ -78,6 +78,11 @@ class MulPlan:
     def log_len(self) -> int:
         return self._c.log_len
 
+    @property
+    def row_tail(self) -> int:
+        """Transform row of N + row_tail*N/4 points, N = 2^log_len."""
+        return self._c.row_tail
+
     def scratch_len(self) -> int:
         return _lib().ntt_mul_plan_scratch_len(ctypes.byref(self._c))
 
@@ -185,6 +190,11 @@ class ConvPlan:
     @property
     def log_len(self) -> int:
         return self._c.log_len
+
+    @property
+    def row_tail(self) -> int:
+        """Transform row of N + row_tail*N/4 points, N = 2^log_len."""
+        return self._c.row_tail
 
     @property
     def schoolbook(self) -> bool:
@@ -482,11 +492,29 @@ def synchronize() -> bool:
 
 
 def mul_geometry(len_a: int, len_b: int):
-    """(primes, chunk_bits, log_len) the engine picks for a product shape."""
-    p, w, k = _ffi.u32(), _ffi.u32(), _ffi.u32()
-    if not _lib().ntt_mul_geometry(len_a, len_b, ctypes.byref(p), ctypes.byref(w), ctypes.byref(k)):
+    """(primes, chunk_bits, log_len, row_tail) the engine picks for a product
+    shape: the transform row has N + row_tail*N/4 points, N = 2^log_len."""
+    p, w, k, t = _ffi.u32(), _ffi.u32(), _ffi.u32(), _ffi.u32()
+    if not _lib().ntt_mul_geometry(len_a, len_b, ctypes.byref(p), ctypes.byref(w), ctypes.byref(k),
+                                   ctypes.byref(t)):
         return None
-    return p.value, w.value, k.value
+    return p.value, w.value, k.value, t.value
+
+
+def row_shape(conv_len: int, longest: int):
+    """(row_len, cyclic_len, table_len) of the reference's row_shape
+    (src/profile/planner.rs:565-591) for a convolution of conv_len
+    coefficients whose operands have at most `longest` coefficients."""
+    r, c, t = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    if not _lib().ntt_row_shape(conv_len, longest, ctypes.byref(r), ctypes.byref(c), ctypes.byref(t)):
+        return None
+    return r.value, c.value, t.value
+
+
+def row_len(log_len: int, row_tail: int) -> int:
+    """Points of a transform row: N + row_tail*N/4 with N = 2^log_len."""
+    n = 1 << log_len
+    return n + row_tail * (n >> 2)
 
 
 class Profile:

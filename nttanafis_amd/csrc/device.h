@@ -105,6 +105,18 @@ struct PassArgs {
     Field field;
     double tinv[kMaxPrimes];                 // centered T^-1 mod p
     double powers[kMaxPrimes][kMaxParts];    // centered 2^(49k) mod p
+    // Part of a composite row: the transform of T points runs below root-table
+    // node node_base (0 for x^T - 1; 2, 4, 6 for the factors x^T - W[node]^2
+    // of x^(2T') + 1 that a composite row adds), so a line of depth d that
+    // is node h of the part is node (node_base << d) + h of the table.
+    uint64_t node_base;
+    // Packing into a part: point n of the part is the sum over q < fold_terms
+    // of coefficient n + q*fold_span times fold_mult[y][q-1] (q >= 1), the
+    // residue of the operand modulo x^fold_span - W[node_base]^2.
+    uint64_t fold_span;
+    int32_t fold_terms;                      // 1: no fold
+    int32_t pad2;
+    double fold_mult[kMaxPrimes][3];         // centered
     // Batched launches: items[blockIdx.x / tiles_per_item] names the product
     // and prime of a workgroup; batch == nullptr for a single operation.
     const BatchProduct* batch;
@@ -154,6 +166,25 @@ struct TableArgs {
     Field field;
     double roots[kMaxPrimes][48];
     double inv_roots[kMaxPrimes][48];
+};
+
+// Recombination of a composite row of T = N + tail*N/4 points (the
+// recombine<5|3|7> step of src/transform/finish.rs:106-158). The cyclic
+// part u holds the product modulo x^N - 1 and the tails hold it modulo the
+// factors of x^N + 1 at nodes 4 (tail 1), 2 (tail 2), or 2 and 6 (tail 3),
+// all canonical; the kernel rewrites the row in place as the T canonical
+// coefficients. W1 = W[1] (a fourth root of unity), W2 = W[2], W3 = W[3].
+struct RecombineArgs {
+    double* rows;  // prime y at rows + y * row_stride
+    uint64_t row_stride;
+    uint64_t n;  // cyclic length N
+    uint32_t tail;
+    uint32_t pad;
+    Field field;
+    double w1[kMaxPrimes];
+    double s[kMaxPrimes][3];  // W2, W2^2, W2^3
+    double t[kMaxPrimes][3];  // W3, W3^2, W3^3
+    double half[kMaxPrimes];  // 2^-1
 };
 
 constexpr uint32_t kInf = 0xFFFFFFFFu;

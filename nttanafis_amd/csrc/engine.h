@@ -34,14 +34,36 @@ struct Geometry {
     uint32_t value_bits_b = 0;
     uint64_t digits_a = 0, digits_b = 0;
     uint64_t conv_len = 0;
+    // Transform row: 2^log_t points (tail 0), or a composite row of
+    // N + tail*N/4 points with N = 2^log_t, evaluated modulo x^N - 1 and the
+    // factors of x^N + 1 at table nodes 4 (tail 1), 2 (tail 2), 2 and 6
+    // (tail 3) (src/profile/planner.rs:565-591, src/transform/forward.rs:95-197).
     int log_t = 0;
+    uint32_t tail = 0;
     uint32_t bound_bits = 0;  // bits of every reconstructed coefficient
     bool disjoint = false;
     bool is_signed = false;
     uint32_t stride = 0;  // limbs per result coefficient (convolution)
 };
 
-bool select_product(uint64_t la, uint64_t lb, Geometry* out);
+// Points of the transform row.
+inline uint64_t row_len(const Geometry& g) {
+    const uint64_t n = (uint64_t)1 << g.log_t;
+    return n + g.tail * (n >> 2);
+}
+
+// One transform of a row: `log_n` points at table node `node`, `offset`
+// points into the row; a composite row has two or three.
+struct RowPart {
+    int log_n;
+    uint32_t node;
+    uint64_t offset;
+};
+int row_parts(const Geometry& g, RowPart* parts);
+
+// batched: geometry of a product inside a batch, which shares launches
+// among its products and runs power-of-two rows only.
+bool select_product(uint64_t la, uint64_t lb, Geometry* out, bool batched = false);
 bool conv_bound_bits(uint64_t terms, uint64_t max_a, uint64_t max_b, uint64_t* bits);
 bool product_geometry(const ntt_mul_plan* plan, Geometry* g);
 bool conv_geometry(const ntt_conv_plan* p, Geometry* g);

@@ -25,6 +25,7 @@ bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, hip
 bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, hipStream_t s);
 bool launch_crt_product(const CrtArgs& a, hipStream_t s);
 bool launch_crt_disjoint(const CrtArgs& a, hipStream_t s);
+bool launch_recombine(const RecombineArgs& a, int primes, hipStream_t s);
 bool launch_conv_columns(const uint64_t* a, uint64_t la, const uint64_t* b, uint64_t lb, uint64_t* out,
                          uint32_t stride, uint32_t is_signed, hipStream_t s);
 
@@ -200,16 +201,87 @@ static int crt_words(int np) {
 
 static int passes_of(int K) { return K <= kSinglePassMaxLog ? 1 : (K <= 23 ? 2 : 3); }
 
+// Device price of transforms, in units of about 0.6 ps of kernel time: HBM
+// passes and levels over the points of each part, a recombination pass over
+// a composite row, and a fixed price per launch.
+static double transform_price(int np, int log_n) {
+    return np * (double)((uint64_t)1 << log_n) * (8.0 * passes_of(log_n) + log_n);
+}
+constexpr double kLaunchPrice = 5e6;
+
+int row_parts(const Geometry& g, RowPart* parts) {
+    const uint64_t n = (uint64_t)1 << g.log_t;
+    parts[0] = RowPart{g.log_t, 0, 0};
+    switch (g.tail) {
+        case 1: parts[1] = RowPart{g.log_t - 2, 4, n}; return 2;
+        case 2: parts[1] = RowPart{g.log_t - 1, 2, n}; return 2;
+        case 3:
+            parts[1] = RowPart{g.log_t - 1, 2, n};
+            parts[2] = RowPart{g.log_t - 2, 6, n + n / 2};
+            return 3;
+        default: return 1;
+    }
+}
+
+static double row_price(const Geometry& g, double launch_price) {
+    RowPart parts[3];
+    const int n = row_parts(g, parts);
+    double price = 0;
+    int launches = 0;
+    for (int i = 0; i < n; ++i) {
+        price += transform_price((int)g.primes, parts[i].log_n);
+        launches += 2 * passes_of(parts[i].log_n) - 1;
+    }
+    if (g.tail) {
+        price += 8.0 * g.primes * (double)row_len(g);
+        ++launches;
+    }
+    return price + launch_price * launches;
+}
+
+// The composite row of row_shape (src/profile/planner.rs:565-591) for a
+// convolution of conv coefficients whose power-of-two row has 2^K points
+// (K >= kMinLog): with N = 2^(K-1) holding both operands, the shortest of
+// 5N/4 (N/4 >= 2^kMinLog), 3N/2 (N/2 >= 2^kMinLog), 7N/4 (N/4 >= 2^kMinLog)
+// that holds conv; 0 for none.
+static uint32_t composite_tail(uint64_t conv, uint64_t longest, int K) {
+    const uint64_t n = (uint64_t)1 << (K - 1), half = n >> 1, quarter = n >> 2;
+    const uint64_t min = (uint64_t)1 << kMinLog;
+    if (longest > n) return 0;
+    if (quarter >= min && conv <= n + quarter) return 1;
+    if (half >= min && conv <= n + half) return 2;
+    if (quarter >= min && conv <= n + half + quarter) return 3;
+    return 0;
+}
+
+// Row of a single operation's convolution: the cheaper of the power-of-two
+// row and the composite row, when one is admitted. NTTAMD_COMPOSITE=0 keeps
+// power-of-two rows, =1 takes every admitted composite row (both give the
+// same results; tests use them to reach every row form).
+static void choose_row(Geometry* g, uint64_t longest, int K) {
+    g->log_t = K;
+    g->tail = 0;
+    const char* env = getenv("NTTAMD_COMPOSITE");
+    const int mode = env ? atoi(env) : -1;
+    const uint32_t t = mode == 0 ? 0 : composite_tail(g->conv_len, longest, K);
+    if (!t) return;
+    Geometry c = *g;
+    c.log_t = K - 1;
+    c.tail = t;
+    if (mode == 1 || row_price(c, kLaunchPrice) < row_price(*g, kLaunchPrice)) *g = c;
+}
+
 // Product geometry: for every prefix, the widest coefficient whose products
 // fit its capacity (the narrowing of src/profile/planner.rs:168-183), priced
-// by device work: HBM passes and transform levels over np*T points.
-bool select_product(uint64_t la, uint64_t lb, Geometry* out) {
+// by device work: HBM passes and transform levels over the row's points.
+bool select_product(uint64_t la, uint64_t lb, Geometry* out, bool batched) {
     const Catalog& c = cat();
     if (la == 0 || lb == 0 || la > (UINT64_MAX >> 7) || lb > (UINT64_MAX >> 7)) return false;
     const uint64_t bits_a = la * 64, bits_b = lb * 64;
     bool found = false;
     double best = 0;
-    static const int forced = getenv("NTTAMD_PRIMES") ? atoi(getenv("NTTAMD_PRIMES")) : 0;
+    const char* forced_env = getenv("NTTAMD_PRIMES");  // a prime count for tests and sweeps
+    const int forced = forced_env ? atoi(forced_env) : 0;
     for (int np = 1; np <= kMaxPrimes; ++np) {
         if (forced && np != forced) continue;
         const uint32_t cap = c.capacity[np];
@@ -236,23 +308,28 @@ bool select_product(uint64_t la, uint64_t lb, Geometry* out) {
         const int nw = crt_words(np);
         const uint64_t span = (64 * (uint64_t)kCrtBlockLimbs + pb) / w + 2;
         if (span * nw > 4096) continue;
-        const double T = (double)((uint64_t)1 << K);
-        const double cost = np * T * (8.0 * passes_of(K) + K) + 4.0 * np * np * (double)conv;
+        Geometry g;
+        g.primes = np;
+        g.width = (uint32_t)w;
+        g.value_bits_a = g.value_bits_b = (uint32_t)w;
+        g.digits_a = da;
+        g.digits_b = db;
+        g.conv_len = conv;
+        g.bound_bits = (uint32_t)pb;
+        // A batch shares every launch among its products.
+        g.log_t = K;
+        const double cost = row_price(g, batched ? 0.0 : kLaunchPrice) + 4.0 * np * np * (double)conv;
         if (!found || cost < best) {
             found = true;
             best = cost;
-            Geometry g;
-            g.primes = np;
-            g.width = (uint32_t)w;
-            g.value_bits_a = g.value_bits_b = (uint32_t)w;
-            g.digits_a = da;
-            g.digits_b = db;
-            g.conv_len = conv;
-            g.log_t = K;
-            g.bound_bits = (uint32_t)pb;
             *out = g;
         }
     }
+    // The prime count is chosen on power-of-two rows; a single operation
+    // then takes the composite row of that geometry when it prices lower
+    // (same primes and coefficients, so the reconstruction is unchanged).
+    // A batch runs power-of-two rows only.
+    if (found && !batched) choose_row(out, std::max(out->digits_a, out->digits_b), out->log_t);
     return found;
 }
 
@@ -394,25 +471,63 @@ Split split_of(int K) {
     return split_with_rows(K, rows);
 }
 
+// Pass arguments of a transform of 2^log_n points over rows of row_len(g)
+// points (a whole power-of-two row, or one part of a composite row).
 static PassArgs base_args(const Geometry& g, double* data, const double* fwd, const double* inv,
-                          uint64_t table_stride) {
+                          uint64_t table_stride, int log_n) {
     const Catalog& c = cat();
     PassArgs a;
     memset(&a, 0, sizeof a);
     a.data = data;
     a.fwd = fwd;
     a.inv = inv;
-    a.row_stride = (uint64_t)1 << g.log_t;
+    a.row_stride = row_len(g);
     a.table_stride = table_stride;
-    a.log_t = g.log_t;
+    a.log_t = log_n;
     a.field = c.field;
+    a.fold_terms = 1;
     memcpy(a.powers, c.powers, sizeof a.powers);
     for (uint32_t y = 0; y < g.primes; ++y) {
         const uint64_t m = c.p[y];
-        const uint64_t tinv = m - ((m - 1) >> g.log_t);  // T^-1 mod p, T | p-1
+        const uint64_t tinv = m - ((m - 1) >> log_n);  // 2^-log_n mod p, 2^log_n | p-1
         a.tinv[y] = centered(tinv, m);
     }
     return a;
+}
+
+static PassArgs base_args(const Geometry& g, double* data, const double* fwd, const double* inv,
+                          uint64_t table_stride) {
+    return base_args(g, data, fwd, inv, table_stride, g.log_t);
+}
+
+static uint64_t mul_mod(uint64_t a, uint64_t b, uint64_t m) { return (uint64_t)((u128)a * b % m); }
+
+// W[j] of the root table for j < 4 (W[1] a fourth root of unity, W[2] an
+// eighth root, W[3] = W[1] W[2]), exact.
+static uint64_t low_root(int y, int j) {
+    const Catalog& c = cat();
+    const uint64_t m = c.p[y], g = kCatalog[y][1];
+    const uint64_t w1 = pow_mod(g, (m - 1) >> 2, m), w2 = pow_mod(g, (m - 1) >> 3, m);
+    return j == 0 ? 1 : (j == 1 ? w1 : (j == 2 ? w2 : mul_mod(w1, w2, m)));
+}
+
+// Packing fold onto a part at table node `node` of span points: with
+// operands of at most 2*span (node 2) or 4*span (nodes 4, 6) coefficients,
+// coefficient n + q*span meets point n times W[node/2]^q, since the part's
+// modulus is x^span - W[node]^2 = x^span - W[node/2].
+static void set_fold(PassArgs* a, const Geometry& g, uint32_t node, uint64_t span) {
+    const Catalog& c = cat();
+    a->node_base = node;
+    a->fold_span = span;
+    a->fold_terms = node == 0 ? 1 : (node == 2 ? 2 : 4);
+    for (uint32_t y = 0; y < g.primes; ++y) {
+        const uint64_t m = c.p[y], w = low_root((int)y, (int)node / 2);
+        uint64_t v = 1;
+        for (int q = 0; q < 3; ++q) {
+            v = mul_mod(v, w, m);
+            a->fold_mult[y][q] = centered(v, m);
+        }
+    }
 }
 
 static Source source_of(const uint64_t* limbs, uint64_t len, uint64_t digits, uint32_t width,
@@ -463,46 +578,50 @@ static bool inverse_tail(const Split& sp, const PassArgs& a0, int primes, hipStr
 }
 
 // Device working memory of one product or convolution: the spectra rows
-// (primes x T doubles each; spec_b unused for a square) and the carry words
-// of the reconstruction (5 per output block).
-uint64_t op_spectra_bytes(const Geometry& g) { return (uint64_t)g.primes * ((uint64_t)1 << g.log_t) * 8; }
+// (primes x row_len doubles each; spec_b unused for a square) and the carry
+// words of the reconstruction (5 per output block).
+uint64_t op_spectra_bytes(const Geometry& g) { return (uint64_t)g.primes * row_len(g) * 8; }
 uint64_t op_carry_bytes(uint64_t out_len) { return ((out_len + kCrtBlockLimbs - 1) / kCrtBlockLimbs) * 5 * 4; }
 
-// Computes the canonical residues of the convolution into buf.spec_b (or
-// spec_a for a square or a single pass) and returns their pointer.
-static bool residues(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint64_t la, const uint64_t* d_b,
-                     uint64_t lb, bool square, const OpBuffers& buf, double** rows_out) {
-    const uint64_t T = (uint64_t)1 << g.log_t;
+// Root-table entries a geometry reads: T/2 for a row of 2^K points; a
+// composite row reaches nodes below 7N/8 of the table of 2N points.
+static uint64_t table_need(const Geometry& g) {
+    const uint64_t n = (uint64_t)1 << g.log_t;
+    return g.tail ? n : n / 2;
+}
+
+// Canonical residues of the convolution modulo x^(2^log_n) - W[node]^2 into
+// the part of the rows at `offset`: forward transforms of both operands
+// (packing folded onto the part), pointwise product and inverse. The result
+// lands in buf.spec_b (spec_a for a square).
+static bool part_residues(hipStream_t s, const Geometry& g, const RowPart& part, const Source& sa,
+                          const Source& sb, bool square, const OpBuffers& buf, const double* fwd,
+                          const double* inv, uint64_t tstride) {
     const int np = (int)g.primes;
-    const double *fwd = nullptr, *inv = nullptr;
-    uint64_t tstride = 0;
-    if (!ensure_tables(T / 2, s, &fwd, &inv, &tstride)) {
-        set_error("root table allocation failed");
-        return false;
-    }
-    const Source sa = source_of(d_a, la, g.digits_a, g.width, g.value_bits_a, g.is_signed);
-    const Source sb = source_of(d_b, lb, g.digits_b, g.width, g.value_bits_b, g.is_signed);
-    const Split sp = split_of(g.log_t);
-    double* A = buf.spec_a;
-    double* B = square ? A : buf.spec_b;
+    double* A = buf.spec_a + part.offset;
+    double* B = (square ? buf.spec_a : buf.spec_b) + part.offset;
+    const Split sp = split_of(part.log_n);
     if (sp.n == 1) {
-        PassArgs a = base_args(g, A, fwd, inv, tstride);
+        // One workgroup per prime packs both operands, transforms, multiplies
+        // and stores the product rows into B.
+        PassArgs a = base_args(g, B, fwd, inv, tstride, part.log_n);
+        set_fold(&a, g, part.node, (uint64_t)1 << part.log_n);
         a.src_a = sa;
         a.src_b = sb;
         a.level_start = 0;
         Timed t("fused_single", s);
-        if (!launch_fused(g.log_t, a, np, true, square, s)) return false;
-        *rows_out = A;
-        return true;
+        return launch_fused(part.log_n, a, np, true, square, s);
     }
     const int row_log = sp.logs[sp.n - 1];
-    const int row_level = g.log_t - row_log;
+    const int row_level = part.log_n - row_log;
     // Strided forward passes of both operands, then one fused row pass:
     // forward rows of both, pointwise product, inverse rows.
-    PassArgs pa = base_args(g, A, fwd, inv, tstride);
+    PassArgs pa = base_args(g, A, fwd, inv, tstride, part.log_n);
+    set_fold(&pa, g, part.node, (uint64_t)1 << part.log_n);
     pa.other = B;
     if (!forward_head(sp, pa, sa, sb, square ? 1 : 2, np, s)) return false;
-    PassArgs pf = base_args(g, B, fwd, inv, tstride);
+    PassArgs pf = base_args(g, B, fwd, inv, tstride, part.log_n);
+    pf.node_base = part.node;
     pf.other = A;
     pf.level_start = row_level;
     pf.line_stride = 1;
@@ -510,8 +629,53 @@ static bool residues(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint
         Timed t("fused_rows", s);
         if (!launch_fused(row_log, pf, np, false, square, s)) return false;
     }
-    if (!inverse_tail(sp, pf, np, s)) return false;
-    *rows_out = B;
+    return inverse_tail(sp, pf, np, s);
+}
+
+static bool recombine(hipStream_t s, const Geometry& g, double* rows) {
+    const Catalog& c = cat();
+    RecombineArgs a;
+    memset(&a, 0, sizeof a);
+    a.rows = rows;
+    a.row_stride = row_len(g);
+    a.n = (uint64_t)1 << g.log_t;
+    a.tail = g.tail;
+    a.field = c.field;
+    for (uint32_t y = 0; y < g.primes; ++y) {
+        const uint64_t m = c.p[y];
+        a.w1[y] = centered(low_root((int)y, 1), m);
+        uint64_t sv = 1, tv = 1;
+        for (int q = 0; q < 3; ++q) {
+            sv = mul_mod(sv, low_root((int)y, 2), m);
+            tv = mul_mod(tv, low_root((int)y, 3), m);
+            a.s[y][q] = centered(sv, m);
+            a.t[y][q] = centered(tv, m);
+        }
+        a.half[y] = centered((m + 1) / 2, m);
+    }
+    Timed t("recombine", s);
+    return launch_recombine(a, (int)g.primes, s);
+}
+
+// Computes the canonical residues of the convolution into buf.spec_b (or
+// spec_a for a square or a single pass) and returns their pointer.
+static bool residues(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint64_t la, const uint64_t* d_b,
+                     uint64_t lb, bool square, const OpBuffers& buf, double** rows_out) {
+    const double *fwd = nullptr, *inv = nullptr;
+    uint64_t tstride = 0;
+    if (!ensure_tables(table_need(g), s, &fwd, &inv, &tstride)) {
+        set_error("root table allocation failed");
+        return false;
+    }
+    const Source sa = source_of(d_a, la, g.digits_a, g.width, g.value_bits_a, g.is_signed);
+    const Source sb = source_of(d_b, lb, g.digits_b, g.width, g.value_bits_b, g.is_signed);
+    RowPart parts[3];
+    const int n = row_parts(g, parts);
+    for (int i = 0; i < n; ++i)
+        if (!part_residues(s, g, parts[i], sa, sb, square, buf, fwd, inv, tstride)) return false;
+    double* rows = square ? buf.spec_a : buf.spec_b;
+    if (g.tail && !recombine(s, g, rows)) return false;
+    *rows_out = rows;
     return true;
 }
 
@@ -520,7 +684,7 @@ static CrtArgs crt_args(const Geometry& g, const double* rows, uint64_t* out, ui
     CrtArgs a;
     memset(&a, 0, sizeof a);
     a.residues = rows;
-    a.row_stride = (uint64_t)1 << g.log_t;
+    a.row_stride = row_len(g);
     a.out_len = out_len;
     a.out = out;
     a.primes = g.primes;
@@ -716,6 +880,7 @@ bool conv_geometry(const ntt_conv_plan* p, Geometry* g) {
     g->digits_b = p->len_b;
     g->conv_len = p->len_a + p->len_b - 1;
     g->log_t = (int)p->log_len;
+    g->tail = p->row_tail;
     g->bound_bits = p->bound_bits;
     g->disjoint = true;
     g->is_signed = p->is_signed != 0;
@@ -912,7 +1077,7 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
         B.square[i] = square ? 1 : 0;
         B.la[i] = la[i];
         B.lb[i] = square ? la[i] : lb[i];
-        if (B.la[i] == 0 || B.lb[i] == 0 || !select_product(B.la[i], B.lb[i], &B.geo[i])) {
+        if (B.la[i] == 0 || B.lb[i] == 0 || !select_product(B.la[i], B.lb[i], &B.geo[i], true)) {
             set_error("batch: product " + std::to_string(i) + " has an unsupported shape");
             return false;
         }
@@ -1300,6 +1465,7 @@ bool ntt_mul_plan_new(size_t len_a, size_t len_b, ntt_mul_plan* out) {
     p.primes = g.primes;
     p.chunk_bits = g.width;
     p.log_len = (uint32_t)g.log_t;
+    p.row_tail = g.tail;
     p.valid = 1;
     *out = p;
     return true;
@@ -1332,12 +1498,11 @@ void ntt_mul_workspace_prepare(ntt_mul_workspace* ws, const ntt_mul_plan* plan) 
     if (!select_product(plan->len_a, plan->len_b, &g)) return;
     std::lock_guard<std::mutex> guard(ws->ex.lock);
     if (!ws->ex.init()) return;
-    const uint64_t T = (uint64_t)1 << g.log_t;
     const double *f, *i;
     uint64_t st;
-    ensure_tables(T / 2, ws->ex.stream, &f, &i, &st);
-    ws->ex.spec_a.ensure((size_t)g.primes * T * 8);
-    ws->ex.spec_b.ensure((size_t)g.primes * T * 8);
+    ensure_tables(table_need(g), ws->ex.stream, &f, &i, &st);
+    ws->ex.spec_a.ensure(op_spectra_bytes(g));
+    ws->ex.spec_b.ensure(op_spectra_bytes(g));
     // Time the admissible splits of this length on operands of this shape.
     const uint64_t la = plan->len_a, lb = plan->len_b;
     Executor& ex = ws->ex;
@@ -1407,6 +1572,10 @@ bool ntt_conv_plan_new(size_t len_a, size_t len_b, uint64_t max_a, uint64_t max_
     const int K = std::max(kMinLog, log2_ceil(conv));
     if (K > kMaxRootOrder - 1) return false;
     if (conv > UINT64_MAX / stride) return false;
+    Geometry row;
+    row.primes = (uint32_t)np;
+    row.conv_len = conv;
+    choose_row(&row, std::max<uint64_t>(len_a, len_b), K);
     ntt_conv_plan p;
     memset(&p, 0, sizeof p);
     p.len_a = len_a;
@@ -1418,7 +1587,8 @@ bool ntt_conv_plan_new(size_t len_a, size_t len_b, uint64_t max_a, uint64_t max_
     p.is_signed = is_signed ? 1 : 0;
     p.schoolbook = terms <= kSchoolbookMaxTerms ? 1 : 0;
     p.primes = (uint32_t)np;
-    p.log_len = (uint32_t)K;
+    p.log_len = (uint32_t)row.log_t;
+    p.row_tail = row.tail;
     p.bound_bits = (uint32_t)bits;
     p.value_bits_a = (uint32_t)vba;
     p.value_bits_b = (uint32_t)vbb;
@@ -1447,18 +1617,18 @@ void ntt_conv_workspace_prepare(ntt_conv_workspace* ws, const ntt_conv_plan* pla
     if (plan->schoolbook) return;
     std::lock_guard<std::mutex> guard(ws->ex.lock);
     if (!ws->ex.init()) return;
-    const uint64_t T = (uint64_t)1 << plan->log_len;
+    Geometry g;
+    if (!conv_geometry(plan, &g)) return;
     const double *f, *i;
     uint64_t st;
-    ensure_tables(T / 2, ws->ex.stream, &f, &i, &st);
-    ws->ex.spec_a.ensure((size_t)plan->primes * T * 8);
-    ws->ex.spec_b.ensure((size_t)plan->primes * T * 8);
+    ensure_tables(table_need(g), ws->ex.stream, &f, &i, &st);
+    ws->ex.spec_a.ensure(op_spectra_bytes(g));
+    ws->ex.spec_b.ensure(op_spectra_bytes(g));
     // Time the admissible splits of this length on zero coefficients of this
     // shape (every coefficient within any bound).
-    Geometry g;
     Executor& ex = ws->ex;
     const uint64_t la = plan->len_a, lb = plan->len_b;
-    if ((int)plan->log_len > kSinglePassMaxLog && conv_geometry(plan, &g) && ex.in_a.ensure(la * 8) &&
+    if (g.log_t > kSinglePassMaxLog && ex.in_a.ensure(la * 8) &&
         ex.in_b.ensure(lb * 8) && ex.out.ensure(plan->output_len * 8) &&
         hipMemsetAsync(ex.in_a.ptr, 0, la * 8, ex.stream) == hipSuccess &&
         hipMemsetAsync(ex.in_b.ptr, 0, lb * 8, ex.stream) == hipSuccess) {
@@ -1890,11 +2060,12 @@ bool ntt_debug_residues(const uint64_t* a, size_t la, const uint64_t* b, size_t 
                         size_t out_len, uint32_t* geometry) {
     Geometry g;
     if (!select_product(la, lb, &g)) return false;
-    const uint64_t T = (uint64_t)1 << g.log_t;
+    const uint64_t T = row_len(g);
     if (geometry) {
         geometry[0] = g.primes;
         geometry[1] = g.width;
         geometry[2] = (uint32_t)g.log_t;
+        geometry[3] = g.tail;
     }
     if (!out) return true;
     if (out_len < g.primes * T) return false;
@@ -1915,7 +2086,7 @@ bool ntt_debug_residues(const uint64_t* a, size_t la, const uint64_t* b, size_t 
 bool ntt_debug_crt(size_t la, size_t lb, const double* rows, uint64_t* out) {
     Geometry g;
     if (!select_product(la, lb, &g)) return false;
-    const uint64_t T = (uint64_t)1 << g.log_t;
+    const uint64_t T = row_len(g);
     Executor* ex = plan_executor();
     std::lock_guard<std::mutex> guard(ex->lock);
     const uint64_t out_len = la + lb;
@@ -1929,12 +2100,29 @@ bool ntt_debug_crt(size_t la, size_t lb, const double* rows, uint64_t* out) {
     return hipMemcpy(out, ex->out.ptr, out_len * 8, hipMemcpyDeviceToHost) == hipSuccess;
 }
 
-bool ntt_mul_geometry(size_t len_a, size_t len_b, uint32_t* primes, uint32_t* chunk_bits, uint32_t* log_len) {
+bool ntt_row_shape(uint64_t conv_len, uint64_t longest, uint64_t* row, uint64_t* cyclic, uint64_t* table) {
+    if (conv_len == 0 || conv_len > ((uint64_t)1 << 40)) return false;
+    const int K = std::max(kMinLog, log2_ceil(conv_len));
+    Geometry g;
+    g.log_t = K;
+    if (const uint32_t t = composite_tail(conv_len, longest, K)) {
+        g.log_t = K - 1;
+        g.tail = t;
+    }
+    if (row) *row = row_len(g);
+    if (cyclic) *cyclic = (uint64_t)1 << g.log_t;
+    if (table) *table = table_need(g);
+    return true;
+}
+
+bool ntt_mul_geometry(size_t len_a, size_t len_b, uint32_t* primes, uint32_t* chunk_bits, uint32_t* log_len,
+                      uint32_t* row_tail) {
     Geometry g;
     if (!select_product(len_a, len_b, &g)) return false;
     if (primes) *primes = g.primes;
     if (chunk_bits) *chunk_bits = g.width;
     if (log_len) *log_len = (uint32_t)g.log_t;
+    if (row_tail) *row_tail = g.tail;
     return true;
 }
 

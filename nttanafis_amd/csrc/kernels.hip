@@ -304,16 +304,68 @@ __device__ __forceinline__ void pack_tile_as(double* lds, const Source& src, uin
     }
 }
 
+// Fold of the operand onto a part of a composite row: point n gains
+// coefficient n + q*span times m[q-1] for 1 <= q < terms.
+struct Fold {
+    uint64_t span;
+    int terms;
+    double m[3];
+};
+
+__device__ __forceinline__ Fold fold_of(const PassArgs& a, int y) {
+    return Fold{a.fold_span, a.fold_terms, {a.fold_mult[y][0], a.fold_mult[y][1], a.fold_mult[y][2]}};
+}
+
+// Adds the folded terms to the points of the tile this thread packed (only
+// points below digits - span have any). Each added term is below 15p/16 and
+// the packed point below p, so the sum stays below 8p before its reduction.
+template <int LOGL, int LINES, int NT, int KIND>
+__device__ __forceinline__ void fold_tile_as(double* lds, const Source& src, uint64_t n0, uint64_t S,
+                                             const Fold& fold, const double* pw, double p, double pinv) {
+    constexpr int L = 1 << LOGL;
+    constexpr int LS = Lay<LOGL>::STRIDE;
+    constexpr int TOTAL = LINES * L;
+    constexpr int R = (TOTAL + NT - 1) / NT;
+    const uint64_t digits = src.digits;
+    if (n0 + fold.span >= digits) return;
+#pragma unroll 1
+    for (int r = 0; r < R; ++r) {
+        const int f = threadIdx.x + r * NT;
+        if (TOTAL % NT != 0 && f >= TOTAL) break;
+        const uint64_t n = n0 + (uint64_t)(f / LINES) * S + (f % LINES);
+        if (n + fold.span >= digits) continue;
+        double* x = lds + (f % LINES) * LS + pidx(f / LINES);
+        double acc = *x;
+#pragma unroll 1
+        for (int q = 1; q < fold.terms; ++q) {
+            const uint64_t m = n + (uint64_t)q * fold.span;
+            if (m >= digits) break;
+            const double w = q == 1 ? fold.m[0] : (q == 2 ? fold.m[1] : fold.m[2]);
+            acc += mulmod(pack_as<KIND, true>(src, m, pw, p, pinv), w, p, pinv);
+        }
+        *x = reduce(acc, p, pinv);
+    }
+}
+
+template <int LOGL, int LINES, int NT, int KIND>
+__device__ __forceinline__ void pack_fold_as(double* lds, const Source& src, uint64_t n0, uint64_t S,
+                                             const Fold& fold, const double* pw, double p, double pinv) {
+    pack_tile_as<LOGL, LINES, NT, KIND>(lds, src, n0, S, pw, p, pinv);
+    if (fold.terms > 1) fold_tile_as<LOGL, LINES, NT, KIND>(lds, src, n0, S, fold, pw, p, pinv);
+}
+
 template <int LOGL, int LINES, int NT>
 __device__ __forceinline__ void pack_strided(double* lds, const Source& src, uint64_t n0, uint64_t S,
-                                             const double* pw, double p, double pinv) {
+                                             const Fold& fold, const double* pw, double p, double pinv) {
     switch (pack_kind(src)) {
-        case kPackW64: pack_tile_as<LOGL, LINES, NT, kPackW64>(lds, src, n0, S, pw, p, pinv); break;
-        case kPackW64Signed: pack_tile_as<LOGL, LINES, NT, kPackW64Signed>(lds, src, n0, S, pw, p, pinv); break;
-        case kPack1: pack_tile_as<LOGL, LINES, NT, kPack1>(lds, src, n0, S, pw, p, pinv); break;
-        case kPack2: pack_tile_as<LOGL, LINES, NT, kPack2>(lds, src, n0, S, pw, p, pinv); break;
-        case kPack3: pack_tile_as<LOGL, LINES, NT, kPack3>(lds, src, n0, S, pw, p, pinv); break;
-        default: pack_tile_as<LOGL, LINES, NT, kPack4>(lds, src, n0, S, pw, p, pinv); break;
+        case kPackW64: pack_fold_as<LOGL, LINES, NT, kPackW64>(lds, src, n0, S, fold, pw, p, pinv); break;
+        case kPackW64Signed:
+            pack_fold_as<LOGL, LINES, NT, kPackW64Signed>(lds, src, n0, S, fold, pw, p, pinv);
+            break;
+        case kPack1: pack_fold_as<LOGL, LINES, NT, kPack1>(lds, src, n0, S, fold, pw, p, pinv); break;
+        case kPack2: pack_fold_as<LOGL, LINES, NT, kPack2>(lds, src, n0, S, fold, pw, p, pinv); break;
+        case kPack3: pack_fold_as<LOGL, LINES, NT, kPack3>(lds, src, n0, S, fold, pw, p, pinv); break;
+        default: pack_fold_as<LOGL, LINES, NT, kPack4>(lds, src, n0, S, fold, pw, p, pinv); break;
     }
 }
 
@@ -451,12 +503,12 @@ __global__ __launch_bounds__(NT) void k_fwd_strided(PassArgs a) {
     const uint64_t base = H * span + lo0;
     double* row = j.data + (uint64_t)y * a.row_stride;
     if constexpr (PACK) {
-        pack_strided<LOGL, LINES, NT>(lds, j.src_a, base, S, a.powers[y], p, pinv);
+        pack_strided<LOGL, LINES, NT>(lds, j.src_a, base, S, fold_of(a, y), a.powers[y], p, pinv);
     } else {
         load_strided<LOGL, LINES, NT>(lds, row + base, S);
     }
     __syncthreads();
-    const Tw tw{a.fwd + (uint64_t)y * a.table_stride, H, 0, 0, 64, p, pinv};
+    const Tw tw{a.fwd + (uint64_t)y * a.table_stride, H + (a.node_base << a.level_start), 0, 0, 64, p, pinv};
     fwd_rounds<LOGL, LINES, NT, 0>(lds, tw, p, pinv);
     store_strided<LOGL, LINES, NT, false>(lds, row + base, S, 0.0, p, pinv);
 }
@@ -481,7 +533,7 @@ __global__ __launch_bounds__(NT) void k_inv_strided(PassArgs a) {
     double* row = j.data + (uint64_t)y * a.row_stride;
     load_strided<LOGL, LINES, NT>(lds, row + base, S);
     __syncthreads();
-    const Tw tw{a.inv + (uint64_t)y * a.table_stride, H, 0, 0, 64, p, pinv};
+    const Tw tw{a.inv + (uint64_t)y * a.table_stride, H + (a.node_base << a.level_start), 0, 0, 64, p, pinv};
     inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, tw, p, pinv);
     store_strided<LOGL, LINES, NT, CANON>(lds, row + base, S, a.tinv[y], p, pinv);
 }
@@ -517,8 +569,9 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     const uint64_t H0 = j.tile * LINES;
     double* row = j.data + (uint64_t)y * a.row_stride + H0 * L;
     if constexpr (PACK) {
-        pack_strided<LOGL, 1, NT>(lds, j.src_a, 0, 1, a.powers[y], p, pinv);
-        if constexpr (!SQUARE) pack_strided<LOGL, 1, NT>(lds + LS, j.src_b, 0, 1, a.powers[y], p, pinv);
+        const Fold fold = fold_of(a, y);
+        pack_strided<LOGL, 1, NT>(lds, j.src_a, 0, 1, fold, a.powers[y], p, pinv);
+        if constexpr (!SQUARE) pack_strided<LOGL, 1, NT>(lds + LS, j.src_b, 0, 1, fold, a.powers[y], p, pinv);
     } else {
         load_rows<LOGL, LINES, NT>(lds, row);
         if constexpr (!SQUARE)
@@ -526,8 +579,9 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     }
     __syncthreads();
     const int split = PACK ? 64 : row_split(LOGL);
-    const Tw twF{a.fwd + (uint64_t)y * a.table_stride, H0, 1, LINES - 1, split, p, pinv};
-    const Tw twI{a.inv + (uint64_t)y * a.table_stride, H0, 1, LINES - 1, split, p, pinv};
+    const uint64_t HR = H0 + (a.node_base << a.level_start);
+    const Tw twF{a.fwd + (uint64_t)y * a.table_stride, HR, 1, LINES - 1, split, p, pinv};
+    const Tw twI{a.inv + (uint64_t)y * a.table_stride, HR, 1, LINES - 1, split, p, pinv};
     // The lines of both tiles go through one set of rounds; line c of either
     // tile is node H0 + (c mod LINES).
     fwd_rounds<LOGL, TILES * LINES, NT, 0>(lds, twF, p, pinv);
@@ -555,6 +609,64 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     } else {
         store_rows<LOGL, LINES, NT>(lds, row);
     }
+}
+
+// Recombination of a composite row, in place. With c = c_lo + x^N c_hi the
+// product (deg c_lo < N, deg c_hi < tail*N/4), the cyclic part holds
+// u = c mod (x^N - 1) = c_lo + c_hi and a tail factor f of x^N + 1 holds
+// v = c mod f = (u - 2 c_hi) mod f, since x^N = -1 mod f. So
+// c_hi mod f = (u mod f - v) / 2, which is c_hi itself when f has degree
+// at least tail*N/4 (one tail), and c_lo = u - c_hi. Two tails (7N/4):
+// g1 = c_hi mod (x^(N/2) - W1), g2 = c_hi mod (x^(N/4) - W3) and
+// c_hi = A0 + x^(N/4) A1 + x^(N/2) B give A1 = g1[N/4..], A0 + W1 B = g1[..N/4]
+// and A0 + W3 A1 - W1 B = g2 (W3^2 = -W1), so A0 = (g2 + g1[..N/4] - W3 A1)/2
+// and B = W1 (A0 - g1[..N/4]) (W1^-1 = -W1). Thread k handles coefficient k
+// of each quarter (of each half for tail 2); every value read is canonical,
+// sums stay below 8p before each reduction and products below p^2.
+__global__ __launch_bounds__(256) void k_recombine(RecombineArgs a) {
+    const int y = blockIdx.y;
+    const double p = a.field.p[y], pinv = a.field.pinv[y];
+    const uint64_t N = a.n, Q = N >> 2;
+    const uint64_t count = a.tail == 2 ? N >> 1 : Q;
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    double* r = a.rows + (uint64_t)y * a.row_stride;
+    const double half = a.half[y], w1 = a.w1[y];
+    auto fix = [&](double x) { return canonical(reduce(x, p, pinv), p); };
+    if (a.tail == 2) {
+        const double u0 = r[k], u1 = r[k + 2 * Q], v = r[N + k];
+        const double h = mulmod(u0 + mulmod(u1, w1, p, pinv) - v, half, p, pinv);
+        r[k] = fix(u0 - h);
+        r[N + k] = fix(h);
+        return;
+    }
+    const double u0 = r[k], u1 = r[k + Q], u2 = r[k + 2 * Q], u3 = r[k + 3 * Q];
+    if (a.tail == 1) {
+        const double* s = a.s[y];
+        const double x = u0 + mulmod(u1, s[0], p, pinv) + mulmod(u2, s[1], p, pinv) + mulmod(u3, s[2], p, pinv) -
+                         r[N + k];
+        const double h = mulmod(reduce(x, p, pinv), half, p, pinv);
+        r[k] = fix(u0 - h);
+        r[N + k] = fix(h);
+        return;
+    }
+    const double* t = a.t[y];
+    const double v1a = r[N + k], v1b = r[N + Q + k], v2 = r[N + 2 * Q + k];
+    const double g1a = mulmod(reduce(u0 + mulmod(u2, w1, p, pinv) - v1a, p, pinv), half, p, pinv);
+    const double g1b = mulmod(reduce(u1 + mulmod(u3, w1, p, pinv) - v1b, p, pinv), half, p, pinv);
+    const double g2 = mulmod(reduce(u0 + mulmod(u1, t[0], p, pinv) + mulmod(u2, t[1], p, pinv) +
+                                        mulmod(u3, t[2], p, pinv) - v2,
+                                    p, pinv),
+                             half, p, pinv);
+    const double A1 = g1b;
+    const double A0 = mulmod(reduce(g2 + g1a - mulmod(t[0], A1, p, pinv), p, pinv), half, p, pinv);
+    const double B = mulmod(A0 - g1a, w1, p, pinv);
+    r[k] = fix(u0 - A0);
+    r[k + Q] = fix(u1 - A1);
+    r[k + 2 * Q] = fix(u2 - B);
+    r[N + k] = fix(A0);
+    r[N + Q + k] = fix(A1);
+    r[N + 2 * Q + k] = fix(B);
 }
 
 // Product reconstruction, one block of kCrtBlockLimbs output limbs: every
@@ -996,6 +1108,13 @@ bool launch_crt_product(const CrtArgs& a, hipStream_t s) {
     }
     if (!NTT_CHECK_LAUNCH()) return false;
     if (a.blocks > 1 || a.batch) NTT_LAUNCH("carry_apply", k_carry_apply, dim3(a.blocks), dim3(kApplyThreads), s, a);
+    return NTT_CHECK_LAUNCH();
+}
+
+bool launch_recombine(const RecombineArgs& a, int primes, hipStream_t s) {
+    const uint64_t count = a.tail == 2 ? a.n >> 1 : a.n >> 2;
+    if (a.tail < 1 || a.tail > 3 || count == 0) return false;
+    NTT_LAUNCH("recombine", k_recombine, dim3((unsigned)((count + 255) / 256), primes), dim3(256), s, a);
     return NTT_CHECK_LAUNCH();
 }
 

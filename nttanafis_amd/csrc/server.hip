@@ -406,9 +406,9 @@ static bool prepare_request(const ntt_request* r, Request* q) {
         q->out_len = q->plan.output_len;
         q->square = r->a == r->b && la == lb && !q->plan.schoolbook;
     } else {
-        if (!select_product(la, lb, &q->g)) return false;
-        q->out_len = la + lb;
         q->small = la + lb <= kSmallLimbs;
+        if (!select_product(la, lb, &q->g, q->small)) return false;
+        q->out_len = la + lb;
     }
     if (r->dst_len < q->out_len) return false;
     uint64_t bytes = 0;

@@ -68,18 +68,53 @@ def test_conv_plan_admission(kat):
 def test_device_geometry_fits_capacity(kat, la, lb):
     """Every coefficient of the engine's geometry stays below the CRT capacity:
     2w + ceil(log2(terms)) <= capacity (src/profile/planner.rs:558-563)."""
-    primes, w, log_len = ntt.mul_geometry(la, lb)
+    primes, w, log_len, tail = ntt.mul_geometry(la, lb)
     da, db = -(-64 * la // w), -(-64 * lb // w)
     terms = min(da, db)
     assert 2 * w + (terms - 1).bit_length() <= kat["capacities"][primes - 1]
-    assert (1 << log_len) >= da + db - 1
+    assert ntt.row_len(log_len, tail) >= da + db - 1
+    if tail:
+        # A composite row keeps both operands in its cyclic part.
+        assert max(da, db) <= 1 << log_len
 
 
 def test_headline_geometry():
     # 2^20 x 2^20 limbs: the engine's geometry holds the whole product in a
     # power-of-two transform of at most 2^21 points.
-    primes, w, log_len = ntt.mul_geometry(1 << 20, 1 << 20)
-    assert primes * (1 << log_len) <= 6 * (1 << 20)
+    primes, w, log_len, tail = ntt.mul_geometry(1 << 20, 1 << 20)
+    assert primes * ntt.row_len(log_len, tail) <= 6 * (1 << 20)
+
+
+# The reference's own row-shape table (src/tests/plans.rs:46-81): operand
+# lengths of byte-valued residues -> (row length, cyclic length, table length).
+ROW_SHAPES = [((1, 1), (32, 32, 16)), ((30, 30), (64, 64, 32)), ((40, 40), (96, 64, 64)),
+              ((49, 48), (96, 64, 64)), ((49, 49), (128, 128, 64)), ((64, 33), (96, 64, 64)),
+              ((65, 32), (128, 128, 64)), ((70, 20), (128, 128, 64)), ((80, 80), (160, 128, 128)),
+              ((100, 100), (224, 128, 128)), ((2100, 2100), (5120, 4096, 4096)),
+              ((3000, 2500), (6144, 4096, 4096)), ((3600, 3400), (7168, 4096, 4096)),
+              ((4097, 1000), (8192, 8192, 4096))]
+
+
+@pytest.mark.parametrize("lengths,expected", ROW_SHAPES)
+def test_row_shape_matches_the_reference_table(lengths, expected):
+    left, right = lengths
+    assert ntt.row_shape(left + right - 1, max(left, right)) == expected
+
+
+def test_row_shape_is_the_shortest_admitted_row():
+    """Brute force over small shapes: the row is the shortest of 2^k, 5N/4,
+    3N/2, 7N/4 that holds the convolution with both operands inside N and
+    every tail of at least 32 points."""
+    for conv in range(1, 3000, 7):
+        for longest in {1, conv // 3 + 1, (conv + 1) // 2, conv}:
+            power = max(32, 1 << (conv - 1).bit_length())
+            n = power // 2
+            rows = [(power, power, power // 2)]
+            if longest <= n:
+                for tail, part in ((1, n // 4), (2, n // 2), (3, n // 4)):
+                    if part >= 32 and conv <= n + tail * n // 4:
+                        rows.append((n + tail * n // 4, n, n))
+            assert ntt.row_shape(conv, longest) == min(rows), (conv, longest)
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])

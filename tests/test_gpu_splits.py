@@ -20,9 +20,11 @@ pytestmark = pytest.mark.gpu
 
 Q61 = (1 << 61) - 1
 
-# (len_a, len_b) per transform length, found with ntt.mul_geometry.
-SHAPES = [(4000, 4000), (8000, 8000), (30000, 30000), (200000, 200000), (800000, 800000),
-          (1 << 20, 1 << 20), (2500000, 2500000), (4000000, 4000000), (8000000, 8000000)]
+# (len_a, len_b, prime count or 0 for the engine's choice) per transform
+# length, found with ntt.mul_geometry (three primes put 4000 x 4000 limbs on
+# 2^13 points, where the engine's own choice is a one-pass 2^12).
+SHAPES = [(4000, 4000, 3), (8000, 8000, 0), (30000, 30000, 0), (200000, 200000, 0), (800000, 800000, 0),
+          (1 << 20, 1 << 20, 0), (2500000, 2500000, 0), (4000000, 4000000, 0), (8000000, 8000000, 0)]
 
 
 def say(msg):
@@ -67,9 +69,12 @@ def product(a, b):
     return dst
 
 
-@pytest.mark.parametrize("la,lb", SHAPES)
-def test_every_split_matches_the_default(oracle, la, lb):
-    primes, width, K = ntt.mul_geometry(la, lb)
+@pytest.mark.parametrize("la,lb,force_primes", SHAPES)
+def test_every_split_matches_the_default(oracle, monkeypatch, la, lb, force_primes):
+    monkeypatch.setenv("NTTAMD_COMPOSITE", "0")  # power-of-two rows; composite rows have their own tests
+    if force_primes:
+        monkeypatch.setenv("NTTAMD_PRIMES", str(force_primes))
+    primes, width, K, _ = ntt.mul_geometry(la, lb)
     say(f"{la}x{lb}: {primes} primes, {width}-bit coefficients, 2^{K} points")
     a, b = operand(la, 3), operand(lb, 4)
     os.environ.pop("NTTAMD_ROW_LOG", None)

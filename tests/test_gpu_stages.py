@@ -52,27 +52,26 @@ def conv_mod(da, db, p):
 
 
 def geometry(la, lb):
-    geo = (ctypes.c_uint32 * 3)()
+    """(primes, coefficient bits, row length)."""
+    geo = (ctypes.c_uint32 * 4)()
     lib = ntt.load()
     assert lib.ntt_debug_residues(None, la, None, lb, 0, None, 0, geo)
-    return int(geo[0]), int(geo[1]), int(geo[2])
+    return int(geo[0]), int(geo[1]), ntt.row_len(int(geo[2]), int(geo[3]))
 
 
 def residue_rows(a, b):
-    np_, width, K = geometry(len(a), len(b))
-    T = 1 << K
+    np_, width, T = geometry(len(a), len(b))
     out = np.zeros(np_ * T, dtype=np.float64)
-    geo = (ctypes.c_uint32 * 3)()
+    geo = (ctypes.c_uint32 * 4)()
     lib = ntt.load()
     ok = lib.ntt_debug_residues(a.ctypes.data_as(_ffi.P64), len(a), b.ctypes.data_as(_ffi.P64), len(b),
                                 1 if a is b else 0, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                 out.size, geo)
     assert ok, ntt.last_error()
-    return np_, width, K, out.reshape(np_, T)
+    return np_, width, T, out.reshape(np_, T)
 
 
-def expected_rows(oracle, a, b, np_, width, K):
-    T = 1 << K
+def expected_rows(oracle, a, b, np_, width, T):
     na = (len(a) * 64 + width - 1) // width
     nb = (len(b) * 64 + width - 1) // width
     da, db = digits(a, width, na), digits(b, width, nb)
@@ -91,8 +90,28 @@ SHAPES = [(40, 30, False), (300, 300, False), (1000, 777, False), (2000, 2000, T
 def test_residue_rows_are_the_convolution(oracle, la, lb, square):
     a = operand(la, 5)
     b = a if square else operand(lb, 6)
-    np_, width, K, rows = residue_rows(a, b)
-    want = expected_rows(oracle, a, b, np_, width, K)
+    np_, width, T, rows = residue_rows(a, b)
+    want = expected_rows(oracle, a, b, np_, width, T)
+    for y in range(np_):
+        bad = np.nonzero(rows[y] != want[y])[0]
+        assert bad.size == 0, f"prime {y}: {bad.size} residues differ (first at {bad[0]})"
+
+
+# Composite rows (forced): after the recombination (src/transform/finish.rs:
+# 106-158) the row holds the convolution's T coefficients mod each prime.
+COMPOSITE = [(2500, 2500, False, 1), (3000, 2900, False, 2), (3500, 3400, True, 3), (9000, 9000, False, 1),
+             (13000, 13000, False, 3)]
+
+
+@pytest.mark.parametrize("la,lb,square,tail", COMPOSITE)
+def test_composite_residue_rows_are_the_convolution(oracle, monkeypatch, la, lb, square, tail):
+    monkeypatch.setenv("NTTAMD_COMPOSITE", "1")
+    monkeypatch.setenv("NTTAMD_PRIMES", "3")
+    assert ntt.mul_geometry(la, lb)[3] == tail
+    a = operand(la, 5)
+    b = a if square else operand(lb, 6)
+    np_, width, T, rows = residue_rows(a, b)
+    want = expected_rows(oracle, a, b, np_, width, T)
     for y in range(np_):
         bad = np.nonzero(rows[y] != want[y])[0]
         assert bad.size == 0, f"prime {y}: {bad.size} residues differ (first at {bad[0]})"
@@ -101,8 +120,8 @@ def test_residue_rows_are_the_convolution(oracle, la, lb, square):
 @pytest.mark.parametrize("la,lb", [(40, 30), (1000, 777), (4000, 3000)])
 def test_reconstruction_of_exact_rows(oracle, la, lb):
     a, b = operand(la, 7), operand(lb, 8)
-    np_, width, K = geometry(la, lb)
-    rows = expected_rows(oracle, a, b, np_, width, K)
+    np_, width, T = geometry(la, lb)
+    rows = expected_rows(oracle, a, b, np_, width, T)
     out = np.zeros(la + lb, dtype=np.uint64)
     lib = ntt.load()
     ok = lib.ntt_debug_crt(la, lb, np.ascontiguousarray(rows).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
