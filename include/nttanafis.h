@@ -338,6 +338,14 @@ void ntt_profile_enable(int on);
 int ntt_profile_read(char *names, size_t names_len, double *ms, uint64_t *counts, int cap);
 void ntt_profile_reset(void);
 
+/* Unbalanced split the engine chose for a product shape
+ * (src/profile/planner.rs:283-393): the longer operand runs in `blocks`
+ * blocks of block_digits coefficients, each on a row of 2^log_len points
+ * (ntt_mul_geometry) against the shorter operand's spectrum; blocks == 0
+ * for a product that is not split. NTTAMD_SPLIT=0 disables splits, =1 takes
+ * the cheapest admissible one (tests); results are the same limbs. */
+bool ntt_mul_split(size_t len_a, size_t len_b, uint32_t *blocks, uint64_t *block_digits);
+
 /* Row shape of src/profile/planner.rs:565-591 (row_shape): the shortest
  * row of 2^k, 5N/4, 3N/2 or 7N/4 points (N = 2^(k-1) holding both operands,
  * every tail of at least 32 points) for a convolution of conv_len

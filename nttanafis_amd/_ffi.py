@@ -173,6 +173,7 @@ SIGNATURES = {
     ),
     "ntt_profile_reset": (None, []),
     "ntt_row_shape": (ctypes.c_bool, [u64, u64, P64, P64, P64]),
+    "ntt_mul_split": (ctypes.c_bool, [size_t, size_t, ctypes.POINTER(u32), P64]),
     "ntt_mul_geometry": (
         ctypes.c_bool,
         [size_t, size_t, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)],

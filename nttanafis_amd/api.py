@@ -511,6 +511,15 @@ def row_shape(conv_len: int, longest: int):
     return r.value, c.value, t.value
 
 
+def mul_split(len_a: int, len_b: int):
+    """(blocks, block_digits) of the unbalanced split the engine picks for a
+    product shape; blocks == 0 when the product is not split."""
+    n, d = _ffi.u32(), ctypes.c_uint64()
+    if not _lib().ntt_mul_split(len_a, len_b, ctypes.byref(n), ctypes.byref(d)):
+        return None
+    return n.value, d.value
+
+
 def row_len(log_len: int, row_tail: int) -> int:
     """Points of a transform row: N + row_tail*N/4 with N = 2^log_len."""
     n = 1 << log_len

@@ -115,8 +115,15 @@ struct PassArgs {
     // residue of the operand modulo x^fold_span - W[node_base]^2.
     uint64_t fold_span;
     int32_t fold_terms;                      // 1: no fold
-    int32_t pad2;
+    // Unbalanced split of a product (src/multiply/execute.rs:131-411): grid z
+    // below split_blocks is block z of the longer operand, whose rows sit at
+    // data + z*block_stride and whose coefficients are z*block_digits.. of
+    // src_a; z == split_blocks is the shorter operand (rows at other, source
+    // src_b), transformed once for every block.
+    uint32_t split_blocks;
     double fold_mult[kMaxPrimes][3];         // centered
+    uint64_t block_digits;
+    uint64_t block_stride;
     // Batched launches: items[blockIdx.x / tiles_per_item] names the product
     // and prime of a workgroup; batch == nullptr for a single operation.
     const BatchProduct* batch;
@@ -140,6 +147,12 @@ struct CrtArgs {
     uint32_t stride;         // conv: limbs per coefficient
     uint32_t is_signed;
     uint32_t primes;
+    // Unbalanced split: residues of coefficient i are the sum of the rows of
+    // the (at most two) blocks covering it, block z at residues +
+    // z*block_stride holding coefficients z*split_digits.. (0: no split).
+    uint32_t split_blocks;
+    uint64_t split_digits;
+    uint64_t block_stride;
     Field field;
     double pmod[kMaxPrimes][kMaxPrimes];  // centered p_j mod p_k (j < k)
     double ginv[kMaxPrimes];              // centered (p_0..p_{k-1})^-1 mod p_k

@@ -40,6 +40,12 @@ struct Geometry {
     // (tail 3) (src/profile/planner.rs:565-591, src/transform/forward.rs:95-197).
     int log_t = 0;
     uint32_t tail = 0;
+    // Unbalanced split of a product (src/profile/planner.rs:283-393): the
+    // longer operand's coefficients run in `blocks` blocks of block_digits (a
+    // multiple of 64), each on its own row of 2^log_t points against the
+    // shorter operand's spectrum, transformed once; 0 for none.
+    uint32_t blocks = 0;
+    uint64_t block_digits = 0;
     uint32_t bound_bits = 0;  // bits of every reconstructed coefficient
     bool disjoint = false;
     bool is_signed = false;
@@ -137,7 +143,10 @@ struct OpBuffers {
     double* spec_b;  // unused for a square
     uint32_t* carry;
 };
+// Spectra of operand a (every block of a split's longer operand) and of
+// operand b (the shorter operand of a split).
 uint64_t op_spectra_bytes(const Geometry& g);
+uint64_t op_spectra_bytes_b(const Geometry& g);
 uint64_t op_carry_bytes(uint64_t out_len);
 
 // Whole operations on device operands, queued on stream s.

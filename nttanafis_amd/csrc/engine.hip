@@ -271,6 +271,55 @@ static void choose_row(Geometry* g, uint64_t longest, int K) {
     if (mode == 1 || row_price(c, kLaunchPrice) < row_price(*g, kLaunchPrice)) *g = c;
 }
 
+// Unbalanced split (Geometry::cheapest_blocks and search_blocks,
+// src/profile/planner.rs:283-393): blocks of D coefficients of the longer
+// operand, D at least the shorter operand's count, a multiple of 64 (so a
+// block starts at a limb) and grown to fill a row of 2^K points. Priced as
+// one forward transform of the shorter operand plus a forward and an
+// inverse per block (a one-pass row repeats the shorter operand's forward
+// in every block); the reconstruction covers the same coefficients either
+// way. Taken when it prices at most three quarters of the whole product;
+// NTTAMD_SPLIT=0 never splits, =1 takes the cheapest admissible split.
+constexpr uint64_t kMaxBlocks = 8192;
+
+static double split_price(const Geometry& g) {
+    const int passes = passes_of(g.log_t);
+    const double tp = transform_price((int)g.primes, g.log_t);
+    const double work = passes == 1 ? tp * g.blocks : tp * (1.0 + 2.0 * g.blocks) / 3.0;
+    return work + kLaunchPrice * (2 * passes - 1);
+}
+
+static void choose_split(Geometry* g) {
+    const char* env = getenv("NTTAMD_SPLIT");
+    const int mode = env ? atoi(env) : -1;
+    if (mode == 0) return;
+    const uint64_t dl = std::max(g->digits_a, g->digits_b), ds = std::min(g->digits_a, g->digits_b);
+    if (dl < 2 * ds) return;
+    double best = mode == 1 ? 1e300 : 0.75 * row_price(*g, kLaunchPrice);
+    bool found = false;
+    Geometry pick = *g;
+    for (int K = std::max(kMinLog, log2_ceil(2 * ds - 1)); K <= g->log_t; ++K) {
+        const uint64_t T = (uint64_t)1 << K;
+        const uint64_t D = ((T - ds + 1) / 64) * 64;
+        if (D < ds) continue;
+        const uint64_t blocks = (dl + D - 1) / D;
+        if (blocks < 2) break;
+        if (blocks > kMaxBlocks) continue;
+        Geometry c = *g;
+        c.log_t = K;
+        c.tail = 0;
+        c.blocks = (uint32_t)blocks;
+        c.block_digits = D;
+        const double price = split_price(c);
+        if (price < best) {
+            best = price;
+            pick = c;
+            found = true;
+        }
+    }
+    if (found) *g = pick;
+}
+
 // Product geometry: for every prefix, the widest coefficient whose products
 // fit its capacity (the narrowing of src/profile/planner.rs:168-183), priced
 // by device work: HBM passes and transform levels over the row's points.
@@ -329,7 +378,10 @@ bool select_product(uint64_t la, uint64_t lb, Geometry* out, bool batched) {
     // then takes the composite row of that geometry when it prices lower
     // (same primes and coefficients, so the reconstruction is unchanged).
     // A batch runs power-of-two rows only.
-    if (found && !batched) choose_row(out, std::max(out->digits_a, out->digits_b), out->log_t);
+    if (found && !batched) {
+        choose_row(out, std::max(out->digits_a, out->digits_b), out->log_t);
+        choose_split(out);
+    }
     return found;
 }
 
@@ -580,7 +632,10 @@ static bool inverse_tail(const Split& sp, const PassArgs& a0, int primes, hipStr
 // Device working memory of one product or convolution: the spectra rows
 // (primes x row_len doubles each; spec_b unused for a square) and the carry
 // words of the reconstruction (5 per output block).
-uint64_t op_spectra_bytes(const Geometry& g) { return (uint64_t)g.primes * row_len(g) * 8; }
+uint64_t op_spectra_bytes(const Geometry& g) {
+    return (uint64_t)g.primes * row_len(g) * 8 * (g.blocks ? g.blocks : 1);
+}
+uint64_t op_spectra_bytes_b(const Geometry& g) { return (uint64_t)g.primes * row_len(g) * 8; }
 uint64_t op_carry_bytes(uint64_t out_len) { return ((out_len + kCrtBlockLimbs - 1) / kCrtBlockLimbs) * 5 * 4; }
 
 // Root-table entries a geometry reads: T/2 for a row of 2^K points; a
@@ -657,8 +712,43 @@ static bool recombine(hipStream_t s, const Geometry& g, double* rows) {
     return launch_recombine(a, (int)g.primes, s);
 }
 
+// Residues of a split product: every block of the longer operand (grid z)
+// times the shorter operand's spectrum, transformed once (z == blocks in the
+// forward passes), into buf.spec_a, block z at z * primes * T.
+static bool split_residues(hipStream_t s, const Geometry& g, const Source& sa, const Source& sb,
+                           const OpBuffers& buf, const double* fwd, const double* inv, uint64_t tstride,
+                           double** rows_out) {
+    const bool swapped = g.digits_a < g.digits_b;
+    const Source& lng = swapped ? sb : sa;
+    const Source& sht = swapped ? sa : sb;
+    const int np = (int)g.primes;
+    const uint64_t T = (uint64_t)1 << g.log_t;
+    PassArgs a = base_args(g, buf.spec_a, fwd, inv, tstride);
+    a.split_blocks = g.blocks;
+    a.block_digits = g.block_digits;
+    a.block_stride = (uint64_t)np * T;
+    a.other = buf.spec_b;
+    a.src_a = lng;
+    a.src_b = sht;
+    const Split sp = split_of(g.log_t);
+    *rows_out = buf.spec_a;
+    if (sp.n == 1) {
+        Timed t("fused_single", s);
+        return launch_fused(g.log_t, a, np, true, false, s);
+    }
+    if (!forward_head(sp, a, lng, sht, (int)g.blocks + 1, np, s)) return false;
+    const int row_log = sp.logs[sp.n - 1];
+    a.level_start = g.log_t - row_log;
+    a.line_stride = 1;
+    {
+        Timed t("fused_rows", s);
+        if (!launch_fused(row_log, a, np, false, false, s)) return false;
+    }
+    return inverse_tail(sp, a, np, s);
+}
+
 // Computes the canonical residues of the convolution into buf.spec_b (or
-// spec_a for a square or a single pass) and returns their pointer.
+// spec_a for a square or a split) and returns their pointer.
 static bool residues(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint64_t la, const uint64_t* d_b,
                      uint64_t lb, bool square, const OpBuffers& buf, double** rows_out) {
     const double *fwd = nullptr, *inv = nullptr;
@@ -669,6 +759,7 @@ static bool residues(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint
     }
     const Source sa = source_of(d_a, la, g.digits_a, g.width, g.value_bits_a, g.is_signed);
     const Source sb = source_of(d_b, lb, g.digits_b, g.width, g.value_bits_b, g.is_signed);
+    if (g.blocks && !square) return split_residues(s, g, sa, sb, buf, fwd, inv, tstride, rows_out);
     RowPart parts[3];
     const int n = row_parts(g, parts);
     for (int i = 0; i < n; ++i)
@@ -697,6 +788,11 @@ static CrtArgs crt_args(const Geometry& g, const double* rows, uint64_t* out, ui
     const uint32_t wbits = g.disjoint ? g.stride * 64 : g.width;
     a.width_magic = (uint32_t)(((1ull << 32) + wbits - 1) / wbits);
     a.is_signed = g.is_signed ? 1 : 0;
+    if (g.blocks) {
+        a.split_blocks = g.blocks;
+        a.split_digits = g.block_digits;
+        a.block_stride = (uint64_t)g.primes * row_len(g);
+    }
     return a;
 }
 
@@ -740,8 +836,8 @@ bool convolution_on(hipStream_t s, const Geometry& g, const ntt_conv_plan* p, co
 
 // Executor forms: grow the executor's own buffers, then run on its stream.
 static bool exec_buffers(Executor& ex, const Geometry& g, bool square, uint64_t out_len, OpBuffers* buf) {
-    const uint64_t sb = op_spectra_bytes(g);
-    if (!ex.spec_a.ensure(sb) || (!square && !ex.spec_b.ensure(sb)) || !ex.carry.ensure(op_carry_bytes(out_len))) {
+    if (!ex.spec_a.ensure(op_spectra_bytes(g)) || (!square && !ex.spec_b.ensure(op_spectra_bytes_b(g))) ||
+        !ex.carry.ensure(op_carry_bytes(out_len))) {
         set_error("device working memory allocation failed");
         return false;
     }
@@ -1502,7 +1598,7 @@ void ntt_mul_workspace_prepare(ntt_mul_workspace* ws, const ntt_mul_plan* plan) 
     uint64_t st;
     ensure_tables(table_need(g), ws->ex.stream, &f, &i, &st);
     ws->ex.spec_a.ensure(op_spectra_bytes(g));
-    ws->ex.spec_b.ensure(op_spectra_bytes(g));
+    ws->ex.spec_b.ensure(op_spectra_bytes_b(g));
     // Time the admissible splits of this length on operands of this shape.
     const uint64_t la = plan->len_a, lb = plan->len_b;
     Executor& ex = ws->ex;
@@ -1623,7 +1719,7 @@ void ntt_conv_workspace_prepare(ntt_conv_workspace* ws, const ntt_conv_plan* pla
     uint64_t st;
     ensure_tables(table_need(g), ws->ex.stream, &f, &i, &st);
     ws->ex.spec_a.ensure(op_spectra_bytes(g));
-    ws->ex.spec_b.ensure(op_spectra_bytes(g));
+    ws->ex.spec_b.ensure(op_spectra_bytes_b(g));
     // Time the admissible splits of this length on zero coefficients of this
     // shape (every coefficient within any bound).
     Executor& ex = ws->ex;
@@ -2060,6 +2156,10 @@ bool ntt_debug_residues(const uint64_t* a, size_t la, const uint64_t* b, size_t 
                         size_t out_len, uint32_t* geometry) {
     Geometry g;
     if (!select_product(la, lb, &g)) return false;
+    if (g.blocks) {
+        set_error("ntt_debug_residues: the shape runs as a split product (set NTTAMD_SPLIT=0)");
+        return false;
+    }
     const uint64_t T = row_len(g);
     if (geometry) {
         geometry[0] = g.primes;
@@ -2085,7 +2185,7 @@ bool ntt_debug_residues(const uint64_t* a, size_t la, const uint64_t* b, size_t 
 
 bool ntt_debug_crt(size_t la, size_t lb, const double* rows, uint64_t* out) {
     Geometry g;
-    if (!select_product(la, lb, &g)) return false;
+    if (!select_product(la, lb, &g) || g.blocks) return false;
     const uint64_t T = row_len(g);
     Executor* ex = plan_executor();
     std::lock_guard<std::mutex> guard(ex->lock);
@@ -2123,6 +2223,14 @@ bool ntt_mul_geometry(size_t len_a, size_t len_b, uint32_t* primes, uint32_t* ch
     if (chunk_bits) *chunk_bits = g.width;
     if (log_len) *log_len = (uint32_t)g.log_t;
     if (row_tail) *row_tail = g.tail;
+    return true;
+}
+
+bool ntt_mul_split(size_t len_a, size_t len_b, uint32_t* blocks, uint64_t* block_digits) {
+    Geometry g;
+    if (!select_product(len_a, len_b, &g)) return false;
+    if (blocks) *blocks = g.blocks;
+    if (block_digits) *block_digits = g.block_digits;
     return true;
 }
 

@@ -449,15 +449,33 @@ __device__ __forceinline__ bool xcd_tile(int units, uint64_t tiles, uint64_t* ti
     return t < tiles;
 }
 
+// Coefficients first.. (at most count of them) of a source, as a source of
+// their own; first*width is a multiple of 64, so they start at a limb.
+__device__ __forceinline__ Source block_source(Source s, uint64_t first, uint64_t count) {
+    const uint64_t limb0 = first * s.width / 64;
+    s.limbs += limb0;
+    s.limb_len -= limb0;
+    s.digits = s.digits - first < count ? s.digits - first : count;
+    return s;
+}
+
 template <bool BATCH, int G = 1>
 __device__ __forceinline__ bool resolve(const PassArgs& a, uint64_t tiles, bool xcd, Job* j) {
     if constexpr (!BATCH) {
-        // Forward passes over both operands in one launch: z = 1 is operand b,
-        // whose rows are a.other and whose source is a.src_b.
-        const bool opb = blockIdx.z != 0;
-        j->data = opb ? a.other : a.data;
+        const uint32_t z = blockIdx.z;
+        if (a.split_blocks) {
+            // Block z of the longer operand, or (z == split_blocks, forward
+            // passes only) the shorter one.
+            const bool block = z < a.split_blocks;
+            j->data = block ? a.data + (uint64_t)z * a.block_stride : a.other;
+            j->src_a = block ? block_source(a.src_a, (uint64_t)z * a.block_digits, a.block_digits) : a.src_b;
+        } else {
+            // Forward passes over both operands in one launch: z = 1 is
+            // operand b, whose rows are a.other and whose source is a.src_b.
+            j->data = z != 0 ? a.other : a.data;
+            j->src_a = z != 0 ? a.src_b : a.src_a;
+        }
         j->other = a.other;
-        j->src_a = opb ? a.src_b : a.src_a;
         j->src_b = a.src_b;
         if (xcd) return xcd_tile<G>(a.primes, tiles, &j->tile, &j->y);
         j->tile = blockIdx.x;
@@ -755,12 +773,43 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
     const int count = i_hi > i_lo ? (int)(i_hi - i_lo) : 0;
     // Residues of this thread's coefficients, every load issued first.
     double r[RMAX][NP];
+    if (a.split_blocks == 0) {
 #pragma unroll
-    for (int q = 0; q < RMAX; ++q) {
-        const int idx = threadIdx.x + q * kCrtThreads;
+        for (int q = 0; q < RMAX; ++q) {
+            const int idx = threadIdx.x + q * kCrtThreads;
 #pragma unroll
-        for (int k = 0; k < NP; ++k)
-            r[q][k] = idx < count ? J.residues[(uint64_t)k * J.row_stride + i_lo + idx] : 0.0;
+            for (int k = 0; k < NP; ++k)
+                r[q][k] = idx < count ? J.residues[(uint64_t)k * J.row_stride + i_lo + idx] : 0.0;
+        }
+    } else {
+        // Coefficient i of a split product: the rows of block z1 = i / D
+        // (the last block for i past it) at i - z1*D, plus those of block
+        // z1 - 1 at i - z1*D + D while that lies in its row; no third block
+        // reaches i, since blocks are at least as long as the shorter operand.
+        const uint64_t D = a.split_digits;
+#pragma unroll
+        for (int q = 0; q < RMAX; ++q) {
+            const int idx = threadIdx.x + q * kCrtThreads;
+            const uint64_t i = i_lo + idx;
+            uint64_t z1 = i / D;
+            if (z1 >= a.split_blocks) z1 = a.split_blocks - 1;
+            const uint64_t k1 = i - z1 * D;
+            const bool two = z1 > 0 && k1 + D < J.row_stride;
+            const double* r1 = J.residues + z1 * a.block_stride + k1;
+            const double* r0 = two ? r1 - a.block_stride + D : r1;
+#pragma unroll
+            for (int k = 0; k < NP; ++k) {
+                double v = 0.0;
+                if (idx < count) {
+                    v = r1[(uint64_t)k * J.row_stride];
+                    if (two) {
+                        v += r0[(uint64_t)k * J.row_stride];
+                        if (v >= a.field.p[k]) v -= a.field.p[k];
+                    }
+                }
+                r[q][k] = v;
+            }
+        }
     }
 #pragma unroll
     for (int q = 0; q < kCrtLimbsPerThread; ++q) {
@@ -992,8 +1041,9 @@ static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, hipSt
     b.primes = units;
     b.tiles_per_item = tiles;
     constexpr uint64_t G8 = 8 * (uint64_t)xcd_group(LINES);
-    dim3 grid((unsigned)tiles, (unsigned)units);
-    if (strided_xcd(false, LINES)) grid = dim3((unsigned)(((tiles + G8 - 1) / G8) * G8 * units), 1);
+    const unsigned blocks = a.split_blocks ? a.split_blocks : 1;
+    dim3 grid((unsigned)tiles, (unsigned)units, blocks);
+    if (strided_xcd(false, LINES)) grid = dim3((unsigned)(((tiles + G8 - 1) / G8) * G8 * units), 1, blocks);
     if (a.batch) grid = dim3((unsigned)(tiles * units), 1);
     if (a.batch) {
         if (canon)
@@ -1029,7 +1079,7 @@ static bool launch_fused_v(const PassArgs& a, int units, hipStream_t s) {
     const uint64_t tiles = rows / F::LINES;
     PassArgs b = a;
     b.tiles_per_item = tiles;
-    dim3 grid((unsigned)tiles, (unsigned)units);
+    dim3 grid((unsigned)tiles, (unsigned)units, a.split_blocks ? a.split_blocks : 1);
     if (a.batch) {
         grid = dim3((unsigned)(tiles * units), 1);
         NTT_LAUNCH(nullptr, (k_fused<LOGL, F::LINES, F::NT, PACK, SQUARE, true>), grid, dim3(F::NT), s, b);

@@ -179,7 +179,7 @@ struct Server {
                 ++it;
                 continue;
             }
-            const uint64_t sb = op_spectra_bytes(q->g) * (q->square ? 1 : 2);
+            const uint64_t sb = op_spectra_bytes(q->g) + (q->square ? 0 : op_spectra_bytes_b(q->g));
             if (!group->empty() && (group->size() >= kGroupMax || spectra + sb > kGroupSpectra)) break;
             spectra += sb;
             group->push_back(q);
@@ -291,9 +291,8 @@ bool Server::run_large(Slot& slot, Request* q) {
     }
     OpBuffers buf{nullptr, nullptr, nullptr};
     if (!(conv && q->plan.schoolbook)) {
-        const uint64_t sb = op_spectra_bytes(q->g);
-        buf.spec_a = reinterpret_cast<double*>(carve(sb));
-        buf.spec_b = q->square ? buf.spec_a : reinterpret_cast<double*>(carve(sb));
+        buf.spec_a = reinterpret_cast<double*>(carve(op_spectra_bytes(q->g)));
+        buf.spec_b = q->square ? buf.spec_a : reinterpret_cast<double*>(carve(op_spectra_bytes_b(q->g)));
         buf.carry = reinterpret_cast<uint32_t*>(carve(op_carry_bytes(q->out_len)));
     }
     const bool ok = conv ? convolution_on(s, q->g, &q->plan, da, db, q->square, buf, dout)
@@ -414,7 +413,8 @@ static bool prepare_request(const ntt_request* r, Request* q) {
     uint64_t bytes = 0;
     if (!q->device) bytes += align_up(la * 8) + (q->square ? 0 : align_up(lb * 8)) + align_up(q->out_len * 8);
     if (!(r->kind == NTT_REQ_CONV && q->plan.schoolbook))
-        bytes += align_up(op_spectra_bytes(q->g)) * (q->square ? 1 : 2) + align_up(op_carry_bytes(q->out_len));
+        bytes += align_up(op_spectra_bytes(q->g)) + (q->square ? 0 : align_up(op_spectra_bytes_b(q->g))) +
+                 align_up(op_carry_bytes(q->out_len));
     q->bytes = bytes;
     return true;
 }

@@ -72,6 +72,15 @@ def test_device_geometry_fits_capacity(kat, la, lb):
     da, db = -(-64 * la // w), -(-64 * lb // w)
     terms = min(da, db)
     assert 2 * w + (terms - 1).bit_length() <= kat["capacities"][primes - 1]
+    blocks, d = ntt.mul_split(la, lb)
+    if blocks:
+        # A split runs blocks of d coefficients of the longer operand (d at
+        # least the shorter operand's count, a multiple of 64) on rows that
+        # hold a block's convolution with the shorter operand.
+        dl, ds = max(da, db), min(da, db)
+        assert blocks >= 2 and d % 64 == 0 and d >= ds and (blocks - 1) * d < dl <= blocks * d
+        assert tail == 0 and (1 << log_len) >= d + ds - 1
+        return
     assert ntt.row_len(log_len, tail) >= da + db - 1
     if tail:
         # A composite row keeps both operands in its cyclic part.

@@ -39,12 +39,14 @@ def device():
     yield
     os.environ.pop("NTTAMD_COMPOSITE", None)
     os.environ.pop("NTTAMD_PRIMES", None)
+    os.environ.pop("NTTAMD_SPLIT", None)
 
 
 @contextlib.contextmanager
 def rows(composite, primes=None):
-    saved = {k: os.environ.get(k) for k in ("NTTAMD_COMPOSITE", "NTTAMD_PRIMES")}
+    saved = {k: os.environ.get(k) for k in ("NTTAMD_COMPOSITE", "NTTAMD_PRIMES", "NTTAMD_SPLIT")}
     os.environ["NTTAMD_COMPOSITE"] = "1" if composite else "0"
+    os.environ["NTTAMD_SPLIT"] = "0"  # whole products; splits have their own tests
     if primes:
         os.environ["NTTAMD_PRIMES"] = str(primes)
     else:
