@@ -41,12 +41,18 @@ void prof_record(const char* kind, hipEvent_t start, hipEvent_t stop);
         }                                                                                  \
     } while (0)
 
-__device__ __forceinline__ int pidx(int e) { return e + (e >> 5); }
+// Padded LDS index: one pad word after every 16 values. The last round of a
+// pass gives thread t the points 16t..16t+15 (or 8t..8t+7 in radix 8), so
+// the 16 lanes of one LDS cycle read values 16 (or 8) apart; with a pad per
+// 16 values they fall on 16 distinct bank pairs (a pad per 32 values left
+// pairs of lanes on one bank pair). Consecutive points and the 2-value
+// stripes of row loads stay conflict-free.
+__device__ __forceinline__ int pidx(int e) { return e + (e >> 4); }
 
 template <int LOGL>
 struct Lay {
     static constexpr int L = 1 << LOGL;
-    static constexpr int STRIDE = L + (L >> 5) + 1;  // padded line in LDS
+    static constexpr int STRIDE = L + (L >> 4) + 1;  // padded line in LDS (odd: lines fall on distinct banks)
     static constexpr int ROUNDS = (LOGL + 3) / 4;
     static constexpr int FIRST = LOGL - 4 * (ROUNDS - 1);  // levels of round 0
 };
@@ -185,6 +191,27 @@ __device__ __forceinline__ void inv_rounds(double* lds, Tw tw, double p, double 
         inv_round<LOGL, LINES, NT, L0, RL>(lds, tw, p, pinv);
         __syncthreads();
         inv_rounds<LOGL, LINES, NT, R - 1>(lds, tw, p, pinv);
+    }
+}
+
+// Inverse rounds of radix 8: rounds of 3 levels (the first of LOGL mod 3, or
+// 3), twice the groups of radix 16, for a tile whose radix-16 groups would
+// leave half of the threads idle. The schedule above holds for rounds of 1,
+// 2 and 3 levels: each ends on a reducing level 0 below 11p/8.
+template <int LOGL>
+struct Lay8 {
+    static constexpr int ROUNDS = (LOGL + 2) / 3;
+    static constexpr int FIRST = LOGL - 3 * (ROUNDS - 1);
+};
+
+template <int LOGL, int LINES, int NT, int R>
+__device__ __forceinline__ void inv_rounds8(double* lds, Tw tw, double p, double pinv) {
+    if constexpr (R >= 0) {
+        constexpr int L0 = R == 0 ? 0 : Lay8<LOGL>::FIRST + 3 * (R - 1);
+        constexpr int RL = R == 0 ? Lay8<LOGL>::FIRST : 3;
+        inv_round<LOGL, LINES, NT, L0, RL>(lds, tw, p, pinv);
+        __syncthreads();
+        inv_rounds8<LOGL, LINES, NT, R - 1>(lds, tw, p, pinv);
     }
 }
 
@@ -615,7 +642,12 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
             *x = mulmod(*x, x[LINES * LS], p, pinv);
     }
     __syncthreads();
-    inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, twI, p, pinv);
+    // The product is one tile: radix-16 groups would occupy half of the
+    // threads, so a product (not a square) runs its inverse in radix 8.
+    if constexpr (SQUARE)
+        inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, twI, p, pinv);
+    else
+        inv_rounds8<LOGL, LINES, NT, Lay8<LOGL>::ROUNDS - 1>(lds, twI, p, pinv);
     if constexpr (PACK) {
         const double tinv = a.tinv[y];
 #pragma unroll
