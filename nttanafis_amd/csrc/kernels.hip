@@ -80,14 +80,17 @@ struct Tw {
 
 // One DIF round: local levels [L0, L0 + RLOG) of every line of the tile.
 //
-// Reduction schedule, fixed at compile time: the upper input of a butterfly
-// is reduced at the odd levels of a round. With |root| <= p/2, a level maps
-// inputs below B to outputs below B + 15p/16 (the product stays below 2p^2),
-// and a reducing level maps inputs below 27p/8 to outputs below
-// 9p/16 + 15p/16 = 3p/2. Every round therefore ends below 3p/2 (its last
-// level is odd) or, for a short first round ending on an even level, below
-// 39p/16; values never exceed 27p/8, and spectra stay below 3p/2, so a
-// pointwise product stays below 9p^2/4.
+// Reduction schedule, fixed at compile time: the added (upper) input of a
+// butterfly is reduced at the last level of each round only. With
+// |root| <= p/2, a level maps inputs below B to outputs below B + 15p/16
+// while the multiplied input stays below 4p (its product below 2p^2), or
+// B + 11p/8 while it stays below 8p (product below 4p^2). A round entered
+// below 31p/16 reaches 31p/16 + 3*15p/16 = 19p/4 before its last level,
+// which reduces the added input to below 9p/16 and multiplies one below
+// 19p/4 (product below 2.4p^2), so it ends below 9p/16 + 11p/8 = 31p/16.
+// A short first round of 1-3 levels, entered below 31p/16, ends below 3p/2.
+// Values never exceed 19p/4 < 2^53, packed inputs are below p, and spectra
+// stay below 31p/16, so a pointwise product stays below 3.76p^2 < 4p^2.
 template <int LOGL, int LINES, int NT, int L0, int RLOG>
 __device__ __forceinline__ void fwd_round(double* lds, Tw tw, double p, double pinv) {
     constexpr int L = 1 << LOGL;
@@ -115,7 +118,7 @@ __device__ __forceinline__ void fwd_round(double* lds, Tw tw, double p, double p
 #pragma unroll
                 for (int j = 0; j < half; ++j) {
                     const int k = blk * 2 * half + j;
-                    const double u = (s & 1) ? reduce(x[k], p, pinv) : x[k];
+                    const double u = s == RLOG - 1 ? reduce(x[k], p, pinv) : x[k];
                     const double t = mulmod(x[k + half], w, p, pinv);
                     x[k] = u + t;
                     x[k + half] = u - t;
@@ -1023,7 +1026,7 @@ __global__ __launch_bounds__(256) void k_crt_disjoint(CrtArgs a) {
 // point of the tile moves in runs of at least 256 bytes.
 template <int LOGL>
 static constexpr int strided_lines() {
-    return LOGL <= 6 ? 64 : LOGL == 7 ? 32 : LOGL <= 9 ? 16 : (LOGL == 10 ? 8 : 4);
+    return LOGL <= 6 ? 64 : LOGL == 7 ? 32 : LOGL == 8 ? 16 : (LOGL == 9 ? 8 : (LOGL == 10 ? 8 : 4));
 }
 
 // Threads of a strided tile: one radix-16 group per thread and round.
