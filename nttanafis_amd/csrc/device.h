@@ -71,16 +71,27 @@ struct BatchProduct {
     uint32_t value_bits;
     uint32_t blocks;         // carry blocks of the output
     uint32_t* carry;         // 4 * blocks: base, thresh, first thresh, ripple end
+    // Unbalanced split (as in PassArgs): block z of the longer operand (src_a)
+    // owns rows at spec_a + z*block_stride; the shorter operand (src_b) is
+    // transformed once into spec_b. 0: no split.
+    uint32_t split_blocks;
+    uint32_t pad;
+    uint64_t block_digits;
+    uint64_t block_stride;
 };
 
-// Work items of a batched launch: product index and prime (passes) or
-// product index and carry block (reconstruction). In a forward pass, the
-// prime carries kItemOperandB when the item transforms the second operand.
+// Work items of a batched launch: product index and sub = prime | block << 4
+// (passes; block 0 unless the product is split) or product index and carry
+// block (reconstruction). In a forward pass, sub carries kItemOperandB when
+// the item transforms the second (shorter) operand.
 struct BatchItem {
     uint32_t product;
     uint32_t sub;
 };
 constexpr uint32_t kItemOperandB = 0x80000000u;
+constexpr uint32_t kItemPrimeMask = 0xFu;
+constexpr int kItemBlockShift = 4;
+constexpr uint32_t kItemBlockMask = 0x07FFFFFFu;  // after the shift
 
 // Which rows of a batched product a pass works on: forward passes take the
 // operand each item names; product passes take the product rows.

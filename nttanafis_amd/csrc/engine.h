@@ -188,6 +188,7 @@ struct Batch {
     size_t count = 0;
     std::vector<uint64_t> la, lb;
     std::vector<uint8_t> square;
+    std::vector<uint8_t> swapped;  // split product whose operand b is the longer one
     std::vector<Geometry> geo;
     std::vector<BatchProduct> host;
     std::vector<BatchBucket> buckets;

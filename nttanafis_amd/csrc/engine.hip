@@ -282,20 +282,21 @@ static void choose_row(Geometry* g, uint64_t longest, int K) {
 // NTTAMD_SPLIT=0 never splits, =1 takes the cheapest admissible split.
 constexpr uint64_t kMaxBlocks = 8192;
 
-static double split_price(const Geometry& g) {
+static double split_price(const Geometry& g, double launch_price) {
     const int passes = passes_of(g.log_t);
     const double tp = transform_price((int)g.primes, g.log_t);
     const double work = passes == 1 ? tp * g.blocks : tp * (1.0 + 2.0 * g.blocks) / 3.0;
-    return work + kLaunchPrice * (2 * passes - 1);
+    return work + launch_price * (2 * passes - 1);
 }
 
-static void choose_split(Geometry* g) {
+// launch_price: 0 inside a batch, whose launches every product shares.
+static void choose_split(Geometry* g, double launch_price) {
     const char* env = getenv("NTTAMD_SPLIT");
     const int mode = env ? atoi(env) : -1;
     if (mode == 0) return;
     const uint64_t dl = std::max(g->digits_a, g->digits_b), ds = std::min(g->digits_a, g->digits_b);
     if (dl < 2 * ds) return;
-    double best = mode == 1 ? 1e300 : 0.75 * row_price(*g, kLaunchPrice);
+    double best = mode == 1 ? 1e300 : 0.75 * row_price(*g, launch_price);
     bool found = false;
     Geometry pick = *g;
     for (int K = std::max(kMinLog, log2_ceil(2 * ds - 1)); K <= g->log_t; ++K) {
@@ -310,7 +311,7 @@ static void choose_split(Geometry* g) {
         c.tail = 0;
         c.blocks = (uint32_t)blocks;
         c.block_digits = D;
-        const double price = split_price(c);
+        const double price = split_price(c, launch_price);
         if (price < best) {
             best = price;
             pick = c;
@@ -377,10 +378,10 @@ bool select_product(uint64_t la, uint64_t lb, Geometry* out, bool batched) {
     // The prime count is chosen on power-of-two rows; a single operation
     // then takes the composite row of that geometry when it prices lower
     // (same primes and coefficients, so the reconstruction is unchanged).
-    // A batch runs power-of-two rows only.
-    if (found && !batched) {
-        choose_row(out, std::max(out->digits_a, out->digits_b), out->log_t);
-        choose_split(out);
+    // A batch runs power-of-two rows; both forms split unbalanced products.
+    if (found) {
+        if (!batched) choose_row(out, std::max(out->digits_a, out->digits_b), out->log_t);
+        choose_split(out, batched ? 0.0 : kLaunchPrice);
     }
     return found;
 }
@@ -1164,6 +1165,7 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
     B.la.assign(count, 0);
     B.lb.assign(count, 0);
     B.square.assign(count, 0);
+    B.swapped.assign(count, 0);
     B.geo.assign(count, Geometry());
     B.host.assign(count, BatchProduct());
     uint64_t arena = 0, carry = 0, max_half = 0;
@@ -1181,9 +1183,9 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
         const uint64_t T = (uint64_t)1 << g.log_t;
         max_half = std::max(max_half, T / 2);
         off_a[i] = arena;
-        arena += (uint64_t)g.primes * T;
+        arena += op_spectra_bytes(g) / 8;
         off_b[i] = arena;
-        if (!square) arena += (uint64_t)g.primes * T;
+        if (!square) arena += op_spectra_bytes_b(g) / 8;
         const uint64_t blocks = (B.la[i] + B.lb[i] + kCrtBlockLimbs - 1) / kCrtBlockLimbs;
         off_c[i] = carry;
         carry += 4 * blocks;
@@ -1206,6 +1208,14 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
         memset(&P, 0, sizeof P);
         P.src_a = source_of(nullptr, B.la[i], g.digits_a, g.width, g.value_bits_a, false);
         P.src_b = source_of(nullptr, B.lb[i], g.digits_b, g.width, g.value_bits_b, false);
+        if (g.blocks) {
+            // src_a is the longer operand (batch_bind keeps the swap).
+            B.swapped[i] = g.digits_a < g.digits_b ? 1 : 0;
+            if (B.swapped[i]) std::swap(P.src_a, P.src_b);
+            P.split_blocks = g.blocks;
+            P.block_digits = g.block_digits;
+            P.block_stride = (uint64_t)g.primes * ((uint64_t)1 << g.log_t);
+        }
         P.spec_a = A + off_a[i];
         P.spec_b = square ? P.spec_a : A + off_b[i];
         P.out_len = B.la[i] + B.lb[i];
@@ -1226,6 +1236,9 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
     B.buckets.clear();
     // which: 0 first operands, 1 second operands (forward items); 2 products
     // with two operands, 3 squares (product items).
+    // A split product has one first-operand item and one product item per
+    // block and prime (sub = prime | block << kItemBlockShift) and one
+    // second-operand item per prime for its shorter operand.
     auto add = [&](int K, int which) {
         for (size_t i = 0; i < count; ++i) {
             if (B.geo[i].log_t != K) continue;
@@ -1233,7 +1246,10 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
             if ((which == 1 || which == 2) && sq) continue;
             if (which == 3 && !sq) continue;
             const uint32_t flag = which == 1 ? kItemOperandB : 0u;
-            for (uint32_t y = 0; y < B.geo[i].primes; ++y) items.push_back({(uint32_t)i, y | flag});
+            const uint32_t blocks = which == 1 || B.geo[i].blocks == 0 ? 1 : B.geo[i].blocks;
+            for (uint32_t z = 0; z < blocks; ++z)
+                for (uint32_t y = 0; y < B.geo[i].primes; ++y)
+                    items.push_back({(uint32_t)i, y | (z << kItemBlockShift) | flag});
         }
     };
     for (int K = kMinLog; K <= kMaxLog; ++K) {
@@ -1277,8 +1293,10 @@ bool batch_bind(Batch& B, const uint64_t* const* d_a, const uint64_t* const* d_b
             set_error("batch: null operand or output pointer at product " + std::to_string(i));
             return false;
         }
-        B.host[i].src_a.limbs = d_a[i];
-        B.host[i].src_b.limbs = pb;
+        // A split product keeps its longer operand in src_a.
+        const bool swap = B.swapped[i] != 0;
+        B.host[i].src_a.limbs = swap ? pb : d_a[i];
+        B.host[i].src_b.limbs = swap ? d_a[i] : pb;
         B.host[i].out = d_out[i];
     }
     if (B.count && hipMemcpyAsync(B.products.ptr, B.host.data(), B.count * sizeof(BatchProduct),

@@ -515,9 +515,23 @@ __device__ __forceinline__ bool resolve(const PassArgs& a, uint64_t tiles, bool 
     const BatchItem it = a.items[blockIdx.x / a.tiles_per_item];
     const BatchProduct& P = a.batch[it.product];
     j->tile = blockIdx.x % a.tiles_per_item;
-    j->y = (int)(it.sub & ~kItemOperandB);
+    j->y = (int)(it.sub & kItemPrimeMask);
     j->src_a = P.src_a;
     j->src_b = P.src_b;
+    if (P.split_blocks) {
+        // Block z of the longer operand, or (forward items flagged as the
+        // second operand) the shorter one; a block's product lands in its rows.
+        const uint32_t z = (it.sub >> kItemBlockShift) & kItemBlockMask;
+        j->other = P.spec_b;
+        if (a.operand == kOperandForward && (it.sub & kItemOperandB)) {
+            j->data = P.spec_b;
+            j->src_a = P.src_b;
+        } else {
+            j->data = P.spec_a + (uint64_t)z * P.block_stride;
+            j->src_a = block_source(P.src_a, (uint64_t)z * P.block_digits, P.block_digits);
+        }
+        return true;
+    }
     j->other = P.spec_a;
     if (a.operand == kOperandProduct) {
         j->data = P.square ? P.spec_a : P.spec_b;
@@ -748,6 +762,8 @@ struct CrtJob {
     uint64_t* out;
     uint32_t width, magic, value_bits, block, blocks;
     uint32_t *agg_base, *agg_thresh, *first_thresh, *ripple_end;
+    uint32_t split_blocks;  // 0: no split
+    uint64_t split_digits, block_stride;
 };
 
 __device__ __forceinline__ CrtJob crt_job(const CrtArgs& a) {
@@ -767,11 +783,17 @@ __device__ __forceinline__ CrtJob crt_job(const CrtArgs& a) {
         j.agg_thresh = a.agg_thresh;
         j.first_thresh = a.first_thresh;
         j.ripple_end = a.ripple_end;
+        j.split_blocks = a.split_blocks;
+        j.split_digits = a.split_digits;
+        j.block_stride = a.block_stride;
         return j;
     }
     const BatchItem it = a.items[blockIdx.x];
     const BatchProduct& P = a.batch[it.product];
-    j.residues = P.square ? P.spec_a : P.spec_b;
+    j.residues = (P.square || P.split_blocks) ? P.spec_a : P.spec_b;
+    j.split_blocks = P.split_blocks;
+    j.split_digits = P.block_digits;
+    j.block_stride = P.block_stride;
     j.row_stride = P.row_stride;
     j.coefficients = P.coefficients;
     j.out_len = P.out_len;
@@ -808,7 +830,7 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
     const int count = i_hi > i_lo ? (int)(i_hi - i_lo) : 0;
     // Residues of this thread's coefficients, every load issued first.
     double r[RMAX][NP];
-    if (a.split_blocks == 0) {
+    if (J.split_blocks == 0) {
 #pragma unroll
         for (int q = 0; q < RMAX; ++q) {
             const int idx = threadIdx.x + q * kCrtThreads;
@@ -821,17 +843,17 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
         // (the last block for i past it) at i - z1*D, plus those of block
         // z1 - 1 at i - z1*D + D while that lies in its row; no third block
         // reaches i, since blocks are at least as long as the shorter operand.
-        const uint64_t D = a.split_digits;
+        const uint64_t D = J.split_digits;
 #pragma unroll
         for (int q = 0; q < RMAX; ++q) {
             const int idx = threadIdx.x + q * kCrtThreads;
             const uint64_t i = i_lo + idx;
             uint64_t z1 = i / D;
-            if (z1 >= a.split_blocks) z1 = a.split_blocks - 1;
+            if (z1 >= J.split_blocks) z1 = J.split_blocks - 1;
             const uint64_t k1 = i - z1 * D;
             const bool two = z1 > 0 && k1 + D < J.row_stride;
-            const double* r1 = J.residues + z1 * a.block_stride + k1;
-            const double* r0 = two ? r1 - a.block_stride + D : r1;
+            const double* r1 = J.residues + z1 * J.block_stride + k1;
+            const double* r0 = two ? r1 - J.block_stride + D : r1;
 #pragma unroll
             for (int k = 0; k < NP; ++k) {
                 double v = 0.0;

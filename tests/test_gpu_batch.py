@@ -119,6 +119,35 @@ def test_batch_mixed_vs_oracle(oracle):
         assert (out[i] == want).all(), f"product {i}: {la[i]}x{lb[i]} square={sq[i]}"
 
 
+def test_batch_split_products(oracle, monkeypatch):
+    """Unbalanced entries split into blocks of the longer operand inside the
+    batch (forced with NTTAMD_SPLIT=1): each block an item of its bucket's
+    launches, the shorter operand transformed once, the reconstruction
+    summing the blocks' residues; beside whole products and squares, with
+    either operand the longer one."""
+    monkeypatch.setenv("NTTAMD_SPLIT", "1")
+    la = [20000, 1000, 65536, 192, 65536, 5000, 300000, 12345, 4096, 3000, 7000]
+    lb = [1000, 20000, 192, 65536, 2000, 2500, 1000, 777, 4096, 3000, 64]
+    sq = [False] * 8 + [True, False, False]
+    for x, y, s in zip(la, lb, sq):
+        if not s and max(x, y) >= 2 * min(x, y):
+            assert ntt.mul_split(x, y)[0] >= 2, (x, y)
+    a, b = batch_operands(la, lb, sq, seed=23)
+    batch = plan_batch(la, lb, sq)
+    out = [np.zeros(batch.output_len(i), dtype=np.uint64) for i in range(len(la))]
+    assert batch.mul_host(a, b, out), ntt.last_error()
+    for i in range(len(la)):
+        want = expected(oracle, a[i], b[i], sq[i])
+        assert (out[i] == want).all(), f"product {i}: {la[i]}x{lb[i]}"
+    # The same entries without splits give the same limbs.
+    monkeypatch.setenv("NTTAMD_SPLIT", "0")
+    whole = plan_batch(la, lb, sq)
+    out2 = [np.zeros(whole.output_len(i), dtype=np.uint64) for i in range(len(la))]
+    assert whole.mul_host(a, b, out2), ntt.last_error()
+    for i in range(len(la)):
+        assert (out2[i] == out[i]).all(), f"product {i} split vs whole"
+
+
 def test_batch_all_ones_and_carries(oracle):
     """Maximal carry chains in every entry, beside ordinary entries."""
     la = [192, 1000, 4096, 30000, 65536, 777, 2048]
