@@ -109,8 +109,9 @@ typedef struct ntt_conv_plan {
     uint32_t value_bits_a;
     uint32_t value_bits_b;
     uint32_t valid;
-    uint32_t row_tail;   /* transform row of N + row_tail * N/4 points (0..3) */
-    uint32_t reserved;
+    uint32_t row_tail;     /* transform row of N + row_tail * N/4 points (0..3) */
+    uint32_t split_blocks; /* unbalanced split: blocks of the longer operand (0: none) */
+    uint64_t block_len;    /* coefficients of the longer operand per block */
 } ntt_conv_plan;
 
 /* ConvPlan::new, src/convolve.rs:75-113. false <=> None: an empty operand,

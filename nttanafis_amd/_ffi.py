@@ -48,7 +48,8 @@ class ConvPlanC(ctypes.Structure):
         ("value_bits_b", u32),
         ("valid", u32),
         ("row_tail", u32),
-        ("reserved", u32),
+        ("split_blocks", u32),
+        ("block_len", u64),
     ]
 
 

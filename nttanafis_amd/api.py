@@ -197,6 +197,11 @@ class ConvPlan:
         return self._c.row_tail
 
     @property
+    def split_blocks(self) -> int:
+        """Blocks of the longer operand in an unbalanced split (0: none)."""
+        return self._c.split_blocks
+
+    @property
     def schoolbook(self) -> bool:
         return bool(self._c.schoolbook)
 

@@ -308,11 +308,33 @@ __host__ __device__ __forceinline__ void garner_r(const CrtArgs& a, const double
     }
 }
 
+// Residues of coefficient i: one row entry per prime, or for a split
+// operation the sum of the entries of the (at most two) blocks covering i:
+// block z1 = i / D (the last block for i past it) at i - z1*D, and block
+// z1 - 1 at i - z1*D + D while that lies in its row.
 template <int NP>
 __host__ __device__ __forceinline__ void garner(const CrtArgs& a, uint64_t i, uint64_t* X, bool balanced) {
     double r[NP];
+    if (a.split_blocks == 0) {
 #pragma unroll
-    for (int k = 0; k < NP; ++k) r[k] = a.residues[(uint64_t)k * a.row_stride + i];
+        for (int k = 0; k < NP; ++k) r[k] = a.residues[(uint64_t)k * a.row_stride + i];
+    } else {
+        const uint64_t D = a.split_digits;
+        uint64_t z1 = i / D;
+        if (z1 >= a.split_blocks) z1 = a.split_blocks - 1;
+        const uint64_t k1 = i - z1 * D;
+        const bool two = z1 > 0 && k1 + D < a.row_stride;
+        const double* r1 = a.residues + z1 * a.block_stride + k1;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            double v = r1[(uint64_t)k * a.row_stride];
+            if (two) {
+                v += r1[(uint64_t)k * a.row_stride - a.block_stride + D];
+                if (v >= a.field.p[k]) v -= a.field.p[k];
+            }
+            r[k] = v;
+        }
+    }
     garner_r<NP>(a, r, X, balanced);
 }
 

@@ -978,6 +978,8 @@ bool conv_geometry(const ntt_conv_plan* p, Geometry* g) {
     g->conv_len = p->len_a + p->len_b - 1;
     g->log_t = (int)p->log_len;
     g->tail = p->row_tail;
+    g->blocks = p->split_blocks;
+    g->block_digits = p->block_len;
     g->bound_bits = p->bound_bits;
     g->disjoint = true;
     g->is_signed = p->is_signed != 0;
@@ -1689,7 +1691,13 @@ bool ntt_conv_plan_new(size_t len_a, size_t len_b, uint64_t max_a, uint64_t max_
     Geometry row;
     row.primes = (uint32_t)np;
     row.conv_len = conv;
+    row.digits_a = len_a;
+    row.digits_b = len_b;
     choose_row(&row, std::max<uint64_t>(len_a, len_b), K);
+    // Unbalanced convolutions split like products (the reference adds the
+    // disjoint blocks modulo their width; here the residues of the blocks
+    // covering a coefficient are summed before the CRT).
+    choose_split(&row, kLaunchPrice);
     ntt_conv_plan p;
     memset(&p, 0, sizeof p);
     p.len_a = len_a;
@@ -1703,6 +1711,8 @@ bool ntt_conv_plan_new(size_t len_a, size_t len_b, uint64_t max_a, uint64_t max_
     p.primes = (uint32_t)np;
     p.log_len = (uint32_t)row.log_t;
     p.row_tail = row.tail;
+    p.split_blocks = row.blocks;
+    p.block_len = row.block_digits;
     p.bound_bits = (uint32_t)bits;
     p.value_bits_a = (uint32_t)vba;
     p.value_bits_b = (uint32_t)vbb;

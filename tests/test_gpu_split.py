@@ -1,4 +1,5 @@
-"""Unbalanced products split into blocks of the longer operand.
+"""Unbalanced products and convolutions split into blocks of the longer
+operand.
 
 The longer operand runs in blocks of D coefficients (D at least the shorter
 operand's count, a multiple of 64), every block multiplied on its own row by
@@ -140,3 +141,35 @@ def test_split_large_rows():
     assert mod_q(got) == mod_q(a) * mod_q(b) % Q61
     with split(0):
         assert (product(a, b) == got).all()
+
+
+@pytest.mark.parametrize("la,lb,bits,signed", [(200000, 3000, 16, False), (3000, 200000, 16, False),
+                                               (100000, 2000, 32, True), (65536, 100, 64, False),
+                                               (50000, 777, 50, True)])
+def test_split_convolutions(oracle, la, lb, bits, signed):
+    """ConvPlan::convolve with the longer operand split (the reference adds
+    disjoint blocks modulo their width, src/multiply/execute.rs:370-411):
+    signed and unsigned coefficients up to 64 bits, either operand longer."""
+    rng = np.random.default_rng(la + lb + bits)
+    if signed:
+        lim = 1 << (bits - 1)
+        a = rng.integers(-lim + 1, lim, la, dtype=np.int64).astype(np.uint64)
+        b = rng.integers(-lim + 1, lim, lb, dtype=np.int64).astype(np.uint64)
+        mx = lim - 1
+    else:
+        mx = (1 << bits) - 1
+        a = rng.integers(0, mx, la, dtype=np.uint64, endpoint=True)
+        b = rng.integers(0, mx, lb, dtype=np.uint64, endpoint=True)
+    with split(1):
+        plan = ntt.ConvPlan.new(la, lb, mx, mx, signed)
+        say(f"conv {la}x{lb} {bits}-bit signed={signed}: {plan.split_blocks} blocks, rows 2^{plan.log_len}")
+        assert plan.split_blocks >= 2
+        dst = np.zeros(plan.output_len(), dtype=np.uint64)
+        assert plan.convolve(dst, a, b), ntt.last_error()
+    assert (dst == oracle.convolve(a, b, mx, mx, signed, threads=8)).all()
+    with split(0):
+        whole = ntt.ConvPlan.new(la, lb, mx, mx, signed)
+        assert whole.split_blocks == 0
+        dst2 = np.zeros(whole.output_len(), dtype=np.uint64)
+        assert whole.convolve(dst2, a, b), ntt.last_error()
+    assert (dst2 == dst).all()
