@@ -339,6 +339,65 @@ void ntt_profile_enable(int on);
 int ntt_profile_read(char *names, size_t names_len, double *ms, uint64_t *counts, int cap);
 void ntt_profile_reset(void);
 
+/* ---------------------------------------------------------------------------
+ * Prime fields and two-prime CRT: the public scalar API of the reference
+ * (src/field/prime_field.rs:91-442). Host functions (no GPU). A residue is a
+ * canonical uint64_t in [0, p); a field's operations expect its own residues.
+ */
+typedef struct ntt_prime_field {
+    uint64_t modulus;  /* prime, 2^49 < modulus < 2^50 */
+    double reciprocal; /* RN(1/modulus) */
+} ntt_prime_field;
+
+/* PrimeField::new (prime_field.rs:99-116): false unless modulus is a prime
+ * with 2^49 < modulus < 2^50. */
+bool ntt_field_new(uint64_t modulus, ntt_prime_field *out);
+/* PrimeField::below (118-129): the largest prime below bound and 2^50,
+ * false when none exceeds 2^49. */
+bool ntt_field_below(uint64_t bound, ntt_prime_field *out);
+/* PrimeField::catalog (131-137): the eight transform primes in
+ * reconstruction order; writes min(cap, 8), returns 8. */
+size_t ntt_field_catalog(ntt_prime_field *out, size_t cap);
+/* PrimeField::residue (147-153): false unless value < modulus. */
+bool ntt_field_residue(const ntt_prime_field *f, uint64_t value, uint64_t *out);
+/* PrimeField::reduce, reduce_u64, signed (158-199): canonical residue of a
+ * signed or unsigned integer; the centered representative in
+ * -(p-1)/2..=(p-1)/2. */
+uint64_t ntt_field_reduce(const ntt_prime_field *f, int64_t value);
+uint64_t ntt_field_reduce_u64(const ntt_prime_field *f, uint64_t value);
+int64_t ntt_field_signed(const ntt_prime_field *f, uint64_t residue);
+/* PrimeField::add, sub, neg, mul, pow (205-265). */
+uint64_t ntt_field_add(const ntt_prime_field *f, uint64_t a, uint64_t b);
+uint64_t ntt_field_sub(const ntt_prime_field *f, uint64_t a, uint64_t b);
+uint64_t ntt_field_neg(const ntt_prime_field *f, uint64_t a);
+uint64_t ntt_field_mul(const ntt_prime_field *f, uint64_t a, uint64_t b);
+uint64_t ntt_field_pow(const ntt_prime_field *f, uint64_t base, uint64_t exponent);
+/* PrimeField::inv (271-290): false for zero. */
+bool ntt_field_inv(const ntt_prime_field *f, uint64_t a, uint64_t *out);
+/* PrimeField::mul_sub, scale, evaluate (296-320): target[i] -= source[i] *
+ * factor; target[i] *= factor; Horner evaluation of coefficients (constant
+ * term first) at point. */
+void ntt_field_mul_sub(const ntt_prime_field *f, uint64_t *target, const uint64_t *source, size_t n,
+                       uint64_t factor);
+void ntt_field_scale(const ntt_prime_field *f, uint64_t *target, size_t n, uint64_t factor);
+uint64_t ntt_field_evaluate(const ntt_prime_field *f, const uint64_t *coefficients, size_t n,
+                            uint64_t point);
+
+typedef struct ntt_crt_pair {
+    ntt_prime_field first;
+    ntt_prime_field second;
+    uint64_t inverse; /* first modulus^-1 in the second field */
+} ntt_crt_pair;
+
+/* CrtPair::new (404-416): false when the fields coincide. */
+bool ntt_crt_pair_new(const ntt_prime_field *first, const ntt_prime_field *second, ntt_crt_pair *out);
+/* CrtPair::modulus (418-422): the 128-bit product of the moduli as two words. */
+void ntt_crt_pair_modulus(const ntt_crt_pair *pair, uint64_t *lo, uint64_t *hi);
+/* CrtPair::combine (424-441): the integer in -(M-1)/2..=(M-1)/2 congruent to
+ * both residues, as a 128-bit two's complement value (lo word, signed hi word). */
+void ntt_crt_pair_combine(const ntt_crt_pair *pair, uint64_t first, uint64_t second, uint64_t *lo,
+                          int64_t *hi);
+
 /* Unbalanced split the engine chose for a product shape
  * (src/profile/planner.rs:283-393): the longer operand runs in `blocks`
  * blocks of block_digits coefficients, each on a row of 2^log_len points

@@ -24,9 +24,12 @@ from .api import (  # noqa: F401
     synchronize,
 )
 from ._ffi import LIB_PATH, load  # noqa: F401
+from .field import CrtPair, PrimeField  # noqa: F401
 
 __all__ = [
     "Batch",
+    "CrtPair",
+    "PrimeField",
     "MulPlan",
     "MulWorkspace",
     "ConvPlan",

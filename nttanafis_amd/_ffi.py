@@ -85,7 +85,37 @@ REQ_MUL, REQ_SQR, REQ_CONV = 0, 1, 2
 REQ_SIGNED, REQ_DEVICE = 1, 2
 
 # name -> (restype, argtypes)
+class PrimeFieldC(ctypes.Structure):
+    _fields_ = [("modulus", u64), ("reciprocal", ctypes.c_double)]
+
+
+class CrtPairC(ctypes.Structure):
+    _fields_ = [("first", PrimeFieldC), ("second", PrimeFieldC), ("inverse", u64)]
+
+
+_PF = ctypes.POINTER(PrimeFieldC)
+_PC = ctypes.POINTER(CrtPairC)
+
 SIGNATURES = {
+    "ntt_field_new": (ctypes.c_bool, [u64, _PF]),
+    "ntt_field_below": (ctypes.c_bool, [u64, _PF]),
+    "ntt_field_catalog": (size_t, [_PF, size_t]),
+    "ntt_field_residue": (ctypes.c_bool, [_PF, u64, P64]),
+    "ntt_field_reduce": (u64, [_PF, ctypes.c_int64]),
+    "ntt_field_reduce_u64": (u64, [_PF, u64]),
+    "ntt_field_signed": (ctypes.c_int64, [_PF, u64]),
+    "ntt_field_add": (u64, [_PF, u64, u64]),
+    "ntt_field_sub": (u64, [_PF, u64, u64]),
+    "ntt_field_neg": (u64, [_PF, u64]),
+    "ntt_field_mul": (u64, [_PF, u64, u64]),
+    "ntt_field_pow": (u64, [_PF, u64, u64]),
+    "ntt_field_inv": (ctypes.c_bool, [_PF, u64, P64]),
+    "ntt_field_mul_sub": (None, [_PF, P64, P64, size_t, u64]),
+    "ntt_field_scale": (None, [_PF, P64, size_t, u64]),
+    "ntt_field_evaluate": (u64, [_PF, P64, size_t, u64]),
+    "ntt_crt_pair_new": (ctypes.c_bool, [_PF, _PF, _PC]),
+    "ntt_crt_pair_modulus": (None, [_PC, P64, P64]),
+    "ntt_crt_pair_combine": (None, [_PC, u64, u64, P64, ctypes.POINTER(ctypes.c_int64)]),
     "ntt_mul_plan_new": (ctypes.c_bool, [size_t, size_t, ctypes.POINTER(MulPlanC)]),
     "ntt_mul_plan_scratch_len": (size_t, [ctypes.POINTER(MulPlanC)]),
     "ntt_mul_plan_sqr_scratch_len": (size_t, [ctypes.POINTER(MulPlanC)]),
