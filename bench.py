@@ -30,6 +30,7 @@ from nttanafis_amd.workloads import (CONV, SQR, batch_operands, batch_shapes, op
                                      output_len, request_operands, served_stream)
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_G = 1024 * 2.4e9 / 4 / 1e9  # wave64 FP64 instructions per second, G
 
 
 def splitmix(length: int, seed: int) -> np.ndarray:
@@ -113,6 +114,20 @@ def step_bytes(name: str, primes: int, log_len: int, tail: int, la: int, lb: int
     if name == "crt_disjoint":
         total = 8.0 * primes * (la + lb - 1) + 8.0 * out_len
     return total
+
+
+def measured_valu(workload: str) -> dict:
+    """VALU wave-instructions per launch of each kernel kind (SQ_INSTS_VALU
+    from the committed rocprofv3 counter summary of this workload), lines of
+    the form `VALU <kind> <instructions>`; empty when none are committed."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_counters.txt")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "VALU":
+                out[parts[1]] = int(float(parts[2]))
+    return out
 
 
 def measured_traffic(workload: str) -> dict:
@@ -530,6 +545,7 @@ def main() -> int:
             dominant, dom_total = name, ms
     traffic = measured_traffic(args.workload)
     roofline = None
+    result_valu = None
     if dominant:
         ms, count = prof[dominant]
         per_step = count // args.steps
@@ -542,6 +558,17 @@ def main() -> int:
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic.get(dominant) if per_step == 1 else None,
                     "algorithmic_bytes": nbytes, "launches_per_step": per_step}
+        # The FP64 vector issue rate of the same kernel: its VALU instructions
+        # per launch (committed counters) over the live launch duration,
+        # against 1024 SIMDs issuing one wave64 FP64 instruction per 4 cycles
+        # at 2.4 GHz (78.6 TFLOP/s of FMA).
+        insts = measured_valu(args.workload).get(dominant)
+        if insts and per_step == 1:
+            rate = insts / (prof[dominant][0] / prof[dominant][1] * 1e-3) / 1e9
+            result_valu = {"kernel": dominant, "insts_per_launch": insts, "achieved": round(rate, 1),
+                           "peak": VALU_PEAK_G, "unit": "G wave-instr/s", "frac": round(rate / VALU_PEAK_G, 4)}
+        else:
+            result_valu = None
 
     unit = "Mcoeff/s" if kind == "conv" else "Mlimbs/s"
     value = la / (ms_clean * 1e-3) / 1e6
@@ -563,6 +590,7 @@ def main() -> int:
                    "chunk_bits": chunk, "transform_len": T, "cyclic_len": 1 << log_len},
         "kernels": per_kernel,
         "roofline": roofline,
+        "valu": result_valu,
     }
     if kind == "mul" and not args.no_host:
         result["host_buffers"] = bench_host_path(a, b, max(3, args.steps // 5))

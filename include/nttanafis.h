@@ -43,7 +43,8 @@ typedef struct ntt_mul_plan {
     uint64_t len_b;
     uint32_t primes;     /* catalog prefix of the device geometry */
     uint32_t chunk_bits; /* bits per coefficient */
-    uint32_t log_len;    /* log2 of the cyclic transform length N */
+    uint32_t log_len;    /* log2 of the cyclic transform length N (of one block's row when the
+                            product is split, see ntt_mul_split) */
     uint32_t valid;      /* 1 for a plan from ntt_mul_plan_new */
     uint32_t row_tail;   /* transform row of N + row_tail * N/4 points (0..3) */
     uint32_t reserved;

@@ -68,7 +68,7 @@ struct RowPart {
 int row_parts(const Geometry& g, RowPart* parts);
 
 // batched: geometry of a product inside a batch, which shares launches
-// among its products and runs power-of-two rows only.
+// among its products and runs power-of-two rows only (whole or split).
 bool select_product(uint64_t la, uint64_t lb, Geometry* out, bool batched = false);
 bool conv_bound_bits(uint64_t terms, uint64_t max_a, uint64_t max_b, uint64_t* bits);
 bool product_geometry(const ntt_mul_plan* plan, Geometry* g);
