@@ -164,3 +164,23 @@ def test_composite_workspace_and_plan_agree(oracle):
         dst = np.zeros(len(a) + len(b), dtype=np.uint64)
         assert ws.mul(dst, a, b), ntt.last_error()
     assert (dst == oracle.mul(a, b, threads=8)).all()
+
+
+# Row-shape boundaries through 16-bit convolutions (one coefficient per
+# element, so the convolution length is la + lb - 1): just above N, exactly
+# 5N/4, 3N/2 and 7N/4, and an operand of exactly N coefficients (N = 8192).
+BOUNDARIES = [(4097, 4097, 1), (5121, 5120, 1), (5121, 5121, 2), (6145, 6144, 2), (6145, 6145, 3),
+              (7169, 7168, 3), (8192, 1000, 1), (8192, 3073, 2)]
+
+
+@pytest.mark.parametrize("la,lb,tail", BOUNDARIES)
+def test_composite_row_boundaries(oracle, la, lb, tail):
+    rng = np.random.default_rng(la * 7 + lb)
+    a = rng.integers(0, 65535, la, dtype=np.uint64, endpoint=True)
+    b = rng.integers(0, 65535, lb, dtype=np.uint64, endpoint=True)
+    with rows(True):
+        plan = ntt.ConvPlan.new(la, lb, 65535, 65535, False)
+        assert (plan.log_len, plan.row_tail) == (13, tail), (plan.log_len, plan.row_tail)
+        dst = np.zeros(plan.output_len(), dtype=np.uint64)
+        assert plan.convolve(dst, a, b), ntt.last_error()
+    assert (dst == oracle.convolve(a, b, 65535, 65535, False, threads=8)).all()

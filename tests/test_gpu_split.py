@@ -173,3 +173,22 @@ def test_split_convolutions(oracle, la, lb, bits, signed):
         dst2 = np.zeros(whole.output_len(), dtype=np.uint64)
         assert whole.convolve(dst2, a, b), ntt.last_error()
     assert (dst2 == dst).all()
+
+
+# Split boundaries through 16-bit convolutions: a last block of one
+# coefficient, blocks exactly as long as the shorter operand, exactly two
+# blocks.
+@pytest.mark.parametrize("la,lb", [(3073, 1000), (5000, 1024), (2048, 1024), (4097, 64), (1000, 3073)])
+def test_split_boundaries(oracle, la, lb):
+    rng = np.random.default_rng(la * 3 + lb)
+    a = rng.integers(0, 65535, la, dtype=np.uint64, endpoint=True)
+    b = rng.integers(0, 65535, lb, dtype=np.uint64, endpoint=True)
+    with split(1):
+        plan = ntt.ConvPlan.new(la, lb, 65535, 65535, False)
+        blocks, d = plan.split_blocks, plan._c.block_len
+        dl, ds = max(la, lb), min(la, lb)
+        say(f"conv {la}x{lb}: {blocks} blocks of {d}, last block {dl - (blocks - 1) * d}")
+        assert blocks >= 2 and d >= ds and (blocks - 1) * d < dl <= blocks * d
+        dst = np.zeros(plan.output_len(), dtype=np.uint64)
+        assert plan.convolve(dst, a, b), ntt.last_error()
+    assert (dst == oracle.convolve(a, b, 65535, 65535, False, threads=8)).all()
