@@ -26,7 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import nttanafis_amd as ntt  # noqa: E402
-from nttanafis_amd.workloads import (CONV, SQR, batch_operands, batch_shapes, operand_pools,  # noqa: E402
+from nttanafis_amd.workloads import (CONV, SQR, batch_operands, batch_shapes, operand, operand_pools,  # noqa: E402
                                      output_len, request_operands, served_stream)
 
 HBM_PEAK_GBS = 8000.0
@@ -192,6 +192,54 @@ def cpu_baseline(wl, sample_seconds: float, threads: int = 1):
 def mod_q61(limbs) -> int:
     q = (1 << 61) - 1
     return int.from_bytes(np.ascontiguousarray(limbs, dtype="<u8").tobytes(), "little") % q
+
+
+SMALL_SHAPES = [(64, 64), (192, 192), (384, 384), (512, 512), (8, 200000), (192, 65536)]
+
+
+def bench_small(reps: int = 200) -> dict:
+    """Per-call latency of single small and very unequal products through the
+    device path, each shape run as the engine chooses, as a schoolbook
+    (NTTAMD_SCHOOL=1) and through transforms (NTTAMD_SCHOOL=0): reps calls
+    queued back to back, then reps calls each waited for. Both forms must
+    give the same limbs."""
+    rows = []
+    saved = os.environ.get("NTTAMD_SCHOOL")
+    try:
+        for la, lb in SMALL_SHAPES:
+            a, b = operand(la, 1), operand(lb, 2)
+            da, db = ntt.DeviceBuffer.from_host(a), ntt.DeviceBuffer.from_host(b)
+            dout = ntt.DeviceBuffer(8 * (la + lb))
+            os.environ.pop("NTTAMD_SCHOOL", None)
+            row = {"shape": [la, lb], "default": "schoolbook" if ntt.mul_schoolbook(la, lb) else "transforms"}
+            results = []
+            for mode, name in (("1", "schoolbook"), ("0", "transforms")):
+                os.environ["NTTAMD_SCHOOL"] = mode
+                ws = ntt.MulWorkspace()
+                ws.prepare(ntt.MulPlan.new(la, lb))
+                for _ in range(5):
+                    assert ws.mul_device(dout.ptr, la + lb, da.ptr, la, db.ptr, lb), ntt.last_error()
+                assert ntt.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    ws.mul_device(dout.ptr, la + lb, da.ptr, la, db.ptr, lb)
+                assert ntt.synchronize()
+                queued = (time.perf_counter() - t0) / reps
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    ws.mul_device(dout.ptr, la + lb, da.ptr, la, db.ptr, lb)
+                    ntt.synchronize()
+                waited = (time.perf_counter() - t0) / reps
+                row[name] = {"us_per_call": round(queued * 1e6, 1), "us_waited": round(waited * 1e6, 1)}
+                results.append(dout.download(la + lb))
+            assert (results[0] == results[1]).all(), f"schoolbook and transform products differ at {la}x{lb}"
+            rows.append(row)
+    finally:
+        if saved is None:
+            os.environ.pop("NTTAMD_SCHOOL", None)
+        else:
+            os.environ["NTTAMD_SCHOOL"] = saved
+    return {"reps": reps, "shapes": rows}
 
 
 def bench_batch(steps: int, warmup: int, plain_steps: int) -> dict:
@@ -420,6 +468,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-small", action="store_true", help="skip the small-product latency table")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer API timing")
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--no-serve", action="store_true")
@@ -445,6 +494,16 @@ def main() -> int:
                           "ms_per_step": res["served"]["p99_ms"], "higher_is_better": True, "scaling": "weak",
                           "vs_baseline": None, "dtype": "f64", "data": "synthetic (SplitMix64 limbs)",
                           "config": {"workload": "served"}, "served": res}), flush=True)
+        return 0
+    if args.workload == "small":
+        res = bench_small()
+        rows = res["shapes"]
+        print(json.dumps({"metric": "per-call latency of small products", "value": rows[1]["schoolbook"]["us_per_call"],
+                          "unit": "us per call (192x192)", "n_gpus": 1, "steps": res["reps"], "warmup": 5,
+                          "ms_per_step": rows[1]["schoolbook"]["us_per_call"] / 1000, "higher_is_better": False,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                          "data": "synthetic (SplitMix64 limbs)", "config": {"workload": "small"}, "small": res}),
+              flush=True)
         return 0
     if args.workload == "batch4096":
         res = bench_batch(args.steps, args.warmup, max(1, args.steps // 5))
@@ -594,6 +653,8 @@ def main() -> int:
     }
     if kind == "mul" and not args.no_host:
         result["host_buffers"] = bench_host_path(a, b, max(3, args.steps // 5))
+    if not args.no_small:
+        result["small"] = bench_small()
     if not args.no_batch:
         result["batched"] = bench_batch(args.batch_steps, 2, 2)
     if not args.no_serve:

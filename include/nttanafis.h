@@ -47,7 +47,7 @@ typedef struct ntt_mul_plan {
                             product is split, see ntt_mul_split) */
     uint32_t valid;      /* 1 for a plan from ntt_mul_plan_new */
     uint32_t row_tail;   /* transform row of N + row_tail * N/4 points (0..3) */
-    uint32_t reserved;
+    uint32_t schoolbook; /* 1: the product runs as a schoolbook (ntt_mul_schoolbook) */
 } ntt_mul_plan;
 
 /* MulPlan::new, src/plan.rs:53-69. false <=> None: a zero length, an
@@ -406,6 +406,15 @@ void ntt_crt_pair_combine(const ntt_crt_pair *pair, uint64_t first, uint64_t sec
  * for a product that is not split. NTTAMD_SPLIT=0 disables splits, =1 takes
  * the cheapest admissible one (tests); results are the same limbs. */
 bool ntt_mul_split(size_t len_a, size_t len_b, uint32_t *blocks, uint64_t *block_digits);
+
+/* 1 when the engine runs a product of this shape as a schoolbook instead
+ * of transforms: direct sums of the 128-bit limb products, one launch of
+ * 1024-limb output blocks plus the carry pass, for products whose shorter
+ * operand has at most 512 limbs and whose transforms would cost more (the
+ * basecase below the transform crossover that README.md:6-8 leaves to
+ * callers). NTTAMD_SCHOOL=0 disables it, =1 takes it wherever admissible
+ * (tests); results are the same limbs. 0 for an unsupported shape. */
+int ntt_mul_schoolbook(size_t len_a, size_t len_b);
 
 /* Row shape of src/profile/planner.rs:565-591 (row_shape): the shortest
  * row of 2^k, 5N/4, 3N/2 or 7N/4 points (N = 2^(k-1) holding both operands,

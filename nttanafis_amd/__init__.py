@@ -18,6 +18,7 @@ from .api import (  # noqa: F401
     pinned_array,
     last_error,
     mul_geometry,
+    mul_schoolbook,
     mul_split,
     row_len,
     row_shape,
@@ -42,6 +43,7 @@ __all__ = [
     "device_available",
     "last_error",
     "mul_geometry",
+    "mul_schoolbook",
     "mul_split",
     "row_len",
     "row_shape",

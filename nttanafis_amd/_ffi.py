@@ -27,7 +27,7 @@ class MulPlanC(ctypes.Structure):
         ("log_len", u32),
         ("valid", u32),
         ("row_tail", u32),
-        ("reserved", u32),
+        ("schoolbook", u32),
     ]
 
 
@@ -205,6 +205,7 @@ SIGNATURES = {
     "ntt_profile_reset": (None, []),
     "ntt_row_shape": (ctypes.c_bool, [u64, u64, P64, P64, P64]),
     "ntt_mul_split": (ctypes.c_bool, [size_t, size_t, ctypes.POINTER(u32), P64]),
+    "ntt_mul_schoolbook": (ctypes.c_int, [size_t, size_t]),
     "ntt_mul_geometry": (
         ctypes.c_bool,
         [size_t, size_t, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)],

@@ -83,6 +83,11 @@ class MulPlan:
         """Transform row of N + row_tail*N/4 points, N = 2^log_len."""
         return self._c.row_tail
 
+    @property
+    def schoolbook(self) -> bool:
+        """The product runs as a schoolbook (mul_schoolbook)."""
+        return bool(self._c.schoolbook)
+
     def scratch_len(self) -> int:
         return _lib().ntt_mul_plan_scratch_len(ctypes.byref(self._c))
 
@@ -523,6 +528,12 @@ def mul_split(len_a: int, len_b: int):
     if not _lib().ntt_mul_split(len_a, len_b, ctypes.byref(n), ctypes.byref(d)):
         return None
     return n.value, d.value
+
+
+def mul_schoolbook(len_a: int, len_b: int) -> bool:
+    """True when the engine runs a product of this shape as a schoolbook
+    (direct sums of limb products) instead of transforms."""
+    return bool(_lib().ntt_mul_schoolbook(len_a, len_b))
 
 
 def row_len(log_len: int, row_tail: int) -> int:

@@ -179,6 +179,11 @@ struct CrtArgs {
     // Batched reconstruction: items[blockIdx.x] names product and block.
     const BatchProduct* batch;
     const BatchItem* items;
+    // Schoolbook products: the operand limbs (single operation; a batch
+    // reads them from src_a and src_b of each product).
+    const uint64_t* op_a;
+    const uint64_t* op_b;
+    uint64_t len_a, len_b;
 };
 
 // Root-table generation arguments.
@@ -215,5 +220,9 @@ constexpr uint32_t kInf = 0xFFFFFFFFu;
 constexpr int kCrtThreads = 256;
 constexpr int kCrtLimbsPerThread = 4;
 constexpr int kCrtBlockLimbs = kCrtThreads * kCrtLimbsPerThread;
+// Schoolbook products: the shorter operand has at most kSchoolMaxShort limbs;
+// carry blocks of kSchoolBlockLimbs output limbs.
+constexpr int kSchoolMaxShort = 512;
+constexpr int kSchoolBlockLimbs = 64;
 
 }  // namespace nttamd

@@ -47,6 +47,9 @@ struct Geometry {
     uint32_t blocks = 0;
     uint64_t block_digits = 0;
     uint32_t bound_bits = 0;  // bits of every reconstructed coefficient
+    // Schoolbook product (no transform; the fields above still describe
+    // the transform geometry the product would otherwise run).
+    bool school = false;
     bool disjoint = false;
     bool is_signed = false;
     uint32_t stride = 0;  // limbs per result coefficient (convolution)
@@ -147,7 +150,10 @@ struct OpBuffers {
 // operand b (the shorter operand of a split).
 uint64_t op_spectra_bytes(const Geometry& g);
 uint64_t op_spectra_bytes_b(const Geometry& g);
-uint64_t op_carry_bytes(uint64_t out_len);
+// Carry blocks of an output (kSchoolBlockLimbs limbs for a schoolbook
+// product, kCrtBlockLimbs otherwise) and their bookkeeping bytes.
+uint64_t carry_blocks(uint64_t out_len, bool school = false);
+uint64_t op_carry_bytes(uint64_t out_len, bool school = false);
 
 // Whole operations on device operands, queued on stream s.
 bool product_on(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint64_t la, const uint64_t* d_b,
@@ -166,7 +172,7 @@ struct Split {
 Split split_of(int K);
 
 struct BatchLaunch {
-    enum Kind { kFwdStrided, kFused, kInvStrided, kCrt } kind;
+    enum Kind { kFwdStrided, kFused, kInvStrided, kCrt, kSchool } kind;
     int logl;
     bool flag;  // pack (forward strided), canonical output (inverse strided), square (fused)
     bool pack;  // fused single pass: pack the operands
@@ -192,6 +198,8 @@ struct Batch {
     std::vector<Geometry> geo;
     std::vector<BatchProduct> host;
     std::vector<BatchBucket> buckets;
+    // Reconstruction items per prime count; index 0 holds the blocks of the
+    // schoolbook products.
     uint32_t crt_begin[kMaxPrimes + 1] = {0}, crt_count[kMaxPrimes + 1] = {0};
     DevBuf products, items, arena, carry;
     std::vector<BatchLaunch> program;

@@ -25,6 +25,7 @@ bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, hip
 bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, hipStream_t s);
 bool launch_crt_product(const CrtArgs& a, hipStream_t s);
 bool launch_crt_disjoint(const CrtArgs& a, hipStream_t s);
+bool launch_school(const CrtArgs& a, hipStream_t s);
 bool launch_recombine(const RecombineArgs& a, int primes, hipStream_t s);
 bool launch_conv_columns(const uint64_t* a, uint64_t la, const uint64_t* b, uint64_t lb, uint64_t* out,
                          uint32_t stride, uint32_t is_signed, hipStream_t s);
@@ -208,6 +209,10 @@ static double transform_price(int np, int log_n) {
     return np * (double)((uint64_t)1 << log_n) * (8.0 * passes_of(log_n) + log_n);
 }
 constexpr double kLaunchPrice = 5e6;
+// Price of one 64 x 64-bit limb product of a schoolbook block on one CU,
+// in the units of transform_price (the whole device).
+constexpr double kSchoolProductPrice = 310.0;
+constexpr int kDeviceCUs = 256;
 
 int row_parts(const Geometry& g, RowPart* parts) {
     const uint64_t n = (uint64_t)1 << g.log_t;
@@ -321,6 +326,28 @@ static void choose_split(Geometry* g, double launch_price) {
     if (found) *g = pick;
 }
 
+// Schoolbook or transforms. A schoolbook block costs its share of the
+// la*lb limb products on one CU and needs no transform launches; it is
+// admissible while the shorter operand fits the kernel's LDS window.
+// NTTAMD_SCHOOL=0|1 forces the choice (tests).
+static bool choose_school(uint64_t la, uint64_t lb, const Geometry& g, double ntt_price, double launch_price) {
+    if (std::min(la, lb) > (uint64_t)kSchoolMaxShort) return false;
+    const char* env = getenv("NTTAMD_SCHOOL");
+    if (env && env[0] == '0') return false;
+    if (env && env[0] == '1') return true;
+    (void)g;
+    // Blocks run one per CU: a single operation waits for its waves of
+    // blocks; a batch shares the device, so it pays its share of all CUs.
+    const uint64_t blocks = carry_blocks(la + lb, true);
+    const double products = (double)la * (double)lb;
+    const double price = launch_price == 0.0
+                             ? kSchoolProductPrice * products / kDeviceCUs
+                             : kSchoolProductPrice * products / (double)blocks *
+                                       (double)((blocks + kDeviceCUs - 1) / kDeviceCUs) +
+                                   launch_price;
+    return price < ntt_price;
+}
+
 // Product geometry: for every prefix, the widest coefficient whose products
 // fit its capacity (the narrowing of src/profile/planner.rs:168-183), priced
 // by device work: HBM passes and transform levels over the row's points.
@@ -380,8 +407,14 @@ bool select_product(uint64_t la, uint64_t lb, Geometry* out, bool batched) {
     // (same primes and coefficients, so the reconstruction is unchanged).
     // A batch runs power-of-two rows; both forms split unbalanced products.
     if (found) {
+        const double launch_price = batched ? 0.0 : kLaunchPrice;
         if (!batched) choose_row(out, std::max(out->digits_a, out->digits_b), out->log_t);
-        choose_split(out, batched ? 0.0 : kLaunchPrice);
+        choose_split(out, launch_price);
+        // Both forms end with the carry pass across output blocks; the
+        // transforms also launch the reconstruction.
+        const double ntt = (out->blocks ? split_price(*out, launch_price) : row_price(*out, launch_price)) +
+                           4.0 * out->primes * out->primes * (double)out->conv_len + launch_price;
+        out->school = choose_school(la, lb, *out, ntt, launch_price);
     }
     return found;
 }
@@ -634,10 +667,14 @@ static bool inverse_tail(const Split& sp, const PassArgs& a0, int primes, hipStr
 // (primes x row_len doubles each; spec_b unused for a square) and the carry
 // words of the reconstruction (5 per output block).
 uint64_t op_spectra_bytes(const Geometry& g) {
-    return (uint64_t)g.primes * row_len(g) * 8 * (g.blocks ? g.blocks : 1);
+    return g.school ? 0 : (uint64_t)g.primes * row_len(g) * 8 * (g.blocks ? g.blocks : 1);
 }
-uint64_t op_spectra_bytes_b(const Geometry& g) { return (uint64_t)g.primes * row_len(g) * 8; }
-uint64_t op_carry_bytes(uint64_t out_len) { return ((out_len + kCrtBlockLimbs - 1) / kCrtBlockLimbs) * 5 * 4; }
+uint64_t op_spectra_bytes_b(const Geometry& g) { return g.school ? 0 : (uint64_t)g.primes * row_len(g) * 8; }
+uint64_t carry_blocks(uint64_t out_len, bool school) {
+    const uint64_t bl = school ? kSchoolBlockLimbs : kCrtBlockLimbs;
+    return (out_len + bl - 1) / bl;
+}
+uint64_t op_carry_bytes(uint64_t out_len, bool school) { return carry_blocks(out_len, school) * 5 * 4; }
 
 // Root-table entries a geometry reads: T/2 for a row of 2^K points; a
 // composite row reaches nodes below 7N/8 of the table of 2N points.
@@ -812,8 +849,34 @@ static bool crt_product_from(hipStream_t s, const Geometry& g, const double* row
     return launch_crt_product(a, s);
 }
 
+// Schoolbook product of device operands: one launch of output blocks, then
+// the carries across blocks.
+static bool school_on(hipStream_t s, const uint64_t* d_a, uint64_t la, const uint64_t* d_b, uint64_t lb,
+                      uint32_t* carry, uint64_t* d_out) {
+    CrtArgs a;
+    memset(&a, 0, sizeof a);
+    a.out = d_out;
+    a.out_len = la + lb;
+    const uint64_t blocks = carry_blocks(la + lb, true);
+    a.blocks = (uint32_t)blocks;
+    a.agg_base = carry;
+    a.agg_thresh = carry + blocks;
+    a.first_thresh = carry + 2 * blocks;
+    a.ripple_end = carry + 3 * blocks;
+    a.carry_in = carry + 4 * blocks;
+    a.op_a = d_a;
+    a.op_b = d_b;
+    a.len_a = la;
+    a.len_b = lb;
+    return launch_school(a, s);
+}
+
 bool product_on(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint64_t la, const uint64_t* d_b,
                 uint64_t lb, bool square, const OpBuffers& buf, uint64_t* d_out) {
+    if (g.school) {
+        if (std::min(la, lb) > (uint64_t)kSchoolMaxShort) return false;
+        return school_on(s, d_a, la, square ? d_a : d_b, square ? la : lb, buf.carry, d_out);
+    }
     double* rows = nullptr;
     if (!residues(s, g, d_a, la, d_b, lb, square, buf, &rows)) return false;
     return crt_product_from(s, g, rows, la + lb, buf.carry, d_out);
@@ -837,8 +900,9 @@ bool convolution_on(hipStream_t s, const Geometry& g, const ntt_conv_plan* p, co
 
 // Executor forms: grow the executor's own buffers, then run on its stream.
 static bool exec_buffers(Executor& ex, const Geometry& g, bool square, uint64_t out_len, OpBuffers* buf) {
-    if (!ex.spec_a.ensure(op_spectra_bytes(g)) || (!square && !ex.spec_b.ensure(op_spectra_bytes_b(g))) ||
-        !ex.carry.ensure(op_carry_bytes(out_len))) {
+    if (!ex.spec_a.ensure(std::max<uint64_t>(op_spectra_bytes(g), 8)) ||
+        (!square && !ex.spec_b.ensure(std::max<uint64_t>(op_spectra_bytes_b(g), 8))) ||
+        !ex.carry.ensure(op_carry_bytes(out_len, g.school))) {
         set_error("device working memory allocation failed");
         return false;
     }
@@ -1047,7 +1111,8 @@ namespace nttamd {
 // Products are bucketed by transform length. Every pass of a bucket is one
 // launch over all (product, prime) items of the bucket, and the
 // reconstruction is one launch per prime count over all (product, block)
-// items. Pointers and shapes of the products live in a device array that the
+// items; products the pricing gives to the schoolbook skip the transforms
+// and take one schoolbook launch over their (product, block) items. Pointers and shapes of the products live in a device array that the
 // kernels read, so the captured graph serves any operands bound later. The
 // split of each bucket's transform into passes comes from an autotuner that
 // times the candidate splits on the batch's own data the first time a
@@ -1067,6 +1132,7 @@ static bool issue_launch(const BatchLaunch& L, hipStream_t s) {
         case BatchLaunch::kFused: return launch_fused(L.logl, L.pass, L.units, L.pack, L.flag, s);
         case BatchLaunch::kInvStrided: return launch_inv_strided(L.logl, L.pass, L.units, L.flag, s);
         case BatchLaunch::kCrt: return launch_crt_product(L.crt, s);
+        case BatchLaunch::kSchool: return launch_school(L.crt, s);
     }
     return false;
 }
@@ -1138,15 +1204,17 @@ static void build_program(Batch& B) {
     const BatchItem* I = B.items.as<BatchItem>();
     for (const BatchBucket& r : B.buckets)
         bucket_program(B.program, bucket_base(B, r.K), r.K, split_of(r.K), I, r);
-    for (int np = 1; np <= kMaxPrimes; ++np) {
+    for (int np = 0; np <= kMaxPrimes; ++np) {
         if (!B.crt_count[np]) continue;
         BatchLaunch L;
         memset(&L, 0, sizeof L);
-        L.kind = BatchLaunch::kCrt;
-        L.name = "batch_crt";
-        Geometry gN;
-        gN.primes = np;
-        L.crt = crt_args(gN, nullptr, nullptr, 0);
+        L.kind = np ? BatchLaunch::kCrt : BatchLaunch::kSchool;
+        L.name = np ? "batch_crt" : "batch_school";
+        if (np) {
+            Geometry gN;
+            gN.primes = np;
+            L.crt = crt_args(gN, nullptr, nullptr, 0);
+        }
         L.crt.batch = B.products.as<BatchProduct>();
         L.crt.items = I + B.crt_begin[np];
         L.crt.blocks = B.crt_count[np];
@@ -1188,7 +1256,7 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
         arena += op_spectra_bytes(g) / 8;
         off_b[i] = arena;
         if (!square) arena += op_spectra_bytes_b(g) / 8;
-        const uint64_t blocks = (B.la[i] + B.lb[i] + kCrtBlockLimbs - 1) / kCrtBlockLimbs;
+        const uint64_t blocks = carry_blocks(B.la[i] + B.lb[i], g.school);
         off_c[i] = carry;
         carry += 4 * blocks;
     }
@@ -1228,12 +1296,12 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
         P.width = g.width;
         P.width_magic = (uint32_t)(((1ull << 32) + g.width - 1) / g.width);
         P.value_bits = g.bound_bits;
-        P.blocks = (uint32_t)((P.out_len + kCrtBlockLimbs - 1) / kCrtBlockLimbs);
+        P.blocks = (uint32_t)carry_blocks(P.out_len, g.school);
         P.carry = C + off_c[i];
     }
     // Work items: per transform length every (product, prime) pair, then the
     // pairs of products with two operands and of squares; per prime count
-    // every (product, output block) pair.
+    // (0: the schoolbook products) every (product, output block) pair.
     std::vector<BatchItem> items;
     B.buckets.clear();
     // which: 0 first operands, 1 second operands (forward items); 2 products
@@ -1243,7 +1311,7 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
     // second-operand item per prime for its shorter operand.
     auto add = [&](int K, int which) {
         for (size_t i = 0; i < count; ++i) {
-            if (B.geo[i].log_t != K) continue;
+            if (B.geo[i].log_t != K || B.geo[i].school) continue;
             const bool sq = B.square[i] != 0;
             if ((which == 1 || which == 2) && sq) continue;
             if (which == 3 && !sq) continue;
@@ -1270,10 +1338,10 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
         r.n_sq = (uint32_t)items.size() - r.sq;
         B.buckets.push_back(r);
     }
-    for (int np = 1; np <= kMaxPrimes; ++np) {
+    for (int np = 0; np <= kMaxPrimes; ++np) {
         B.crt_begin[np] = (uint32_t)items.size();
         for (size_t i = 0; i < count; ++i)
-            if ((int)B.geo[i].primes == np)
+            if (B.geo[i].school ? np == 0 : (int)B.geo[i].primes == np)
                 for (uint32_t blk = 0; blk < B.host[i].blocks; ++blk) items.push_back({(uint32_t)i, blk});
         B.crt_count[np] = (uint32_t)items.size() - B.crt_begin[np];
     }
@@ -1582,6 +1650,7 @@ bool ntt_mul_plan_new(size_t len_a, size_t len_b, ntt_mul_plan* out) {
     p.chunk_bits = g.width;
     p.log_len = (uint32_t)g.log_t;
     p.row_tail = g.tail;
+    p.schoolbook = g.school ? 1 : 0;
     p.valid = 1;
     *out = p;
     return true;
@@ -1614,6 +1683,10 @@ void ntt_mul_workspace_prepare(ntt_mul_workspace* ws, const ntt_mul_plan* plan) 
     if (!select_product(plan->len_a, plan->len_b, &g)) return;
     std::lock_guard<std::mutex> guard(ws->ex.lock);
     if (!ws->ex.init()) return;
+    if (g.school) {
+        ws->ex.carry.ensure(op_carry_bytes(plan->len_a + plan->len_b, true));
+        return;
+    }
     const double *f, *i;
     uint64_t st;
     ensure_tables(table_need(g), ws->ex.stream, &f, &i, &st);
@@ -2188,6 +2261,7 @@ bool ntt_debug_residues(const uint64_t* a, size_t la, const uint64_t* b, size_t 
         set_error("ntt_debug_residues: the shape runs as a split product (set NTTAMD_SPLIT=0)");
         return false;
     }
+    g.school = false;  // the residues of the transform path, whatever a product of this shape runs
     const uint64_t T = row_len(g);
     if (geometry) {
         geometry[0] = g.primes;
@@ -2214,6 +2288,7 @@ bool ntt_debug_residues(const uint64_t* a, size_t la, const uint64_t* b, size_t 
 bool ntt_debug_crt(size_t la, size_t lb, const double* rows, uint64_t* out) {
     Geometry g;
     if (!select_product(la, lb, &g) || g.blocks) return false;
+    g.school = false;
     const uint64_t T = row_len(g);
     Executor* ex = plan_executor();
     std::lock_guard<std::mutex> guard(ex->lock);
@@ -2260,6 +2335,12 @@ bool ntt_mul_split(size_t len_a, size_t len_b, uint32_t* blocks, uint64_t* block
     if (blocks) *blocks = g.blocks;
     if (block_digits) *block_digits = g.block_digits;
     return true;
+}
+
+int ntt_mul_schoolbook(size_t len_a, size_t len_b) {
+    Geometry g;
+    if (len_a == 0 || len_b == 0 || !select_product(len_a, len_b, &g)) return 0;
+    return g.school ? 1 : 0;
 }
 
 }  // extern "C"

@@ -764,6 +764,8 @@ struct CrtJob {
     uint32_t *agg_base, *agg_thresh, *first_thresh, *ripple_end;
     uint32_t split_blocks;  // 0: no split
     uint64_t split_digits, block_stride;
+    const uint64_t *op_a, *op_b;  // schoolbook operands
+    uint64_t len_a, len_b;
 };
 
 __device__ __forceinline__ CrtJob crt_job(const CrtArgs& a) {
@@ -786,6 +788,10 @@ __device__ __forceinline__ CrtJob crt_job(const CrtArgs& a) {
         j.split_blocks = a.split_blocks;
         j.split_digits = a.split_digits;
         j.block_stride = a.block_stride;
+        j.op_a = a.op_a;
+        j.op_b = a.op_b;
+        j.len_a = a.len_a;
+        j.len_b = a.len_b;
         return j;
     }
     const BatchItem it = a.items[blockIdx.x];
@@ -807,7 +813,81 @@ __device__ __forceinline__ CrtJob crt_job(const CrtArgs& a) {
     j.agg_thresh = P.carry + P.blocks;
     j.first_thresh = P.carry + 2 * (uint64_t)P.blocks;
     j.ripple_end = P.carry + 3 * (uint64_t)P.blocks;
+    j.op_a = P.src_a.limbs;
+    j.op_b = P.src_b.limbs;
+    j.len_a = P.src_a.limb_len;
+    j.len_b = P.src_b.limb_len;
     return j;
+}
+
+// Second half of a reconstruction block: acc[j] holds limb j's sum mod 2^64
+// and wraps[j] the carries it sends on. Resolves the carries inside the
+// block, stores the limbs and publishes the block's carry function.
+__device__ __forceinline__ void block_finish(const CrtJob& J, unsigned long long* acc, uint32_t* wraps,
+                                             CarryFn* wave_total, uint32_t* ripple_s) {
+    const uint64_t L0 = (uint64_t)J.block * kCrtBlockLimbs;
+    uint64_t v[kCrtLimbsPerThread];
+    CarryFn f[kCrtLimbsPerThread];
+#pragma unroll
+    for (int q = 0; q < kCrtLimbsPerThread; ++q) {
+        const int jl = threadIdx.x * kCrtLimbsPerThread + q;
+        const uint64_t sum = acc[jl];
+        v[q] = sum;
+        f[q] = CarryFn{wraps[jl], limb_thresh(sum)};
+    }
+    // Inclusive scan of the threads' carry functions within each wave.
+    CarryFn agg = f[0];
+#pragma unroll
+    for (int q = 1; q < kCrtLimbsPerThread; ++q) agg = compose(agg, f[q]);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const CarryFn prev = shfl_up_fn(agg, d);
+        if (lane >= d) agg = compose(prev, agg);
+    }
+    if (lane == 63) wave_total[wave] = agg;
+    const CarryFn lane_excl = shfl_up_fn(agg, 1);
+    __syncthreads();
+    // Exclusive prefix of this thread: earlier waves, then earlier lanes.
+    bool have = false;
+    CarryFn g{0, kInf};
+    for (int u = 0; u < wave; ++u) {
+        g = have ? compose(g, wave_total[u]) : wave_total[u];
+        have = true;
+    }
+    if (lane > 0) {
+        g = have ? compose(g, lane_excl) : lane_excl;
+        have = true;
+    }
+    const uint32_t first_t = f[0].thresh;
+    uint32_t my_ripple = 0;
+    uint64_t outv[kCrtLimbsPerThread];
+#pragma unroll
+    for (int q = 0; q < kCrtLimbsPerThread; ++q) {
+        const uint32_t rel = threadIdx.x * kCrtLimbsPerThread + q;
+        if (have && g.thresh != kInf) my_ripple = rel;
+        outv[q] = v[q] + (have ? g.base : 0u);
+        g = have ? compose(g, f[q]) : f[q];
+        have = true;
+    }
+    if (my_ripple) atomicMax(ripple_s, my_ripple);
+    __syncthreads();  // every accumulator read is done; reuse them to coalesce the stores
+#pragma unroll
+    for (int q = 0; q < kCrtLimbsPerThread; ++q) acc[threadIdx.x * kCrtLimbsPerThread + q] = outv[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kCrtLimbsPerThread; ++q) {
+        const uint64_t j = L0 + threadIdx.x + q * kCrtThreads;
+        if (j < J.out_len) J.out[j] = acc[threadIdx.x + q * kCrtThreads];
+    }
+    if (threadIdx.x == kCrtThreads - 1) {
+        J.agg_base[J.block] = g.base;
+        J.agg_thresh[J.block] = g.thresh;
+    }
+    if (threadIdx.x == 0) {
+        J.first_thresh[J.block] = first_t;
+        J.ripple_end[J.block] = *ripple_s;
+    }
 }
 
 template <int NP>
@@ -904,67 +984,174 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
         }
     }
     __syncthreads();
-    uint64_t v[kCrtLimbsPerThread];
-    CarryFn f[kCrtLimbsPerThread];
-#pragma unroll
-    for (int q = 0; q < kCrtLimbsPerThread; ++q) {
-        const int jl = threadIdx.x * kCrtLimbsPerThread + q;
-        const uint64_t sum = acc[jl];
-        v[q] = sum;
-        f[q] = CarryFn{wraps[jl], limb_thresh(sum)};
+    block_finish(J, acc, wraps, wave_total, &ripple);
+}
+
+// Schoolbook product, no transform, for a shorter operand of at most
+// kSchoolMaxShort limbs. One workgroup per block of kSchoolBlockLimbs output
+// limbs, so that even a small product spreads over many CUs. Limbs are cut
+// into pieces of 22, 22 and 20 bits held as doubles: a piece product is
+// below 2^44 and exact, and the sums of one chunk of kSchoolChunk terms stay
+// below 2^53, so nine FMAs per limb product accumulate exactly (the FP64
+// pipes issue them at full rate). Column c = i + j of the limb products
+// feeds limbs c, c + 1 and c + 2, so the block takes the columns L0 - 2 ..
+// L0 + kSchoolBlockLimbs - 1; a work unit is four consecutive columns times
+// one chunk of terms, whose window of the longer operand slides by one limb
+// per term. Each unit's column values, three words each, go into the limb
+// accumulators with LDS atomics that count wrap-arounds; one wave then
+// resolves the carries inside the block and publishes the block's carry
+// function for k_carry_apply.
+constexpr int kSchoolChunk = 16;
+constexpr int kSchoolCols = 4;
+constexpr int kSchoolThreads = 256;
+
+__device__ __forceinline__ void lds_add(unsigned long long* acc, uint32_t* wraps, int r, uint64_t w) {
+    if (w != 0 && r >= 0 && r < kSchoolBlockLimbs) {
+        const unsigned long long old = atomicAdd(&acc[r], (unsigned long long)w);
+        if (old + w < old) atomicAdd(&wraps[r], 1u);
     }
-    // Inclusive scan of the threads' carry functions within each wave.
-    CarryFn agg = f[0];
-#pragma unroll
-    for (int q = 1; q < kCrtLimbsPerThread; ++q) agg = compose(agg, f[q]);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const CarryFn prev = shfl_up_fn(agg, d);
-        if (lane >= d) agg = compose(prev, agg);
+}
+
+// Pieces of a limb: bits 0-21, 22-43 and 44-63.
+__device__ __forceinline__ void limb_pieces(uint64_t x, double* p0, double* p1, double* p2) {
+    *p0 = (double)(uint32_t)(x & 0x3FFFFFu);
+    *p1 = (double)(uint32_t)((x >> 22) & 0x3FFFFFu);
+    *p2 = (double)(uint32_t)(x >> 44);
+}
+
+__global__ __launch_bounds__(kSchoolThreads) void k_school(CrtArgs a) {
+    constexpr int BL = kSchoolBlockLimbs;
+    constexpr int NC = BL + 2;                                    // columns L0-2 .. L0+BL-1
+    constexpr int GROUPS = (NC + kSchoolCols - 1) / kSchoolCols;  // column groups
+    constexpr int PAD = kSchoolChunk;                             // window slack below
+    constexpr int WIN = PAD + GROUPS * kSchoolCols + kSchoolMaxShort;
+    static_assert(kSchoolMaxShort % kSchoolChunk == 0 && kSchoolChunk % kSchoolCols == 0, "chunking");
+    __shared__ unsigned long long acc[BL];
+    __shared__ uint32_t wraps[BL];
+    __shared__ double lp0[WIN], lp1[WIN], lp2[WIN];
+    __shared__ double sp0[kSchoolMaxShort], sp1[kSchoolMaxShort], sp2[kSchoolMaxShort];
+    const CrtJob J = crt_job(a);
+    const bool swap = J.len_b > J.len_a;
+    const uint64_t* pl = swap ? J.op_b : J.op_a;
+    const uint64_t* ps = swap ? J.op_a : J.op_b;
+    const int64_t ll = (int64_t)(swap ? J.len_b : J.len_a);
+    const int ls = (int)(swap ? J.len_a : J.len_b);  // 1 .. kSchoolMaxShort (host-checked)
+    const int nch = (ls + kSchoolChunk - 1) / kSchoolChunk;
+    const int64_t L0 = (int64_t)J.block * BL;
+    const int64_t c0 = L0 - 2;
+    // Window: limb c - j of column c = c0 + cr sits at index PAD + cr + ls - 1 - j
+    // for cr < GROUPS*kSchoolCols and 0 <= j < nch*kSchoolChunk; limbs outside
+    // the operand are zero, and terms past ls multiply zero pieces of the
+    // shorter operand.
+    const int64_t w0 = c0 - (ls - 1) - PAD;
+    const int W = PAD + GROUPS * kSchoolCols + ls - 1;
+    for (int t = threadIdx.x; t < W; t += kSchoolThreads) {
+        const int64_t i = w0 + t;
+        limb_pieces((i >= 0 && i < ll) ? pl[i] : 0, &lp0[t], &lp1[t], &lp2[t]);
     }
-    if (lane == 63) wave_total[wave] = agg;
-    const CarryFn lane_excl = shfl_up_fn(agg, 1);
+    for (int t = threadIdx.x; t < nch * kSchoolChunk; t += kSchoolThreads)
+        limb_pieces(t < ls ? ps[t] : 0, &sp0[t], &sp1[t], &sp2[t]);
+    for (int t = threadIdx.x; t < BL; t += kSchoolThreads) {
+        acc[t] = 0;
+        wraps[t] = 0;
+    }
     __syncthreads();
-    // Exclusive prefix of this thread: earlier waves, then earlier lanes.
-    bool have = false;
-    CarryFn g{0, kInf};
-    for (int u = 0; u < wave; ++u) {
-        g = have ? compose(g, wave_total[u]) : wave_total[u];
-        have = true;
-    }
-    if (lane > 0) {
-        g = have ? compose(g, lane_excl) : lane_excl;
-        have = true;
-    }
-    const uint32_t first_t = f[0].thresh;
-    uint32_t my_ripple = 0;
-    uint64_t outv[kCrtLimbsPerThread];
+    for (int u = threadIdx.x; u < GROUPS * nch; u += kSchoolThreads) {
+        const int g = u % GROUPS, ch = u / GROUPS;
+        const int cr = g * kSchoolCols;  // first column of the unit, relative to c0
+        const int jb = ch * kSchoolChunk;
+        // The unit reads limbs c - j in [c0+cr-jb-CH+1, c0+cr+3-jb]: skip it when
+        // they all lie outside the longer operand.
+        const int64_t i_hi = c0 + cr + kSchoolCols - 1 - jb;
+        const int64_t i_lo = c0 + cr - (jb + kSchoolChunk - 1);
+        if (i_hi < 0 || i_lo >= ll) continue;
+        double s[kSchoolCols][5];
 #pragma unroll
-    for (int q = 0; q < kCrtLimbsPerThread; ++q) {
-        const uint32_t rel = threadIdx.x * kCrtLimbsPerThread + q;
-        if (have && g.thresh != kInf) my_ripple = rel;
-        outv[q] = v[q] + (have ? g.base : 0u);
-        g = have ? compose(g, f[q]) : f[q];
-        have = true;
-    }
-    if (my_ripple) atomicMax(&ripple, my_ripple);
-    __syncthreads();  // every accumulator read is done; reuse them to coalesce the stores
+        for (int q = 0; q < kSchoolCols; ++q)
 #pragma unroll
-    for (int q = 0; q < kCrtLimbsPerThread; ++q) acc[threadIdx.x * kCrtLimbsPerThread + q] = outv[q];
+            for (int k = 0; k < 5; ++k) s[q][k] = 0.0;
+        // Ring of the four limbs in use: at term jb + t, column q takes slot
+        // (q - t) mod 4, and the new limb of column 0 replaces the one column 3
+        // no longer needs.
+        const int base = PAD + cr + ls - 1 - jb;  // window index of limb c0 + cr - jb
+        double r0[kSchoolCols], r1[kSchoolCols], r2[kSchoolCols];
+#pragma unroll
+        for (int q = 1; q < kSchoolCols; ++q) {
+            r0[q] = lp0[base + q];
+            r1[q] = lp1[base + q];
+            r2[q] = lp2[base + q];
+        }
+#pragma unroll 1
+        for (int t4 = 0; t4 < kSchoolChunk; t4 += kSchoolCols) {
+#pragma unroll
+            for (int tt = 0; tt < kSchoolCols; ++tt) {
+                const int t = t4 + tt;
+                const int in = (kSchoolCols - tt) & (kSchoolCols - 1);
+                r0[in] = lp0[base - t];
+                r1[in] = lp1[base - t];
+                r2[in] = lp2[base - t];
+                const double y0 = sp0[jb + t], y1 = sp1[jb + t], y2 = sp2[jb + t];
+#pragma unroll
+                for (int q = 0; q < kSchoolCols; ++q) {
+                    const int sl = (q - tt + kSchoolCols) & (kSchoolCols - 1);
+                    const double x0 = r0[sl], x1 = r1[sl], x2 = r2[sl];
+                    s[q][0] = fma(x0, y0, s[q][0]);
+                    s[q][1] = fma(x0, y1, s[q][1]);
+                    s[q][1] = fma(x1, y0, s[q][1]);
+                    s[q][2] = fma(x0, y2, s[q][2]);
+                    s[q][2] = fma(x1, y1, s[q][2]);
+                    s[q][2] = fma(x2, y0, s[q][2]);
+                    s[q][3] = fma(x1, y2, s[q][3]);
+                    s[q][3] = fma(x2, y1, s[q][3]);
+                    s[q][4] = fma(x2, y2, s[q][4]);
+                }
+            }
+        }
+        // Column value sum_k s[q][k] * 2^(22k) in three words.
+#pragma unroll
+        for (int q = 0; q < kSchoolCols; ++q) {
+            typedef unsigned __int128 u128;
+            const uint64_t a0 = (uint64_t)s[q][0], a1 = (uint64_t)s[q][1], a2 = (uint64_t)s[q][2];
+            const uint64_t a3 = (uint64_t)s[q][3], a4 = (uint64_t)s[q][4];
+            const u128 t = (u128)a0 + ((u128)a1 << 22) + ((u128)a2 << 44);
+            const u128 w = ((u128)a3 << 2) + ((u128)a4 << 24) + (uint64_t)(t >> 64);
+            const int r = cr + q - 2;  // limb c relative to L0
+            lds_add(acc, wraps, r, (uint64_t)t);
+            lds_add(acc, wraps, r + 1, (uint64_t)w);
+            lds_add(acc, wraps, r + 2, (uint64_t)(w >> 64));
+        }
+    }
     __syncthreads();
+    // One wave resolves the block's carries: lane = limb.
+    if (threadIdx.x < 64) {
+        static_assert(BL == 64, "one limb per lane");
+        const int lane = threadIdx.x;
+        const uint64_t v = acc[lane];
+        const CarryFn f{wraps[lane], limb_thresh(v)};
+        CarryFn agg = f;
 #pragma unroll
-    for (int q = 0; q < kCrtLimbsPerThread; ++q) {
-        const uint64_t j = L0 + threadIdx.x + q * kCrtThreads;
-        if (j < J.out_len) J.out[j] = acc[threadIdx.x + q * kCrtThreads];
-    }
-    if (threadIdx.x == kCrtThreads - 1) {
-        J.agg_base[J.block] = g.base;
-        J.agg_thresh[J.block] = g.thresh;
-    }
-    if (threadIdx.x == 0) {
-        J.first_thresh[J.block] = first_t;
-        J.ripple_end[J.block] = ripple;
+        for (int d = 1; d < 64; d <<= 1) {
+            const CarryFn prev = shfl_up_fn(agg, d);
+            if (lane >= d) agg = compose(prev, agg);
+        }
+        const CarryFn excl = shfl_up_fn(agg, 1);
+        const uint64_t outv = v + (lane > 0 ? excl.base : 0u);
+        uint32_t my_ripple = (lane > 0 && excl.thresh != kInf) ? (uint32_t)lane : 0u;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            const uint32_t o = __shfl_xor(my_ripple, d, 64);
+            my_ripple = o > my_ripple ? o : my_ripple;
+        }
+        const uint64_t jl = (uint64_t)L0 + lane;
+        if (jl < J.out_len) J.out[jl] = outv;
+        if (lane == 63) {
+            J.agg_base[J.block] = agg.base;
+            J.agg_thresh[J.block] = agg.thresh;
+        }
+        if (lane == 0) {
+            J.first_thresh[J.block] = f.thresh;
+            J.ripple_end[J.block] = my_ripple;
+        }
     }
 }
 
@@ -976,6 +1163,7 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
 // ripple end take one more when the carry reaches the first limb's threshold.
 constexpr int kApplyThreads = 64;
 
+template <int BL>
 __global__ __launch_bounds__(kApplyThreads) void k_carry_apply(CrtArgs a) {
     __shared__ uint32_t found;
     __shared__ uint32_t cin_s;
@@ -1015,7 +1203,7 @@ __global__ __launch_bounds__(kApplyThreads) void k_carry_apply(CrtArgs a) {
     __syncthreads();
     c = cin_s;
     if (c == 0) return;
-    const uint64_t L0 = (uint64_t)b * kCrtBlockLimbs;
+    const uint64_t L0 = (uint64_t)b * BL;
     if (threadIdx.x == 0 && L0 < J.out_len) J.out[L0] += c;
     if (c >= J.first_thresh[b]) {
         const uint32_t rend = J.ripple_end[b];
@@ -1214,7 +1402,18 @@ bool launch_crt_product(const CrtArgs& a, hipStream_t s) {
         default: return false;
     }
     if (!NTT_CHECK_LAUNCH()) return false;
-    if (a.blocks > 1 || a.batch) NTT_LAUNCH("carry_apply", k_carry_apply, dim3(a.blocks), dim3(kApplyThreads), s, a);
+    if (a.blocks > 1 || a.batch)
+        NTT_LAUNCH("carry_apply", k_carry_apply<kCrtBlockLimbs>, dim3(a.blocks), dim3(kApplyThreads), s, a);
+    return NTT_CHECK_LAUNCH();
+}
+
+// Schoolbook product blocks of kSchoolBlockLimbs limbs, then the carries
+// across blocks; in a batched launch a.blocks counts (product, block) items.
+bool launch_school(const CrtArgs& a, hipStream_t s) {
+    NTT_LAUNCH("school", k_school, dim3(a.blocks), dim3(kSchoolThreads), s, a);
+    if (!NTT_CHECK_LAUNCH()) return false;
+    if (a.blocks > 1 || a.batch)
+        NTT_LAUNCH("carry_apply", k_carry_apply<kSchoolBlockLimbs>, dim3(a.blocks), dim3(kApplyThreads), s, a);
     return NTT_CHECK_LAUNCH();
 }
 

@@ -293,7 +293,7 @@ bool Server::run_large(Slot& slot, Request* q) {
     if (!(conv && q->plan.schoolbook)) {
         buf.spec_a = reinterpret_cast<double*>(carve(op_spectra_bytes(q->g)));
         buf.spec_b = q->square ? buf.spec_a : reinterpret_cast<double*>(carve(op_spectra_bytes_b(q->g)));
-        buf.carry = reinterpret_cast<uint32_t*>(carve(op_carry_bytes(q->out_len)));
+        buf.carry = reinterpret_cast<uint32_t*>(carve(op_carry_bytes(q->out_len, q->g.school)));
     }
     const bool ok = conv ? convolution_on(s, q->g, &q->plan, da, db, q->square, buf, dout)
                          : product_on(s, q->g, da, la, db, lb, q->square, buf, dout);
@@ -414,7 +414,7 @@ static bool prepare_request(const ntt_request* r, Request* q) {
     if (!q->device) bytes += align_up(la * 8) + (q->square ? 0 : align_up(lb * 8)) + align_up(q->out_len * 8);
     if (!(r->kind == NTT_REQ_CONV && q->plan.schoolbook))
         bytes += align_up(op_spectra_bytes(q->g)) + (q->square ? 0 : align_up(op_spectra_bytes_b(q->g))) +
-                 align_up(op_carry_bytes(q->out_len));
+                 align_up(op_carry_bytes(q->out_len, q->g.school));
     q->bytes = bytes;
     return true;
 }

@@ -87,6 +87,22 @@ def test_device_geometry_fits_capacity(kat, la, lb):
         assert max(da, db) <= 1 << log_len
 
 
+def test_schoolbook_admission(monkeypatch):
+    """The schoolbook takes shorter operands of at most 512 limbs: forced on,
+    every such shape runs it and no other; forced off, none; by price, tiny
+    products do and large balanced ones do not."""
+    monkeypatch.setenv("NTTAMD_SCHOOL", "1")
+    for la, lb in [(1, 1), (512, 512), (512, 1 << 20), (1 << 20, 7)]:
+        assert ntt.mul_schoolbook(la, lb) and ntt.MulPlan.new(la, lb).schoolbook, (la, lb)
+    for la, lb in [(513, 513), (1000, 600), (1 << 20, 1 << 20)]:
+        assert not ntt.mul_schoolbook(la, lb), (la, lb)
+    monkeypatch.setenv("NTTAMD_SCHOOL", "0")
+    assert not any(ntt.mul_schoolbook(la, lb) for la, lb in [(1, 1), (64, 64), (512, 512)])
+    monkeypatch.delenv("NTTAMD_SCHOOL")
+    assert ntt.mul_schoolbook(1, 1) and ntt.mul_schoolbook(8, 8)
+    assert not ntt.mul_schoolbook(1 << 16, 1 << 16) and not ntt.mul_schoolbook(0, 5)
+
+
 def test_headline_geometry():
     # 2^20 x 2^20 limbs: the engine's geometry holds the whole product in a
     # power-of-two transform of at most 2^21 points.

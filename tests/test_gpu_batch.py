@@ -126,6 +126,7 @@ def test_batch_split_products(oracle, monkeypatch):
     summing the blocks' residues; beside whole products and squares, with
     either operand the longer one."""
     monkeypatch.setenv("NTTAMD_SPLIT", "1")
+    monkeypatch.setenv("NTTAMD_SCHOOL", "0")
     la = [20000, 1000, 65536, 192, 65536, 5000, 300000, 12345, 4096, 3000, 7000]
     lb = [1000, 20000, 192, 65536, 2000, 2500, 1000, 777, 4096, 3000, 64]
     sq = [False] * 8 + [True, False, False]

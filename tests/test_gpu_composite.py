@@ -40,13 +40,15 @@ def device():
     os.environ.pop("NTTAMD_COMPOSITE", None)
     os.environ.pop("NTTAMD_PRIMES", None)
     os.environ.pop("NTTAMD_SPLIT", None)
+    os.environ.pop("NTTAMD_SCHOOL", None)
 
 
 @contextlib.contextmanager
 def rows(composite, primes=None):
-    saved = {k: os.environ.get(k) for k in ("NTTAMD_COMPOSITE", "NTTAMD_PRIMES", "NTTAMD_SPLIT")}
+    saved = {k: os.environ.get(k) for k in ("NTTAMD_COMPOSITE", "NTTAMD_PRIMES", "NTTAMD_SPLIT", "NTTAMD_SCHOOL")}
     os.environ["NTTAMD_COMPOSITE"] = "1" if composite else "0"
     os.environ["NTTAMD_SPLIT"] = "0"  # whole products; splits have their own tests
+    os.environ["NTTAMD_SCHOOL"] = "0"  # transforms, also for the shortest operands
     if primes:
         os.environ["NTTAMD_PRIMES"] = str(primes)
     else:

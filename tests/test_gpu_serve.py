@@ -56,7 +56,11 @@ def submit(server, req, a, b, dst):
     return server.submit_mul(a, b, dst)
 
 
-def test_served_mix_equals_alone(pools, oracle):
+def test_served_mix_equals_alone(pools, oracle, monkeypatch):
+    # Transforms for every request, so that the large working sets (spectra
+    # included) exceed the budget together; the schoolbook runs in the other
+    # served tests.
+    monkeypatch.setenv("NTTAMD_SCHOOL", "0")
     reqs = served_stream(96, 0.0, lo=256, hi=1 << 18, seed=21)
     say(f"served mix: {len(reqs)} requests")
     # A budget of 48 MiB holds only a few large working sets at once, so

@@ -72,6 +72,7 @@ def product(a, b):
 @pytest.mark.parametrize("la,lb,force_primes", SHAPES)
 def test_every_split_matches_the_default(oracle, monkeypatch, la, lb, force_primes):
     monkeypatch.setenv("NTTAMD_COMPOSITE", "0")  # power-of-two rows; composite rows have their own tests
+    monkeypatch.setenv("NTTAMD_SCHOOL", "0")
     if force_primes:
         monkeypatch.setenv("NTTAMD_PRIMES", str(force_primes))
     primes, width, K, _ = ntt.mul_geometry(la, lb)
