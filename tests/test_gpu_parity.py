@@ -87,6 +87,28 @@ def test_random_shapes_match_oracle(oracle, seed):
         assert (engine_mul(a, b) == oracle.mul(a, b, threads=4)).all(), (la, lb)
 
 
+def test_default_forms_over_random_shapes(oracle):
+    """Shapes log-uniform in [1, 200,000] limbs each way run in whatever form
+    the engine's pricing picks (schoolbook, unbalanced split, whole
+    power-of-two row); every form occurs and every product equals the
+    oracle."""
+    rng = np.random.default_rng(77)
+    seen = {}
+    for k in range(48):
+        la = int(np.exp(rng.uniform(0, np.log(200000))))
+        lb = int(np.exp(rng.uniform(0, np.log(200000))))
+        form = ("schoolbook" if ntt.mul_schoolbook(la, lb) else
+                "split" if ntt.mul_split(la, lb)[0] else "row")
+        seen[form] = seen.get(form, 0) + 1
+        say(f"shape {la}x{lb}: {form}")
+        a = rng.integers(0, 2 ** 64, la, dtype=np.uint64)
+        b = rng.integers(0, 2 ** 64, lb, dtype=np.uint64)
+        if k % 4 == 1:
+            a[:] = np.uint64(MASK)
+        assert (engine_mul(a, b) == oracle.mul(a, b, threads=8)).all(), (la, lb, form)
+    assert set(seen) == {"schoolbook", "split", "row"}, seen
+
+
 @pytest.mark.parametrize("n", [1, 5, 1000, 1024, 4096, 65536, 1 << 18])
 def test_squares_match_oracle(oracle, n):
     say(f"square {n}")
