@@ -190,3 +190,25 @@ def test_school_in_batches(oracle):
         assert whole.mul_host(a, b, out2), ntt.last_error()
     for i in range(len(la)):
         assert (out2[i] == want[i]).all(), f"product {i} without the schoolbook"
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_carries_across_few_and_many_blocks(mode):
+    """All-ones products (a carry into every block, rippling through it) with
+    outputs of 2 to 40 carry blocks, of the schoolbook (64 limbs) and of the
+    reconstruction (1024 limbs), each run twice in a row and in both orders,
+    equal the closed form."""
+    bl = 64 if mode else 1024
+    shapes = []
+    for blocks in (2, 3, 31, 32, 33, 40):
+        out = blocks * bl - (bl // 2 if blocks > 2 else 0)
+        la = min(out // 2, 512) if mode else out // 2
+        shapes.append((la, out - la))
+    with school(mode):
+        for la, lb in shapes + shapes[::-1]:
+            assert ntt.mul_schoolbook(la, lb) == bool(mode), (la, lb)
+            a = np.full(la, MASK, dtype=np.uint64)
+            b = np.full(lb, MASK, dtype=np.uint64)
+            want = all_ones_product(la, lb)
+            for _ in range(2):
+                assert (product(a, b) == want).all(), (la, lb)
