@@ -91,7 +91,12 @@ struct Tw {
 // A short first round of 1-3 levels, entered below 31p/16, ends below 3p/2.
 // Values never exceed 19p/4 < 2^53, packed inputs are below p, and spectra
 // stay below 31p/16, so a pointwise product stays below 3.76p^2 < 4p^2.
-template <int LOGL, int LINES, int NT, int L0, int RLOG>
+//
+// ROOT1: level 0 of this round is the first level of the transform at table
+// node 0, whose root W[0] is 1, and its inputs are packed (below p), so the
+// multiplied input enters its butterfly as it is: outputs below 2p at level
+// 0 and, for a round of one level, below 9p/16 + p < 31p/16.
+template <int LOGL, int LINES, int NT, int L0, int RLOG, bool ROOT1 = false>
 __device__ __forceinline__ void fwd_round(double* lds, Tw tw, double p, double pinv) {
     constexpr int L = 1 << LOGL;
     constexpr int RADIX = 1 << RLOG;
@@ -119,7 +124,7 @@ __device__ __forceinline__ void fwd_round(double* lds, Tw tw, double p, double p
                 for (int j = 0; j < half; ++j) {
                     const int k = blk * 2 * half + j;
                     const double u = s == RLOG - 1 ? reduce(x[k], p, pinv) : x[k];
-                    const double t = mulmod(x[k + half], w, p, pinv);
+                    const double t = (ROOT1 && L0 == 0 && s == 0) ? x[k + half] : mulmod(x[k + half], w, p, pinv);
                     x[k] = u + t;
                     x[k + half] = u - t;
                 }
@@ -136,7 +141,10 @@ __device__ __forceinline__ void fwd_round(double* lds, Tw tw, double p, double p
 // an odd level; an even level then sees inputs below 11p/4, reduces sums
 // below 11p/2 to 9p/16 and keeps differences times a root below 4p^2
 // (outputs below 11p/8). Every round ends on level 0 below 11p/8.
-template <int LOGL, int LINES, int NT, int L0, int RLOG>
+//
+// ROOT1: level 0 of this round is the last level of the transform at table
+// node 0, whose root is 1: the difference is only reduced (below 9p/16).
+template <int LOGL, int LINES, int NT, int L0, int RLOG, bool ROOT1 = false>
 __device__ __forceinline__ void inv_round(double* lds, Tw tw, double p, double pinv) {
     constexpr int L = 1 << LOGL;
     constexpr int RADIX = 1 << RLOG;
@@ -166,7 +174,7 @@ __device__ __forceinline__ void inv_round(double* lds, Tw tw, double p, double p
                     const double u = x[k], v = x[k + half];
                     const double sum = u + v;
                     x[k] = (s & 1) ? sum : reduce(sum, p, pinv);
-                    x[k + half] = mulmod(u - v, w, p, pinv);
+                    x[k + half] = (ROOT1 && L0 == 0 && s == 0) ? reduce(u - v, p, pinv) : mulmod(u - v, w, p, pinv);
                 }
             }
         }
@@ -175,25 +183,35 @@ __device__ __forceinline__ void inv_round(double* lds, Tw tw, double p, double p
     }
 }
 
+// Rounds R.. of a forward pass; root1 (uniform): the pass starts the
+// transform at table node 0 from packed inputs (fwd_round's ROOT1).
 template <int LOGL, int LINES, int NT, int R>
-__device__ __forceinline__ void fwd_rounds(double* lds, Tw tw, double p, double pinv) {
+__device__ __forceinline__ void fwd_rounds(double* lds, Tw tw, double p, double pinv, bool root1 = false) {
     if constexpr (R < Lay<LOGL>::ROUNDS) {
         constexpr int L0 = R == 0 ? 0 : Lay<LOGL>::FIRST + 4 * (R - 1);
         constexpr int RL = R == 0 ? Lay<LOGL>::FIRST : 4;
-        fwd_round<LOGL, LINES, NT, L0, RL>(lds, tw, p, pinv);
+        if (R == 0 && root1)
+            fwd_round<LOGL, LINES, NT, L0, RL, true>(lds, tw, p, pinv);
+        else
+            fwd_round<LOGL, LINES, NT, L0, RL, false>(lds, tw, p, pinv);
         __syncthreads();
         fwd_rounds<LOGL, LINES, NT, R + 1>(lds, tw, p, pinv);
     }
 }
 
+// Rounds R..0 of an inverse pass; root1 (uniform): the pass ends the
+// transform at table node 0 (inv_round's ROOT1).
 template <int LOGL, int LINES, int NT, int R>
-__device__ __forceinline__ void inv_rounds(double* lds, Tw tw, double p, double pinv) {
+__device__ __forceinline__ void inv_rounds(double* lds, Tw tw, double p, double pinv, bool root1 = false) {
     if constexpr (R >= 0) {
         constexpr int L0 = R == 0 ? 0 : Lay<LOGL>::FIRST + 4 * (R - 1);
         constexpr int RL = R == 0 ? Lay<LOGL>::FIRST : 4;
-        inv_round<LOGL, LINES, NT, L0, RL>(lds, tw, p, pinv);
+        if (R == 0 && root1)
+            inv_round<LOGL, LINES, NT, L0, RL, true>(lds, tw, p, pinv);
+        else
+            inv_round<LOGL, LINES, NT, L0, RL, false>(lds, tw, p, pinv);
         __syncthreads();
-        inv_rounds<LOGL, LINES, NT, R - 1>(lds, tw, p, pinv);
+        inv_rounds<LOGL, LINES, NT, R - 1>(lds, tw, p, pinv, root1);
     }
 }
 
@@ -208,13 +226,16 @@ struct Lay8 {
 };
 
 template <int LOGL, int LINES, int NT, int R>
-__device__ __forceinline__ void inv_rounds8(double* lds, Tw tw, double p, double pinv) {
+__device__ __forceinline__ void inv_rounds8(double* lds, Tw tw, double p, double pinv, bool root1 = false) {
     if constexpr (R >= 0) {
         constexpr int L0 = R == 0 ? 0 : Lay8<LOGL>::FIRST + 3 * (R - 1);
         constexpr int RL = R == 0 ? Lay8<LOGL>::FIRST : 3;
-        inv_round<LOGL, LINES, NT, L0, RL>(lds, tw, p, pinv);
+        if (R == 0 && root1)
+            inv_round<LOGL, LINES, NT, L0, RL, true>(lds, tw, p, pinv);
+        else
+            inv_round<LOGL, LINES, NT, L0, RL, false>(lds, tw, p, pinv);
         __syncthreads();
-        inv_rounds8<LOGL, LINES, NT, R - 1>(lds, tw, p, pinv);
+        inv_rounds8<LOGL, LINES, NT, R - 1>(lds, tw, p, pinv, root1);
     }
 }
 
@@ -571,7 +592,9 @@ __global__ __launch_bounds__(NT) void k_fwd_strided(PassArgs a) {
     }
     __syncthreads();
     const Tw tw{a.fwd + (uint64_t)y * a.table_stride, H + (a.node_base << a.level_start), 0, 0, 64, p, pinv};
-    fwd_rounds<LOGL, LINES, NT, 0>(lds, tw, p, pinv);
+    // A packing pass starts the transform (level_start 0); at node 0 its first
+    // root is 1.
+    fwd_rounds<LOGL, LINES, NT, 0>(lds, tw, p, pinv, PACK && a.node_base == 0);
     store_strided<LOGL, LINES, NT, false>(lds, row + base, S, 0.0, p, pinv);
 }
 
@@ -596,7 +619,9 @@ __global__ __launch_bounds__(NT) void k_inv_strided(PassArgs a) {
     load_strided<LOGL, LINES, NT>(lds, row + base, S);
     __syncthreads();
     const Tw tw{a.inv + (uint64_t)y * a.table_stride, H + (a.node_base << a.level_start), 0, 0, 64, p, pinv};
-    inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, tw, p, pinv);
+    // The canonical pass ends the transform (level_start 0); at node 0 its
+    // last root is 1.
+    inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, tw, p, pinv, CANON && a.node_base == 0);
     store_strided<LOGL, LINES, NT, CANON>(lds, row + base, S, a.tinv[y], p, pinv);
 }
 
@@ -646,7 +671,7 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     const Tw twI{a.inv + (uint64_t)y * a.table_stride, HR, 1, LINES - 1, split, p, pinv};
     // The lines of both tiles go through one set of rounds; line c of either
     // tile is node H0 + (c mod LINES).
-    fwd_rounds<LOGL, TILES * LINES, NT, 0>(lds, twF, p, pinv);
+    fwd_rounds<LOGL, TILES * LINES, NT, 0>(lds, twF, p, pinv, PACK && a.node_base == 0);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int e = threadIdx.x + r * NT;
@@ -662,9 +687,9 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     // The product is one tile: radix-16 groups would occupy half of the
     // threads, so a product (not a square) runs its inverse in radix 8.
     if constexpr (SQUARE)
-        inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, twI, p, pinv);
+        inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, twI, p, pinv, PACK && a.node_base == 0);
     else
-        inv_rounds8<LOGL, LINES, NT, Lay8<LOGL>::ROUNDS - 1>(lds, twI, p, pinv);
+        inv_rounds8<LOGL, LINES, NT, Lay8<LOGL>::ROUNDS - 1>(lds, twI, p, pinv, PACK && a.node_base == 0);
     if constexpr (PACK) {
         const double tinv = a.tinv[y];
 #pragma unroll
