@@ -150,6 +150,32 @@ def test_all_ones_2p20():
     assert (engine_mul(a, a.copy()) == all_ones_product(1 << 20, 1 << 20)).all()
 
 
+def test_largest_tested_product():
+    """2^26 x 2^26 limbs (512 MiB operands, six primes on 2^26-point rows in
+    three passes, 1 GiB result) against its residue modulo 2^61 - 1, and
+    2^26 all-ones limbs squared against the closed form."""
+    n = 1 << 26
+    a = oracle_operand(n, 5)
+    b = oracle_operand(n, 6)
+    primes, width, log_n, tail = ntt.mul_geometry(n, n)
+    say(f"2^26 x 2^26: {primes} primes x {width} bits, row 2^{log_n} (+{tail}/4)")
+    got = engine_mul(a, b)
+    q = (1 << 61) - 1
+    assert residue(got, q) == residue(a, q) * residue(b, q) % q
+    del got, b
+    a[:] = np.uint64(MASK)
+    assert (engine_sqr(a) == all_ones_product(n, n)).all()
+
+
+def oracle_operand(n, seed):
+    import oracle
+    return oracle.operand(n, seed)
+
+
+def residue(limbs, q):
+    return int.from_bytes(np.ascontiguousarray(limbs, dtype="<u8").tobytes(), "little") % q
+
+
 def test_sqr_2p24_golden(oracle, digests):
     say("sqr 2^24")
     a = oracle.operand(1 << 24, 1)
