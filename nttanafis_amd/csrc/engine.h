@@ -203,6 +203,11 @@ struct Batch {
     uint32_t crt_begin[kMaxPrimes + 1] = {0}, crt_count[kMaxPrimes + 1] = {0};
     DevBuf products, items, arena, carry;
     std::vector<BatchLaunch> program;
+    // program[seg[i], seg[i+1]) for i < seg.size() - 1 are independent
+    // segments (the launches of one bucket, or the schoolbook launch); the
+    // reconstruction launches from seg.back() on depend on all of them.
+    std::vector<size_t> seg;
+    std::vector<hipStream_t> side;  // branches of the captured graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool bound = false;
@@ -210,7 +215,10 @@ struct Batch {
     const double *fwd = nullptr, *inv = nullptr;
     uint64_t tstride = 0;
 
-    ~Batch() { drop_graph(); }
+    ~Batch() {
+        drop_graph();
+        for (hipStream_t x : side) hipStreamDestroy(x);
+    }
     void drop_graph() {
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);

@@ -1201,10 +1201,16 @@ static PassArgs bucket_base(const Batch& B, int K) {
 
 static void build_program(Batch& B) {
     B.program.clear();
+    B.seg.clear();
     const BatchItem* I = B.items.as<BatchItem>();
-    for (const BatchBucket& r : B.buckets)
+    for (const BatchBucket& r : B.buckets) {
+        B.seg.push_back(B.program.size());
         bucket_program(B.program, bucket_base(B, r.K), r.K, split_of(r.K), I, r);
+    }
+    // The schoolbook launch (np == 0) depends on no transform: its own
+    // segment; the reconstructions follow every bucket.
     for (int np = 0; np <= kMaxPrimes; ++np) {
+        if (np <= 1) B.seg.push_back(B.program.size());
         if (!B.crt_count[np]) continue;
         BatchLaunch L;
         memset(&L, 0, sizeof L);
@@ -1443,6 +1449,66 @@ static bool batch_autotune(Batch& B) {
     return true;
 }
 
+// Branches of the captured graph: the independent segments (buckets and the
+// schoolbook launch) are dealt to kBatchBranches side streams forked from
+// the capturing stream, so that small launches and the tails of large ones
+// overlap; the reconstructions run after the join, one branch per prime
+// count. Streams keep the default priority.
+constexpr int kBatchBranches = 4;
+
+static bool issue_branched(Batch& B, hipStream_t s) {
+    // NTTAMD_BATCH_BRANCHES=1 captures the launches in one chain (tests, A/B).
+    const char* env = getenv("NTTAMD_BATCH_BRANCHES");
+    const int branches = env ? std::max(1, std::min(atoi(env), 8)) : kBatchBranches;
+    if (branches == 1) return issue_program(B.program, s);
+    const size_t nseg = B.seg.empty() ? 0 : B.seg.size() - 1;
+    if ((int)B.side.size() != branches) {
+        for (hipStream_t x : B.side) hipStreamDestroy(x);
+        B.side.clear();
+        for (int k = 0; k < branches; ++k) {
+            hipStream_t x = nullptr;
+            if (hipStreamCreateWithFlags(&x, hipStreamNonBlocking) != hipSuccess) return false;
+            B.side.push_back(x);
+        }
+    }
+    std::vector<hipEvent_t> ev;
+    auto event = [&]() -> hipEvent_t {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+        ev.push_back(e);
+        return e;
+    };
+    // fork: every branch waits for what s has queued so far, runs its
+    // launches, and s waits for every branch.
+    auto fork_join = [&](const std::vector<std::vector<const BatchLaunch*>>& work) -> bool {
+        hipEvent_t f = event();
+        if (!f || hipEventRecord(f, s) != hipSuccess) return false;
+        for (size_t k = 0; k < work.size(); ++k) {
+            if (work[k].empty()) continue;
+            if (hipStreamWaitEvent(B.side[k], f, 0) != hipSuccess) return false;
+            for (const BatchLaunch* L : work[k])
+                if (!issue_launch(*L, B.side[k])) return false;
+            hipEvent_t j = event();
+            if (!j || hipEventRecord(j, B.side[k]) != hipSuccess || hipStreamWaitEvent(s, j, 0) != hipSuccess)
+                return false;
+        }
+        return true;
+    };
+    // Segments in transform-length order, dealt round robin.
+    std::vector<std::vector<const BatchLaunch*>> work(B.side.size());
+    for (size_t i = 0; i < nseg; ++i)
+        for (size_t l = B.seg[i]; l < B.seg[i + 1]; ++l) work[i % work.size()].push_back(&B.program[l]);
+    bool ok = fork_join(work);
+    if (ok) {
+        std::vector<std::vector<const BatchLaunch*>> crt(B.side.size());
+        const size_t first = B.seg.empty() ? 0 : B.seg.back();
+        for (size_t l = first; l < B.program.size(); ++l) crt[(l - first) % crt.size()].push_back(&B.program[l]);
+        ok = fork_join(crt);
+    }
+    for (hipEvent_t e : ev) hipEventDestroy(e);
+    return ok;
+}
+
 static bool batch_capture(Batch& B) {
     const hipStream_t s = B.ex.stream;
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
@@ -1450,7 +1516,7 @@ static bool batch_capture(Batch& B) {
         return false;
     }
     t_capturing = true;
-    const bool ok = issue_program(B.program, s);
+    const bool ok = issue_branched(B, s);
     t_capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
