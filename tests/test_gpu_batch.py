@@ -209,6 +209,27 @@ def test_batch_rebind_and_replays(oracle):
         assert (got[i] == want1[i]).all(), f"rebound product {i}"
 
 
+@pytest.mark.parametrize("branches", ["1", "2", "8"])
+def test_batch_graph_branches(oracle, monkeypatch, branches):
+    """The graph's buckets on one chain or on 2 and 8 forked branches (the
+    reconstructions after the join), with schoolbook products as their own
+    segment: every replay equals the oracle."""
+    monkeypatch.setenv("NTTAMD_BATCH_BRANCHES", branches)
+    monkeypatch.setenv("NTTAMD_SCHOOL", "1")
+    la, lb, sq = batch_shapes(count=40, lo=20, hi=30000, seed=8)
+    batch = plan_batch(la, lb, sq)
+    a, b = batch_operands(la, lb, sq, seed=31)
+    want = [expected(oracle, a[i], b[i], sq[i]) for i in range(len(la))]
+    dev = DeviceBatch(batch, a, b, sq)
+    assert batch.bind(dev.pa, dev.pb, dev.po), ntt.last_error()
+    for replay in range(3):
+        assert batch.run(), ntt.last_error()
+        assert ntt.synchronize()
+        got = dev.outputs()
+        for i in range(len(la)):
+            assert (got[i] == want[i]).all(), f"branches {branches} replay {replay} product {i}"
+
+
 def test_batch_config_4096(oracle):
     """The 4,096-product uneven batch: every entry against the plain per-call
     path bit for bit and against its residue mod 2^61-1; a sample against the
