@@ -1456,12 +1456,14 @@ static bool batch_autotune(Batch& B) {
 // count. Streams keep the default priority.
 constexpr int kBatchBranches = 4;
 
-static bool issue_branched(Batch& B, hipStream_t s) {
-    // NTTAMD_BATCH_BRANCHES=1 captures the launches in one chain (tests, A/B).
+// Branch count (NTTAMD_BATCH_BRANCHES=1 captures one chain: tests, A/B).
+static int batch_branches() {
     const char* env = getenv("NTTAMD_BATCH_BRANCHES");
-    const int branches = env ? std::max(1, std::min(atoi(env), 8)) : kBatchBranches;
-    if (branches == 1) return issue_program(B.program, s);
-    const size_t nseg = B.seg.empty() ? 0 : B.seg.size() - 1;
+    return env ? std::max(1, std::min(atoi(env), 8)) : kBatchBranches;
+}
+
+// Side streams and the events of one capture, made before it starts.
+static bool prepare_branches(Batch& B, int branches, std::vector<hipEvent_t>* ev) {
     if ((int)B.side.size() != branches) {
         for (hipStream_t x : B.side) hipStreamDestroy(x);
         B.side.clear();
@@ -1471,26 +1473,30 @@ static bool issue_branched(Batch& B, hipStream_t s) {
             B.side.push_back(x);
         }
     }
-    std::vector<hipEvent_t> ev;
-    auto event = [&]() -> hipEvent_t {
+    // Two fork events and a join event per branch and fork.
+    for (int k = 0; k < 2 * (branches + 1); ++k) {
         hipEvent_t e = nullptr;
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
-        ev.push_back(e);
-        return e;
-    };
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return false;
+        ev->push_back(e);
+    }
+    return true;
+}
+
+static bool issue_branched(Batch& B, hipStream_t s, const std::vector<hipEvent_t>& ev) {
+    const size_t nseg = B.seg.empty() ? 0 : B.seg.size() - 1;
+    size_t next = 0;
     // fork: every branch waits for what s has queued so far, runs its
     // launches, and s waits for every branch.
     auto fork_join = [&](const std::vector<std::vector<const BatchLaunch*>>& work) -> bool {
-        hipEvent_t f = event();
-        if (!f || hipEventRecord(f, s) != hipSuccess) return false;
+        const hipEvent_t f = ev[next++];
+        if (hipEventRecord(f, s) != hipSuccess) return false;
         for (size_t k = 0; k < work.size(); ++k) {
+            const hipEvent_t j = ev[next++];
             if (work[k].empty()) continue;
             if (hipStreamWaitEvent(B.side[k], f, 0) != hipSuccess) return false;
             for (const BatchLaunch* L : work[k])
                 if (!issue_launch(*L, B.side[k])) return false;
-            hipEvent_t j = event();
-            if (!j || hipEventRecord(j, B.side[k]) != hipSuccess || hipStreamWaitEvent(s, j, 0) != hipSuccess)
-                return false;
+            if (hipEventRecord(j, B.side[k]) != hipSuccess || hipStreamWaitEvent(s, j, 0) != hipSuccess) return false;
         }
         return true;
     };
@@ -1498,28 +1504,33 @@ static bool issue_branched(Batch& B, hipStream_t s) {
     std::vector<std::vector<const BatchLaunch*>> work(B.side.size());
     for (size_t i = 0; i < nseg; ++i)
         for (size_t l = B.seg[i]; l < B.seg[i + 1]; ++l) work[i % work.size()].push_back(&B.program[l]);
-    bool ok = fork_join(work);
-    if (ok) {
-        std::vector<std::vector<const BatchLaunch*>> crt(B.side.size());
-        const size_t first = B.seg.empty() ? 0 : B.seg.back();
-        for (size_t l = first; l < B.program.size(); ++l) crt[(l - first) % crt.size()].push_back(&B.program[l]);
-        ok = fork_join(crt);
-    }
-    for (hipEvent_t e : ev) hipEventDestroy(e);
-    return ok;
+    if (!fork_join(work)) return false;
+    std::vector<std::vector<const BatchLaunch*>> crt(B.side.size());
+    const size_t first = B.seg.empty() ? 0 : B.seg.back();
+    for (size_t l = first; l < B.program.size(); ++l) crt[(l - first) % crt.size()].push_back(&B.program[l]);
+    return fork_join(crt);
 }
 
 static bool batch_capture(Batch& B) {
     const hipStream_t s = B.ex.stream;
+    const int branches = batch_branches();
+    std::vector<hipEvent_t> ev;
+    if (branches > 1 && !prepare_branches(B, branches, &ev)) {
+        for (hipEvent_t e : ev) hipEventDestroy(e);
+        set_error("batch: stream or event creation failed");
+        return false;
+    }
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        for (hipEvent_t e : ev) hipEventDestroy(e);
         set_error("batch: graph capture failed to start");
         return false;
     }
     t_capturing = true;
-    const bool ok = issue_branched(B, s);
+    const bool ok = branches > 1 ? issue_branched(B, s, ev) : issue_program(B.program, s);
     t_capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
+    for (hipEvent_t x : ev) hipEventDestroy(x);
     if (!ok || e != hipSuccess || !g) {
         if (g) hipGraphDestroy(g);
         set_error("batch: graph capture failed");
