@@ -194,6 +194,17 @@ def mod_q61(limbs) -> int:
     return int.from_bytes(np.ascontiguousarray(limbs, dtype="<u8").tobytes(), "little") % q
 
 
+# The reference's published single-core figures for the same workloads
+# (README.md:28 and :36 of the reference: one Zen 5 core at 5.09 GHz, AVX-512
+# IFMA, MulWorkspace / ConvWorkspace with retained tables), as operand length
+# over time.
+PUBLISHED = {
+    "mul_2p20": {"value": 24.2, "unit": "Mlimbs/s", "time_ms": 43.4,
+                 "source": "reference README.md:28, one Zen 5 core (AVX-512 IFMA)"},
+    "conv16_2p22": {"value": 31.3, "unit": "Mcoeff/s", "time_ms": 134.0,
+                    "source": "reference README.md:36, one Zen 5 core (AVX-512 IFMA)"},
+}
+
 SMALL_SHAPES = [(64, 64), (192, 192), (384, 384), (512, 512), (8, 200000), (192, 65536)]
 
 
@@ -631,6 +642,7 @@ def main() -> int:
 
     unit = "Mcoeff/s" if kind == "conv" else "Mlimbs/s"
     value = la / (ms_clean * 1e-3) / 1e6
+    published = PUBLISHED.get(args.workload)
     result = {
         "metric": "Mlimbs/s at 2^20-limb mul" if args.workload == "mul_2p20" else f"{unit} ({args.workload})",
         "value": round(value, 2),
@@ -642,7 +654,8 @@ def main() -> int:
         "ms_per_step_with_events": round(ms_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": round(value / published["value"], 1) if published else None,
+        "baseline": published,
         "dtype": "f64",
         "data": "synthetic (SplitMix64 limbs, seeds 1 and 2), operands resident in HBM",
         "config": {"workload": args.workload, "len_a": la, "len_b": lb, "primes": primes,
