@@ -173,7 +173,6 @@ struct CrtArgs {
     uint32_t* agg_thresh;    // T_end of each block
     uint32_t* first_thresh;  // t of the block's first limb
     uint32_t* ripple_end;    // last limb index (relative) reached by a ripple
-    uint32_t* carry_in;      // carry into each block, from the scan
     uint32_t blocks;
     uint32_t pad;
     // Batched reconstruction: items[blockIdx.x] names product and block.

@@ -665,7 +665,8 @@ static bool inverse_tail(const Split& sp, const PassArgs& a0, int primes, hipStr
 
 // Device working memory of one product or convolution: the spectra rows
 // (primes x row_len doubles each; spec_b unused for a square) and the carry
-// words of the reconstruction (5 per output block).
+// words of the reconstruction (4 per output block: carry function base and
+// threshold, first-limb threshold, ripple end).
 uint64_t op_spectra_bytes(const Geometry& g) {
     return g.school ? 0 : (uint64_t)g.primes * row_len(g) * 8 * (g.blocks ? g.blocks : 1);
 }
@@ -674,7 +675,7 @@ uint64_t carry_blocks(uint64_t out_len, bool school) {
     const uint64_t bl = school ? kSchoolBlockLimbs : kCrtBlockLimbs;
     return (out_len + bl - 1) / bl;
 }
-uint64_t op_carry_bytes(uint64_t out_len, bool school) { return carry_blocks(out_len, school) * 5 * 4; }
+uint64_t op_carry_bytes(uint64_t out_len, bool school) { return carry_blocks(out_len, school) * 4 * 4; }
 
 // Root-table entries a geometry reads: T/2 for a row of 2^K points; a
 // composite row reaches nodes below 7N/8 of the table of 2N points.
@@ -845,7 +846,6 @@ static bool crt_product_from(hipStream_t s, const Geometry& g, const double* row
     a.agg_thresh = carry + blocks;
     a.first_thresh = carry + 2 * blocks;
     a.ripple_end = carry + 3 * blocks;
-    a.carry_in = carry + 4 * blocks;
     return launch_crt_product(a, s);
 }
 
@@ -863,7 +863,6 @@ static bool school_on(hipStream_t s, const uint64_t* d_a, uint64_t la, const uin
     a.agg_thresh = carry + blocks;
     a.first_thresh = carry + 2 * blocks;
     a.ripple_end = carry + 3 * blocks;
-    a.carry_in = carry + 4 * blocks;
     a.op_a = d_a;
     a.op_b = d_b;
     a.len_a = la;
