@@ -175,9 +175,14 @@ struct CrtArgs {
     uint32_t* ripple_end;    // last limb index (relative) reached by a ripple
     uint32_t blocks;
     uint32_t pad;
-    // Batched reconstruction: items[blockIdx.x] names product and block.
+    // Batched reconstruction: items[blockIdx.x] names product and block;
+    // scan_items (scan_count of them) name the products whose carries go
+    // through k_carry_scan (those of more than kCarryLookBackBlocks blocks).
     const BatchProduct* batch;
     const BatchItem* items;
+    const BatchItem* scan_items;
+    uint32_t scan_count;
+    uint32_t pad2;
     // Schoolbook products: the operand limbs (single operation; a batch
     // reads them from src_a and src_b of each product).
     const uint64_t* op_a;
@@ -216,6 +221,12 @@ struct RecombineArgs {
 };
 
 constexpr uint32_t kInf = 0xFFFFFFFFu;
+// Carries across the blocks of an output: up to kCarryLookBackBlocks blocks
+// each block looks back over the published block functions itself (at most
+// that many steps); longer outputs first run one scan over all of them.
+constexpr uint32_t kCarryLookBackBlocks = 64;
+constexpr int kCarryScanThreads = 1024;
+__host__ __device__ __forceinline__ bool carry_scanned(uint32_t blocks) { return blocks > kCarryLookBackBlocks; }
 constexpr int kCrtThreads = 256;
 constexpr int kCrtLimbsPerThread = 4;
 constexpr int kCrtBlockLimbs = kCrtThreads * kCrtLimbsPerThread;

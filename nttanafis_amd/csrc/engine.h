@@ -201,6 +201,9 @@ struct Batch {
     // Reconstruction items per prime count; index 0 holds the blocks of the
     // schoolbook products.
     uint32_t crt_begin[kMaxPrimes + 1] = {0}, crt_count[kMaxPrimes + 1] = {0};
+    // Products of each prime count whose carries take the scan (more than
+    // kCarryLookBackBlocks output blocks): one item each.
+    uint32_t scan_begin[kMaxPrimes + 1] = {0}, scan_count[kMaxPrimes + 1] = {0};
     DevBuf products, items, arena, carry;
     std::vector<BatchLaunch> program;
     // program[seg[i], seg[i+1]) for i < seg.size() - 1 are independent

@@ -549,7 +549,7 @@ Split split_of(int K) {
     int rows = t_force_rows;
     if (!rows)
         if (const char* e = getenv("NTTAMD_ROW_LOG")) rows = atoi(e);
-    if (!rows) {
+    if (!rows && K <= kMaxLog) {
         std::lock_guard<std::mutex> g(g_tune_lock);
         rows = g_tuned_row_log[K];
     }
@@ -924,6 +924,7 @@ static bool device_product(Executor& ex, const Geometry& g, const uint64_t* d_a,
 // buffers (operands of the plan's shape, contents irrelevant to the timing)
 // and records the fastest for every later call of that length.
 static void tune_split(Executor& ex, int K, const std::function<bool()>& run) {
+    if (K < 0 || K > kMaxLog) return;
     {
         std::lock_guard<std::mutex> g(g_tune_lock);
         if (g_tuned_row_log[K]) return;
@@ -1223,6 +1224,8 @@ static void build_program(Batch& B) {
         L.crt.batch = B.products.as<BatchProduct>();
         L.crt.items = I + B.crt_begin[np];
         L.crt.blocks = B.crt_count[np];
+        L.crt.scan_items = I + B.scan_begin[np];
+        L.crt.scan_count = B.scan_count[np];
         B.program.push_back(L);
     }
 }
@@ -1349,6 +1352,13 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
             if (B.geo[i].school ? np == 0 : (int)B.geo[i].primes == np)
                 for (uint32_t blk = 0; blk < B.host[i].blocks; ++blk) items.push_back({(uint32_t)i, blk});
         B.crt_count[np] = (uint32_t)items.size() - B.crt_begin[np];
+    }
+    for (int np = 0; np <= kMaxPrimes; ++np) {
+        B.scan_begin[np] = (uint32_t)items.size();
+        for (size_t i = 0; i < count; ++i)
+            if ((B.geo[i].school ? np == 0 : (int)B.geo[i].primes == np) && carry_scanned(B.host[i].blocks))
+                items.push_back({(uint32_t)i, 0});
+        B.scan_count[np] = (uint32_t)items.size() - B.scan_begin[np];
     }
     if (!B.items.ensure(std::max<size_t>(items.size(), 1) * sizeof(BatchItem)) ||
         (!items.empty() && hipMemcpy(B.items.ptr, items.data(), items.size() * sizeof(BatchItem),
@@ -1834,8 +1844,8 @@ bool ntt_conv_plan_new(size_t len_a, size_t len_b, uint64_t max_a, uint64_t max_
         }
     if (!np) return false;
     const uint64_t conv = (uint64_t)len_a + len_b - 1;
+    if (conv > ((uint64_t)1 << kMaxRootOrder)) return false;
     const int K = std::max(kMinLog, log2_ceil(conv));
-    if (K > kMaxRootOrder - 1) return false;
     if (conv > UINT64_MAX / stride) return false;
     Geometry row;
     row.primes = (uint32_t)np;
@@ -1847,6 +1857,13 @@ bool ntt_conv_plan_new(size_t len_a, size_t len_b, uint64_t max_a, uint64_t max_
     // disjoint blocks modulo their width; here the residues of the blocks
     // covering a coefficient are summed before the CRT).
     choose_split(&row, kLaunchPrice);
+    // Transforms of at most 2^kMaxLog points (the pass kernels and the
+    // per-length split table cover no more); a composite row of 2^kMaxLog
+    // cyclic points still serves a slightly longer convolution.
+    if (row.log_t > kMaxLog) {
+        set_error("convolution longer than the largest transform");
+        return false;
+    }
     ntt_conv_plan p;
     memset(&p, 0, sizeof p);
     p.len_a = len_a;

@@ -1181,12 +1181,78 @@ __global__ __launch_bounds__(kSchoolThreads) void k_school(CrtArgs a) {
 }
 
 // Carry into every block and its application. Block b's carry in is
-// F_{b-1}(F_{b-2}(... F_0(0))) for the published carry functions F. A block
-// whose function has no threshold outputs its base whatever comes in, so the
-// look-back stops at the nearest such block and applies the functions after
-// it. Then the first limb takes the carry, and the limbs up to the recorded
-// ripple end take one more when the carry reaches the first limb's threshold.
+// F_{b-1}(F_{b-2}(... F_0(0))) for the published carry functions F (the block
+// merge of src/limb/reconstruction.rs:211-247).
+//
+// An output of more than kCarryLookBackBlocks blocks first runs k_carry_scan:
+// one workgroup composes the functions of consecutive chunks of blocks (one
+// chunk per thread), scans the chunk aggregates, and walks each chunk again
+// replacing every block's function base by its carry in — O(blocks) work for
+// any carry pattern, all-ones products included. A shorter output skips the
+// scan: each block looks back to the nearest block whose function has no
+// threshold (its output is fixed whatever comes in) and applies the
+// functions after it, at most kCarryLookBackBlocks steps.
+//
+// Then the first limb of the block takes the carry, and the limbs up to the
+// recorded ripple end take one more when the carry reaches the first limb's
+// threshold.
 constexpr int kApplyThreads = 64;
+
+__global__ __launch_bounds__(kCarryScanThreads) void k_carry_scan(CrtArgs a) {
+    constexpr int WAVES = kCarryScanThreads / 64;
+    __shared__ CarryFn wave_total[WAVES];
+    uint32_t *base, *thresh;
+    uint32_t n;
+    if (a.batch == nullptr) {
+        base = a.agg_base;
+        thresh = a.agg_thresh;
+        n = a.blocks;
+    } else {
+        const BatchProduct& P = a.batch[a.scan_items[blockIdx.x].product];
+        base = P.carry;
+        thresh = P.carry + P.blocks;
+        n = P.blocks;
+    }
+    const uint32_t per = (n + kCarryScanThreads - 1) / kCarryScanThreads;
+    const uint32_t b0 = threadIdx.x * per;
+    const uint32_t b1 = b0 + per < n ? b0 + per : n;
+    // The threads whose chunk is not empty are a prefix of the workgroup, so
+    // the placeholder of an empty chunk only reaches threads after it, whose
+    // chunks are empty too.
+    CarryFn agg{0u, kInf};
+    for (uint32_t b = b0; b < b1; ++b) {
+        const CarryFn f{base[b], thresh[b]};
+        agg = b == b0 ? f : compose(agg, f);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    CarryFn inc = agg;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const CarryFn prev = shfl_up_fn(inc, d);
+        if (lane >= d) inc = compose(prev, inc);
+    }
+    if (lane == 63) wave_total[wave] = inc;
+    const CarryFn lane_excl = shfl_up_fn(inc, 1);
+    __syncthreads();
+    bool have = false;
+    CarryFn g{0u, kInf};
+    for (int u = 0; u < wave; ++u) {
+        g = have ? compose(g, wave_total[u]) : wave_total[u];
+        have = true;
+    }
+    if (lane > 0) {
+        g = have ? compose(g, lane_excl) : lane_excl;
+        have = true;
+    }
+    // Carry into the chunk: its exclusive prefix applied to the zero carry
+    // into block 0 (every threshold is at least 1, so F(0) is F's base).
+    uint32_t c = have ? g.base : 0u;
+    for (uint32_t b = b0; b < b1; ++b) {
+        const uint32_t fb = base[b], ft = thresh[b];
+        base[b] = c;
+        c = fb + (c >= ft ? 1u : 0u);
+    }
+}
 
 template <int BL>
 __global__ __launch_bounds__(kApplyThreads) void k_carry_apply(CrtArgs a) {
@@ -1195,38 +1261,28 @@ __global__ __launch_bounds__(kApplyThreads) void k_carry_apply(CrtArgs a) {
     const CrtJob J = crt_job(a);
     const int b = (int)J.block;
     if (b == 0) return;
-    if (threadIdx.x == 0) found = 0xFFFFFFFFu;
-    __syncthreads();
     uint32_t c = 0;
-    int end = b;  // functions of blocks [k, end) still to apply
-    int start = 0;
-    bool resolved = false;
-    while (!resolved) {
-        const int first = end - kApplyThreads;
-        const int k = end - 1 - (int)threadIdx.x;
-        if (k >= 0 && k >= first && J.agg_thresh[k] == kInf) atomicMin(&found, (uint32_t)threadIdx.x);
+    if (carry_scanned(J.blocks)) {
+        // k_carry_scan left the carry in of every block in its base word.
+        c = J.agg_base[b];
+    } else {
+        // b <= kCarryLookBackBlocks = kApplyThreads: one look-back step.
+        if (threadIdx.x == 0) found = 0xFFFFFFFFu;
+        __syncthreads();
+        const int k = b - 1 - (int)threadIdx.x;
+        if (k >= 0 && J.agg_thresh[k] == kInf) atomicMin(&found, (uint32_t)threadIdx.x);
         __syncthreads();
         const uint32_t t = found;
-        __syncthreads();
-        if (t != 0xFFFFFFFFu) {
-            start = end - 1 - (int)t;  // block whose output is fixed
-            resolved = true;
-        } else if (first <= 0) {
-            start = -1;  // no fixed block: begin with a zero carry into block 0
-            resolved = true;
-        } else {
-            end = first;
-            if (threadIdx.x == 0) found = 0xFFFFFFFFu;
-            __syncthreads();
+        // Block whose output is fixed, or -1: begin with a zero carry into block 0.
+        const int start = t != 0xFFFFFFFFu ? b - 1 - (int)t : -1;
+        if (threadIdx.x == 0) {
+            uint32_t cc = start >= 0 ? J.agg_base[start] : 0u;
+            for (int q = start + 1; q < b; ++q) cc = J.agg_base[q] + (cc >= J.agg_thresh[q] ? 1u : 0u);
+            cin_s = cc;
         }
+        __syncthreads();
+        c = cin_s;
     }
-    if (threadIdx.x == 0) {
-        uint32_t cc = start >= 0 ? J.agg_base[start] : 0u;
-        for (int k = start + 1; k < b; ++k) cc = J.agg_base[k] + (cc >= J.agg_thresh[k] ? 1u : 0u);
-        cin_s = cc;
-    }
-    __syncthreads();
-    c = cin_s;
     if (c == 0) return;
     const uint64_t L0 = (uint64_t)b * BL;
     if (threadIdx.x == 0 && L0 < J.out_len) J.out[L0] += c;
@@ -1412,6 +1468,21 @@ bool launch_tables(const TableArgs& a, int primes, hipStream_t s) {
     return NTT_CHECK_LAUNCH();
 }
 
+// Carries across the output blocks of blocks of BL limbs: the scan over
+// the block functions of every long output, then one workgroup per block.
+// In a batched launch a.blocks counts (product, block) items and
+// a.scan_count the products that take the scan.
+template <int BL>
+static bool launch_carries(const CrtArgs& a, hipStream_t s) {
+    if (a.batch ? a.scan_count > 0 : carry_scanned(a.blocks)) {
+        NTT_LAUNCH("carry_scan", k_carry_scan, dim3(a.batch ? a.scan_count : 1), dim3(kCarryScanThreads), s, a);
+        if (!NTT_CHECK_LAUNCH()) return false;
+    }
+    if (a.blocks > 1 || a.batch)
+        NTT_LAUNCH("carry_apply", k_carry_apply<BL>, dim3(a.blocks), dim3(kApplyThreads), s, a);
+    return NTT_CHECK_LAUNCH();
+}
+
 // In a batched launch a.blocks counts (product, block) items.
 bool launch_crt_product(const CrtArgs& a, hipStream_t s) {
     dim3 grid(a.blocks);
@@ -1427,9 +1498,7 @@ bool launch_crt_product(const CrtArgs& a, hipStream_t s) {
         default: return false;
     }
     if (!NTT_CHECK_LAUNCH()) return false;
-    if (a.blocks > 1 || a.batch)
-        NTT_LAUNCH("carry_apply", k_carry_apply<kCrtBlockLimbs>, dim3(a.blocks), dim3(kApplyThreads), s, a);
-    return NTT_CHECK_LAUNCH();
+    return launch_carries<kCrtBlockLimbs>(a, s);
 }
 
 // Schoolbook product blocks of kSchoolBlockLimbs limbs, then the carries
@@ -1437,9 +1506,7 @@ bool launch_crt_product(const CrtArgs& a, hipStream_t s) {
 bool launch_school(const CrtArgs& a, hipStream_t s) {
     NTT_LAUNCH("school", k_school, dim3(a.blocks), dim3(kSchoolThreads), s, a);
     if (!NTT_CHECK_LAUNCH()) return false;
-    if (a.blocks > 1 || a.batch)
-        NTT_LAUNCH("carry_apply", k_carry_apply<kSchoolBlockLimbs>, dim3(a.blocks), dim3(kApplyThreads), s, a);
-    return NTT_CHECK_LAUNCH();
+    return launch_carries<kSchoolBlockLimbs>(a, s);
 }
 
 bool launch_recombine(const RecombineArgs& a, int primes, hipStream_t s) {

@@ -150,6 +150,15 @@ def sqr(a: np.ndarray, threads: int = 1) -> np.ndarray:
     return dst
 
 
+def workspace_mul(dst: np.ndarray, a: np.ndarray, b: np.ndarray, square: bool = False, threads: int = 1) -> bool:
+    """MulWorkspace::run (src/plan.rs:182-205): an empty operand writes zeros."""
+    aa = np.ascontiguousarray(a, dtype=np.uint64)
+    bb = aa if square else np.ascontiguousarray(b, dtype=np.uint64)
+    ok = lib().oracle_workspace_mul(_p(dst), len(dst), _p(aa), len(aa), _p(bb), len(bb), 1 if square else 0,
+                                    threads)
+    return bool(ok)
+
+
 def conv_plan(len_a: int, len_b: int, max_a: int, max_b: int, signed: bool):
     plan = ConvPlan()
     if not lib().oracle_conv_plan_new(len_a, len_b, max_a, max_b, 1 if signed else 0, ctypes.byref(plan)):

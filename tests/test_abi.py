@@ -63,6 +63,20 @@ def test_conv_plan_admission(kat):
     assert ntt.ConvPlan.new(25, 1000, 5, 5, False).scratch_len() == 0
 
 
+def test_conv_plan_rejects_transforms_beyond_the_largest():
+    """Transforms are at most 2^30 points: a convolution of 2^31 - 1 output
+    coefficients has no admitted row and is refused (its plan would index the
+    per-length split table past its end), while a composite row of 2^30 cyclic
+    points still admits a slightly longer convolution than 2^30."""
+    assert ntt.ConvPlan.new(1 << 30, 1 << 30, 1, 1, False) is None
+    assert ntt.ConvPlan.new((1 << 40) + 1, 1 << 40, 1, 1, False) is None
+    n = (1 << 29) + (1 << 27)  # 2^30 + 2^28 - 1 coefficients: a 5N/4 row, N = 2^30
+    plan = ntt.ConvPlan.new(n, n, 1, 1, False)
+    assert plan is not None and plan.log_len == 30 and plan.row_tail == 1
+    plan = ntt.ConvPlan.new(1 << 29, 1 << 29, 1, 1, False)
+    assert plan is not None and plan.log_len == 30 and plan.row_tail == 0
+
+
 @pytest.mark.parametrize("la,lb", [(1, 1), (3, 700), (192, 65536), (1024, 1024), (1 << 20, 1 << 20),
                                    (1 << 24, 1 << 24), (4194304, 256), (65535, 65537)])
 def test_device_geometry_fits_capacity(kat, la, lb):

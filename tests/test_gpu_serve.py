@@ -169,3 +169,56 @@ def test_served_failures(pools):
     assert (ok == alone({"kind": MUL, "len_a": 1000, "len_b": 1000}, a, a)).all()
     assert not server.drain()  # one request failed
     server.close()
+
+
+def test_served_full_size_range(oracle):
+    """Config 5's whole size range: 48 requests with operands log-uniform in
+    [256, 4,194,304] limbs (16-bit coefficients for convolutions), 10 % sparse
+    or all-ones, all submitted at once under a 2 GiB budget. Every result
+    equals the same request alone through the per-call path; the largest
+    requests of each kind and a random sample also match the oracle."""
+    hi = 1 << 22
+    pools = operand_pools(hi)
+    reqs = served_stream(48, 0.0, lo=256, hi=hi, seed=55)
+    sizes = [max(r["len_a"], r["len_b"]) for r in reqs]
+    say(f"served full range: {len(reqs)} requests, largest operand {max(sizes)}, "
+        f"{sum(s > (1 << 18) for s in sizes)} above 2^18 limbs")
+    assert max(sizes) > (1 << 21), "the draw must reach the top of the range"
+    server = ntt.Server(budget_bytes=2 << 30, slots=4)
+    outs, tickets = [], []
+    for req in reqs:
+        a, b = request_operands(req, pools)
+        dst = np.zeros(output_len(req), dtype=np.uint64)
+        t = submit(server, req, a, b, dst)
+        assert t >= 0, ntt.last_error()
+        outs.append(dst)
+        tickets.append(t)
+    assert server.drain()
+    st = server.stats()
+    say(f"stats {st}")
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["peak_bytes"] <= st["budget_bytes"]
+    for i, req in enumerate(reqs):
+        assert server.wait(tickets[i]) == 1
+        a, b = request_operands(req, pools)
+        assert (outs[i] == alone(req, a, b)).all(), f"request {i} ({req['kind']} {req['len_a']}x{req['len_b']})"
+    say("served full range: every request equals the per-call path")
+    largest = {}
+    for i, req in enumerate(reqs):
+        k = req["kind"]
+        if k not in largest or sizes[i] > sizes[largest[k]]:
+            largest[k] = i
+    rng = np.random.default_rng(2)
+    sample = sorted(set(largest.values()) | set(rng.choice(len(reqs), size=4, replace=False).tolist()))
+    for i in sample:
+        req = reqs[i]
+        a, b = request_operands(req, pools)
+        if req["kind"] == CONV:
+            want = oracle.convolve(a, b, 0xFFFF, 0xFFFF, False, threads=16)
+        elif req["kind"] == SQR:
+            want = oracle.sqr(a, threads=16)
+        else:
+            want = oracle.mul(a, b, threads=16)
+        assert (outs[i] == want).all(), f"request {i} ({req['kind']} {req['len_a']}x{req['len_b']}) vs oracle"
+        say(f"oracle ok: request {i} {req['kind']} {req['len_a']}x{req['len_b']}")
+    server.close()

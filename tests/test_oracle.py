@@ -232,3 +232,43 @@ def test_digest_check_catches_a_planted_error(oracle, digests):
     out[123457] ^= np.uint64(1)
     with pytest.raises(AssertionError):
         check_record(out, digests["mul_2p20"])
+
+
+def test_mod_limbs_matches_integers(oracle):
+    """oracle_mod_limbs (Horner over limbs, most significant first) against
+    Python's exact remainder, for moduli of 2..64 bits and edge operands."""
+    rng = np.random.default_rng(61)
+    moduli = [2, 3, 65537, (1 << 61) - 1, (1 << 63) + 29, (1 << 64) - 59] + [oracle.prime(i) for i in range(8)]
+    operands = [np.zeros(1, dtype=np.uint64), np.full(7, MASK, dtype=np.uint64),
+                np.array([MASK, 0, 1], dtype=np.uint64)]
+    operands += [rng.integers(0, MASK, n, dtype=np.uint64, endpoint=True) for n in (1, 2, 5, 64, 1000)]
+    for q in moduli:
+        for a in operands:
+            assert oracle.mod_limbs(a, q) == to_int(a) % q, (q, len(a))
+    # A plausible slip (limbs taken least significant first) would differ.
+    a = np.array([1, 2], dtype=np.uint64)
+    assert oracle.mod_limbs(a, (1 << 61) - 1) == (1 + (2 << 64)) % ((1 << 61) - 1)
+
+
+def test_workspace_mul_empty_operand_rule(oracle):
+    """MulWorkspace::run (src/plan.rs:182-205): an empty operand gives a zero
+    result of a_len + b_len limbs and returns true; nothing beyond it is
+    written; non-empty operands give the product, the square flag squares a."""
+    empty = np.zeros(0, dtype=np.uint64)
+    a = oracle.operand(37, 5)
+    b = oracle.operand(12, 6)
+    for x, y in [(empty, b), (a, empty), (empty, empty)]:
+        n = len(x) + len(y)
+        dst = np.full(n + 3, 0xDEAD, dtype=np.uint64)
+        assert oracle.workspace_mul(dst, x, y)
+        assert (dst[:n] == 0).all() and (dst[n:] == 0xDEAD).all()
+    # Too short a destination for the zero result: false, dst untouched.
+    dst = np.full(4, 7, dtype=np.uint64)
+    assert not oracle.workspace_mul(dst, empty, b)
+    assert (dst == 7).all()
+    dst = np.zeros(len(a) + len(b), dtype=np.uint64)
+    assert oracle.workspace_mul(dst, a, b)
+    assert to_int(dst) == to_int(a) * to_int(b)
+    dst = np.zeros(2 * len(a), dtype=np.uint64)
+    assert oracle.workspace_mul(dst, a, a, square=True)
+    assert to_int(dst) == to_int(a) ** 2
