@@ -170,6 +170,23 @@ bool ntt_conv_workspace_convolve_device(ntt_conv_workspace *ws, uint64_t *d_dst,
                                         const uint64_t *d_a, size_t a_len, const uint64_t *d_b,
                                         size_t b_len, void *stream);
 
+/* Allocation-free product of device operands. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It computes exactly what ntt_mul_workspace_mul_device computes (the same
+ * slice twice squares) and calls no hipMalloc, hipFree or other allocating
+ * routine: its spectra, carry words and root tables live in one fixed device
+ * array of 32 MiB per GPU (8 MiB of root tables, 24 MiB of working memory),
+ * held from library load to process end. The first call on a device takes
+ * the array's address and generates the root tables into it (synchronizing
+ * `stream` once). Limit: transforms of at most 2^17 points and a working set
+ * within 24 MiB (every product of two operands up to 65,536 limbs fits); a
+ * larger product returns false with a message in ntt_last_error and dst
+ * unchanged. Every call on one GPU shares that GPU's array, so calls that
+ * overlap on one GPU (concurrent streams or threads) race: serialize them. */
+bool ntt_mul_device_noalloc(uint64_t *d_dst, size_t dst_len, const uint64_t *d_a, size_t a_len,
+                            const uint64_t *d_b, size_t b_len, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Batched products (no counterpart in the reference, which multiplies one
  * pair per call: MulWorkspace::mul, src/plan.rs:164-170, is what each batch

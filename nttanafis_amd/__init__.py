@@ -17,6 +17,7 @@ from .api import (  # noqa: F401
     device_available,
     pinned_array,
     last_error,
+    mul_device_noalloc,
     mul_geometry,
     mul_schoolbook,
     mul_split,
@@ -42,6 +43,7 @@ __all__ = [
     "Profile",
     "device_available",
     "last_error",
+    "mul_device_noalloc",
     "mul_geometry",
     "mul_schoolbook",
     "mul_split",

@@ -153,6 +153,10 @@ SIGNATURES = {
         ctypes.c_bool,
         [ctypes.c_void_p, ctypes.c_void_p, size_t, ctypes.c_void_p, size_t, ctypes.c_void_p, size_t, ctypes.c_void_p],
     ),
+    "ntt_mul_device_noalloc": (
+        ctypes.c_bool,
+        [ctypes.c_void_p, size_t, ctypes.c_void_p, size_t, ctypes.c_void_p, size_t, ctypes.c_void_p],
+    ),
     "ntt_batch_new": (ctypes.c_void_p, []),
     "ntt_batch_free": (None, [ctypes.c_void_p]),
     "ntt_batch_plan": (

@@ -493,6 +493,14 @@ class DeviceBuffer:
             pass
 
 
+def mul_device_noalloc(d_dst: int, dst_len: int, d_a: int, a_len: int, d_b: int, b_len: int,
+                       stream: Optional[int] = None) -> bool:
+    """Product of device operands that allocates no device memory (its
+    working memory is the library's fixed 32 MiB device array; products of
+    transforms up to 2^17 points). The same address and length twice squares."""
+    return bool(_lib().ntt_mul_device_noalloc(d_dst, dst_len, d_a, a_len, d_b, b_len, stream))
+
+
 def device_available() -> bool:
     return bool(_lib().ntt_device_available())
 

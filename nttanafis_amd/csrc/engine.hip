@@ -519,9 +519,21 @@ static Split split_with_rows(int K, int rows) {
 }
 
 // Row-pass sizes whose strided passes stay within 5..11 levels, largest
-// first (at most four).
-static std::vector<int> row_candidates(int K) {
-    std::vector<int> c;
+// first (at most four), in a fixed array (split_of runs on the
+// allocation-free path too).
+struct RowCandidates {
+    int n = 0;
+    int r[4] = {0, 0, 0, 0};
+    const int* begin() const { return r; }
+    const int* end() const { return r + n; }
+    bool empty() const { return n == 0; }
+    size_t size() const { return (size_t)n; }
+    int operator[](int i) const { return r[i]; }
+    void push_back(int v) { r[n++] = v; }
+};
+
+static RowCandidates row_candidates(int K) {
+    RowCandidates c;
     if (K <= kSinglePassMaxLog) return c;
     for (int r = 12; r >= 5 && c.size() < 4; --r) {
         const int rest = K - r;
@@ -545,7 +557,7 @@ static thread_local int t_force_rows = 0;  // set while the autotuner times a ca
 
 Split split_of(int K) {
     if (K <= kSinglePassMaxLog) return split_with_rows(K, K);
-    const std::vector<int> cand = row_candidates(K);
+    const RowCandidates cand = row_candidates(K);
     int rows = t_force_rows;
     if (!rows)
         if (const char* e = getenv("NTTAMD_ROW_LOG")) rows = atoi(e);
@@ -788,14 +800,9 @@ static bool split_residues(hipStream_t s, const Geometry& g, const Source& sa, c
 
 // Computes the canonical residues of the convolution into buf.spec_b (or
 // spec_a for a square or a split) and returns their pointer.
-static bool residues(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint64_t la, const uint64_t* d_b,
-                     uint64_t lb, bool square, const OpBuffers& buf, double** rows_out) {
-    const double *fwd = nullptr, *inv = nullptr;
-    uint64_t tstride = 0;
-    if (!ensure_tables(table_need(g), s, &fwd, &inv, &tstride)) {
-        set_error("root table allocation failed");
-        return false;
-    }
+static bool residues_with(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint64_t la, const uint64_t* d_b,
+                          uint64_t lb, bool square, const OpBuffers& buf, const double* fwd, const double* inv,
+                          uint64_t tstride, double** rows_out) {
     const Source sa = source_of(d_a, la, g.digits_a, g.width, g.value_bits_a, g.is_signed);
     const Source sb = source_of(d_b, lb, g.digits_b, g.width, g.value_bits_b, g.is_signed);
     if (g.blocks && !square) return split_residues(s, g, sa, sb, buf, fwd, inv, tstride, rows_out);
@@ -807,6 +814,18 @@ static bool residues(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint
     if (g.tail && !recombine(s, g, rows)) return false;
     *rows_out = rows;
     return true;
+}
+
+// The same with the shared root tables, grown as the geometry needs.
+static bool residues(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint64_t la, const uint64_t* d_b,
+                     uint64_t lb, bool square, const OpBuffers& buf, double** rows_out) {
+    const double *fwd = nullptr, *inv = nullptr;
+    uint64_t tstride = 0;
+    if (!ensure_tables(table_need(g), s, &fwd, &inv, &tstride)) {
+        set_error("root table allocation failed");
+        return false;
+    }
+    return residues_with(s, g, d_a, la, d_b, lb, square, buf, fwd, inv, tstride, rows_out);
 }
 
 static CrtArgs crt_args(const Geometry& g, const double* rows, uint64_t* out, uint64_t out_len) {
@@ -870,15 +889,25 @@ static bool school_on(hipStream_t s, const uint64_t* d_a, uint64_t la, const uin
     return launch_school(a, s);
 }
 
-bool product_on(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint64_t la, const uint64_t* d_b,
-                uint64_t lb, bool square, const OpBuffers& buf, uint64_t* d_out) {
+// tables: root tables of at least table_need(g) entries per prime, or null
+// for the shared tables.
+static bool product_with(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint64_t la, const uint64_t* d_b,
+                         uint64_t lb, bool square, const OpBuffers& buf, const double* const* tables,
+                         uint64_t tstride, uint64_t* d_out) {
     if (g.school) {
         if (std::min(la, lb) > (uint64_t)kSchoolMaxShort) return false;
         return school_on(s, d_a, la, square ? d_a : d_b, square ? la : lb, buf.carry, d_out);
     }
     double* rows = nullptr;
-    if (!residues(s, g, d_a, la, d_b, lb, square, buf, &rows)) return false;
+    const bool ok = tables ? residues_with(s, g, d_a, la, d_b, lb, square, buf, tables[0], tables[1], tstride, &rows)
+                           : residues(s, g, d_a, la, d_b, lb, square, buf, &rows);
+    if (!ok) return false;
     return crt_product_from(s, g, rows, la + lb, buf.carry, d_out);
+}
+
+bool product_on(hipStream_t s, const Geometry& g, const uint64_t* d_a, uint64_t la, const uint64_t* d_b,
+                uint64_t lb, bool square, const OpBuffers& buf, uint64_t* d_out) {
+    return product_with(s, g, d_a, la, d_b, lb, square, buf, nullptr, 0, d_out);
 }
 
 bool convolution_on(hipStream_t s, const Geometry& g, const ntt_conv_plan* p, const uint64_t* d_a,
@@ -929,7 +958,7 @@ static void tune_split(Executor& ex, int K, const std::function<bool()>& run) {
         std::lock_guard<std::mutex> g(g_tune_lock);
         if (g_tuned_row_log[K]) return;
     }
-    const std::vector<int> cand = row_candidates(K);
+    const RowCandidates cand = row_candidates(K);
     int best = cand.empty() ? K : cand[0];
     if (cand.size() > 1) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1423,7 +1452,7 @@ static bool batch_autotune(Batch& B) {
             std::lock_guard<std::mutex> g(g_tune_lock);
             if (g_tuned_row_log[r.K]) continue;
         }
-        const std::vector<int> cand = row_candidates(r.K);
+        const RowCandidates cand = row_candidates(r.K);
         int best = cand.empty() ? r.K : cand[0];
         if (cand.size() > 1) {
             float best_ms = 1e30f;
@@ -1694,6 +1723,84 @@ int garner_case(const Catalog& c, Rng& rng, bool balanced) {
     return fails ? 1 : 0;
 }
 }  // namespace
+}  // namespace nttamd
+
+namespace nttamd {
+
+// ---------------------------------------------------------------------------
+// Allocation-free products (ntt_mul_device_noalloc): every device byte the
+// call needs beyond its operands and result lives in one file-scope device
+// array. Each GPU holds its own copy from the moment the library's code
+// object is loaded until the process ends. Its first kNoAllocTableBytes hold
+// the forward and inverse root tables of all eight primes (kNoAllocTableLen
+// entries each: transforms of up to 2^17 points), generated once per device
+// by the first call there; the rest holds one product's spectra and carry
+// words. Every word of that working part is written before it is read
+// (packing writes whole rows, the reconstruction writes every carry word),
+// so nothing has to be cleared between calls.
+constexpr uint64_t kNoAllocTableLen = (uint64_t)1 << 16;
+constexpr uint64_t kNoAllocTableBytes = 2ull * kMaxPrimes * kNoAllocTableLen * sizeof(double);  // 8 MiB
+constexpr uint64_t kNoAllocWorkBytes = 24ull << 20;
+constexpr uint64_t kNoAllocScratchBytes = kNoAllocTableBytes + kNoAllocWorkBytes;
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) uint64_t scratch[kNoAllocScratchBytes / 8];
+
+// Device address of `scratch` on each device (the host name of the array is
+// only a stand-in), taken once per device with that device current, and
+// whether that device's root tables are in place.
+struct NoAllocDevice {
+    uint8_t* base = nullptr;
+    bool tables = false;
+};
+static std::mutex g_noalloc_lock;
+static NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+static uint64_t align256(uint64_t n) { return (n + 255) & ~(uint64_t)255; }
+
+// Set-up on the calling thread's current device: the scratch address, and
+// the root tables generated into it (synchronously, so that calls on any
+// stream find them complete).
+static bool noalloc_device(hipStream_t s, uint8_t** base) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices) {
+        set_error("ntt_mul_device_noalloc: no usable current device");
+        return false;
+    }
+    std::lock_guard<std::mutex> guard(g_noalloc_lock);
+    NoAllocDevice& d = g_noalloc[dev];
+    if (!d.base) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            set_error("ntt_mul_device_noalloc: the scratch array has no device address");
+            return false;
+        }
+        d.base = static_cast<uint8_t*>(p);
+    }
+    if (!d.tables) {
+        const Catalog& c = cat();
+        TableArgs a;
+        memset(&a, 0, sizeof a);
+        a.fwd = reinterpret_cast<double*>(d.base);
+        a.inv = a.fwd + kMaxPrimes * kNoAllocTableLen;
+        a.table_stride = kNoAllocTableLen;
+        a.len = kNoAllocTableLen;
+        a.field = c.field;
+        memcpy(a.roots, c.roots, sizeof a.roots);
+        memcpy(a.inv_roots, c.inv_roots, sizeof a.inv_roots);
+        {
+            Timed t("tables", s);
+            if (!launch_tables(a, kMaxPrimes, s) || hipStreamSynchronize(s) != hipSuccess) {
+                set_error("ntt_mul_device_noalloc: root table generation failed");
+                return false;
+            }
+        }
+        d.tables = true;
+    }
+    *base = d.base;
+    return true;
+}
+
 }  // namespace nttamd
 
 // ---------------------------------------------------------------------------
@@ -1984,6 +2091,37 @@ bool ntt_conv_workspace_convolve_device(ntt_conv_workspace* ws, uint64_t* d_dst,
     const bool ok = run_conv(ws->ex, &ctx, d_a, d_b, d_dst);
     ws->ex.stream = saved;
     return ok;
+}
+
+bool ntt_mul_device_noalloc(uint64_t* d_dst, size_t dst_len, const uint64_t* d_a, size_t a_len,
+                            const uint64_t* d_b, size_t b_len, void* stream) {
+    t_error.clear();
+    if (!d_dst || !d_a || !d_b || a_len == 0 || b_len == 0) return false;
+    if ((uint64_t)a_len + b_len < a_len || dst_len < (uint64_t)a_len + b_len) return false;
+    const bool square = d_a == d_b && a_len == b_len;
+    Geometry g;
+    if (!select_product(a_len, b_len, &g)) return false;
+    const uint64_t spec_a = g.school ? 0 : align256(op_spectra_bytes(g));
+    const uint64_t spec_b = g.school || square ? 0 : align256(op_spectra_bytes_b(g));
+    const uint64_t carry = align256(op_carry_bytes((uint64_t)a_len + b_len, g.school));
+    if ((!g.school && table_need(g) > kNoAllocTableLen) || spec_a + spec_b + carry > kNoAllocWorkBytes) {
+        set_error("ntt_mul_device_noalloc: a product of " + std::to_string(a_len) + " x " + std::to_string(b_len) +
+                  " limbs needs " + std::to_string(spec_a + spec_b + carry) + " bytes of working memory and " +
+                  std::to_string(g.school ? 0 : table_need(g)) + " root-table entries; the fixed scratch holds " +
+                  std::to_string(kNoAllocWorkBytes) + " bytes and " + std::to_string(kNoAllocTableLen) + " entries");
+        return false;
+    }
+    const hipStream_t s = (hipStream_t)stream;
+    uint8_t* base = nullptr;
+    if (!noalloc_device(s, &base)) return false;
+    const double* tables[2] = {reinterpret_cast<const double*>(base),
+                               reinterpret_cast<const double*>(base) + kMaxPrimes * kNoAllocTableLen};
+    uint8_t* work = base + kNoAllocTableBytes;
+    OpBuffers buf;
+    buf.spec_a = reinterpret_cast<double*>(work);
+    buf.spec_b = square ? buf.spec_a : reinterpret_cast<double*>(work + spec_a);
+    buf.carry = reinterpret_cast<uint32_t*>(work + spec_a + spec_b);
+    return product_with(s, g, d_a, a_len, d_b, b_len, square, buf, tables, kNoAllocTableLen, d_dst);
 }
 
 // Batched products.
