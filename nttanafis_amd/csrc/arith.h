@@ -32,10 +32,12 @@ __host__ __device__ __forceinline__ uint64_t window64(const uint64_t* limbs, uin
     return (lo >> off) | ((hi << 1) << (63 - off));
 }
 
-// Parts of a coefficient as doubles: a part below 2^49 converts exactly from
-// its two 32-bit halves.
+// Parts of a coefficient as doubles: a part v below 2^52 sits in the low
+// mantissa bits of 2^52 + v, so that bit pattern minus 2^52 is v exactly (an
+// OR and one FP64 subtraction instead of two conversions and an FMA; the
+// biased conversion of src/limb/packing.rs:224-230).
 __host__ __device__ __forceinline__ double part_to_double(uint64_t v) {
-    return (double)(uint32_t)(v >> 32) * 4294967296.0 + (double)(uint32_t)v;
+    return __builtin_bit_cast(double, v | 0x4330000000000000ull) - 4503599627370496.0;
 }
 
 // One limb per coefficient (width 64): at most two parts, both in the limb;
@@ -180,6 +182,54 @@ __host__ __device__ __forceinline__ double pack_as(const Source& s, uint64_t n, 
     if constexpr (KIND == kPackW64Signed) return pack_w64<true, CLAMP>(s, n, pw, p, pinv);
     if constexpr (KIND >= kPack1) return pack_parts<KIND - kPack1 + 1, CLAMP>(s, n, pw, p, pinv);
     return 0.0;
+}
+
+// Parts of one coefficient, extracted once and combined per prime (packing
+// shared across the primes of a tile, as the reference's digit tiles are,
+// src/limb/packing.rs:207-218 and src/arch/schedule/expression/packing.rs:31-60).
+// Whole-limb kinds keep two parts: the low 49 bits and the rest (sign-extended
+// for a signed coefficient), or the whole value and 0 when it has at most 49
+// bits. General widths keep ceil(value_bits / 49) parts.
+template <int KIND>
+struct PackParts {
+    static constexpr int value = (KIND == kPackW64 || KIND == kPackW64Signed) ? 2 : KIND - kPack1 + 1;
+};
+
+template <int KIND, bool CLAMP>
+__host__ __device__ __forceinline__ void extract_as(const Source& s, uint64_t n, double* d) {
+    if constexpr (KIND == kPackW64 || KIND == kPackW64Signed) {
+        constexpr bool SIGNED = KIND == kPackW64Signed;
+        const uint32_t vb = s.value_bits;
+        const uint64_t last = s.limb_len - 1;
+        const uint64_t limb = CLAMP ? (n < s.digits ? s.limbs[n < last ? n : last] : 0) : s.limbs[n];
+        if (vb > (uint32_t)kPartBits) {
+            const int high_bits = (int)vb - kPartBits;
+            d[0] = part_to_double(limb & ((1ull << kPartBits) - 1));
+            const uint64_t masked = (limb >> kPartBits) & ((1ull << high_bits) - 1);
+            d[1] = SIGNED ? (double)((int64_t)(masked ^ (1ull << (high_bits - 1))) - (int64_t)(1ull << (high_bits - 1)))
+                          : part_to_double(masked);
+        } else {
+            const uint64_t masked = limb & ((1ull << vb) - 1);
+            d[0] = SIGNED ? (double)((int64_t)(masked ^ (1ull << (vb - 1))) - (int64_t)(1ull << (vb - 1)))
+                          : part_to_double(masked);
+            d[1] = 0.0;
+        }
+    } else {
+        extract_parts<KIND - kPack1 + 1, CLAMP>(s, n, d);
+    }
+}
+
+// Residue of a coefficient from its extracted parts: below 9p/16 in
+// magnitude after the one reduction, or the part itself (below 2^49 < p)
+// when there is only one.
+template <int KIND>
+__host__ __device__ __forceinline__ double combine_as(const Source& s, const double* d, const double* pw, double p,
+                                                      double pinv) {
+    if constexpr (KIND == kPackW64 || KIND == kPackW64Signed) {
+        return s.value_bits > (uint32_t)kPartBits ? reduce(d[0] + mulmod(d[1], pw[1], p, pinv), p, pinv) : d[0];
+    } else {
+        return combine_parts<KIND - kPack1 + 1>(d, pw, p, pinv);
+    }
 }
 
 // Number of leading coefficients whose window lies inside the operand and

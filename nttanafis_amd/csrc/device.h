@@ -70,7 +70,7 @@ struct BatchProduct {
     uint32_t width_magic;
     uint32_t value_bits;
     uint32_t blocks;         // carry blocks of the output
-    uint32_t* carry;         // 4 * blocks: base, thresh, first thresh, ripple end
+    uint32_t* carry;         // carry_words(blocks): base, thresh, first thresh, ripple end; chunk functions
     // Unbalanced split (as in PassArgs): block z of the longer operand (src_a)
     // owns rows at spec_a + z*block_stride; the shorter operand (src_b) is
     // transformed once into spec_b. 0: no split.
@@ -176,8 +176,9 @@ struct CrtArgs {
     uint32_t blocks;
     uint32_t pad;
     // Batched reconstruction: items[blockIdx.x] names product and block;
-    // scan_items (scan_count of them) name the products whose carries go
-    // through k_carry_scan (those of more than kCarryLookBackBlocks blocks).
+    // scan_items (scan_count of them) name the (product, chunk) pairs of the
+    // products whose carries go through k_carry_scan (those of more than
+    // kCarryLookBackBlocks blocks).
     const BatchProduct* batch;
     const BatchItem* items;
     const BatchItem* scan_items;
@@ -223,10 +224,20 @@ struct RecombineArgs {
 constexpr uint32_t kInf = 0xFFFFFFFFu;
 // Carries across the blocks of an output: up to kCarryLookBackBlocks blocks
 // each block looks back over the published block functions itself (at most
-// that many steps); longer outputs first run one scan over all of them.
+// that many steps); longer outputs first scan their block functions in chunks
+// of kCarryChunk blocks, one workgroup per chunk.
 constexpr uint32_t kCarryLookBackBlocks = 64;
-constexpr int kCarryScanThreads = 1024;
+constexpr int kCarryChunk = 1024;
 __host__ __device__ __forceinline__ bool carry_scanned(uint32_t blocks) { return blocks > kCarryLookBackBlocks; }
+__host__ __device__ __forceinline__ uint32_t carry_chunks(uint32_t blocks) {
+    return (blocks + kCarryChunk - 1) / kCarryChunk;
+}
+// Carry bookkeeping words of an output of `blocks` carry blocks: four per
+// block (function base and threshold, first-limb threshold, ripple end), then
+// two per chunk (the chunk's composed function).
+__host__ __device__ __forceinline__ uint64_t carry_words(uint32_t blocks) {
+    return 4ull * blocks + 2ull * carry_chunks(blocks);
+}
 constexpr int kCrtThreads = 256;
 constexpr int kCrtLimbsPerThread = 4;
 constexpr int kCrtBlockLimbs = kCrtThreads * kCrtLimbsPerThread;

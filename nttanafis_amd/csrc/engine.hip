@@ -21,6 +21,7 @@ namespace nttamd {
 typedef unsigned __int128 u128;
 
 bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, bool pack, hipStream_t s);
+bool launch_pack_primes(int logl, const PassArgs& a, int primes, int operands, hipStream_t s);
 bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, hipStream_t s);
 bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, hipStream_t s);
 bool launch_crt_product(const CrtArgs& a, hipStream_t s);
@@ -519,11 +520,11 @@ static Split split_with_rows(int K, int rows) {
 }
 
 // Row-pass sizes whose strided passes stay within 5..11 levels, largest
-// first (at most four), in a fixed array (split_of runs on the
+// first (at most eight), in a fixed array (split_of runs on the
 // allocation-free path too).
 struct RowCandidates {
     int n = 0;
-    int r[4] = {0, 0, 0, 0};
+    int r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int* begin() const { return r; }
     const int* end() const { return r + n; }
     bool empty() const { return n == 0; }
@@ -535,7 +536,7 @@ struct RowCandidates {
 static RowCandidates row_candidates(int K) {
     RowCandidates c;
     if (K <= kSinglePassMaxLog) return c;
-    for (int r = 12; r >= 5 && c.size() < 4; --r) {
+    for (int r = 12; r >= 5 && c.size() < 8; --r) {
         const int rest = K - r;
         if (K <= 23) {
             if (rest < 5 || rest > 11) continue;
@@ -646,6 +647,14 @@ static Source source_of(const uint64_t* limbs, uint64_t len, uint64_t digits, ui
 // Strided forward passes of the transform (all but the row pass). With
 // operands == 2 every launch covers both operands: a0.data with source sa and
 // a0.other with source sb.
+// The first pass packs the limbs of each tile once for every prime
+// (k_pack_primes) unless NTTAMD_PACK_SHARED=0 or the pass cannot take that
+// form; then one workgroup per tile and prime packs its own residues.
+static bool pack_shared() {
+    const char* env = getenv("NTTAMD_PACK_SHARED");
+    return !(env && env[0] == '0');
+}
+
 static bool forward_head(const Split& sp, const PassArgs& a0, const Source& sa, const Source& sb, int operands,
                          int primes, hipStream_t s) {
     PassArgs a = a0;
@@ -656,7 +665,8 @@ static bool forward_head(const Split& sp, const PassArgs& a0, const Source& sa, 
         a.level_start = level;
         a.line_stride = (uint64_t)1 << (a.log_t - level - sp.logs[i]);
         Timed t(i == 0 ? "fwd_pack_strided" : "fwd_strided", s);
-        if (!launch_fwd_strided(sp.logs[i], a, primes, operands, i == 0, s)) return false;
+        const bool shared = i == 0 && pack_shared() && launch_pack_primes(sp.logs[i], a, primes, operands, s);
+        if (!shared && !launch_fwd_strided(sp.logs[i], a, primes, operands, i == 0, s)) return false;
         level += sp.logs[i];
     }
     return true;
@@ -687,7 +697,7 @@ uint64_t carry_blocks(uint64_t out_len, bool school) {
     const uint64_t bl = school ? kSchoolBlockLimbs : kCrtBlockLimbs;
     return (out_len + bl - 1) / bl;
 }
-uint64_t op_carry_bytes(uint64_t out_len, bool school) { return carry_blocks(out_len, school) * 4 * 4; }
+uint64_t op_carry_bytes(uint64_t out_len, bool school) { return carry_words((uint32_t)carry_blocks(out_len, school)) * 4; }
 
 // Root-table entries a geometry reads: T/2 for a row of 2^K points; a
 // composite row reaches nodes below 7N/8 of the table of 2N points.
@@ -954,6 +964,7 @@ static bool device_product(Executor& ex, const Geometry& g, const uint64_t* d_a,
 // and records the fastest for every later call of that length.
 static void tune_split(Executor& ex, int K, const std::function<bool()>& run) {
     if (K < 0 || K > kMaxLog) return;
+    if (getenv("NTTAMD_ROW_LOG")) return;  // a forced split needs no trials
     {
         std::lock_guard<std::mutex> g(g_tune_lock);
         if (g_tuned_row_log[K]) return;
@@ -1295,7 +1306,7 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
         if (!square) arena += op_spectra_bytes_b(g) / 8;
         const uint64_t blocks = carry_blocks(B.la[i] + B.lb[i], g.school);
         off_c[i] = carry;
-        carry += 4 * blocks;
+        carry += carry_words((uint32_t)blocks);
     }
     if (!ensure_tables(max_half, B.ex.stream, &B.fwd, &B.inv, &B.tstride)) {
         set_error("batch: root table allocation failed");
@@ -1386,7 +1397,7 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
         B.scan_begin[np] = (uint32_t)items.size();
         for (size_t i = 0; i < count; ++i)
             if ((B.geo[i].school ? np == 0 : (int)B.geo[i].primes == np) && carry_scanned(B.host[i].blocks))
-                items.push_back({(uint32_t)i, 0});
+                for (uint32_t ch = 0; ch < carry_chunks(B.host[i].blocks); ++ch) items.push_back({(uint32_t)i, ch});
         B.scan_count[np] = (uint32_t)items.size() - B.scan_begin[np];
     }
     if (!B.items.ensure(std::max<size_t>(items.size(), 1) * sizeof(BatchItem)) ||
@@ -2389,6 +2400,32 @@ int ntt_selftest_host(uint64_t seed) {
                     }
                     if (fast != got) ++fails;
                     if (admitted) ++interior;
+                    // Parts extracted once and combined per prime (the shared
+                    // packing pass), clamped and, where admitted, unclamped.
+                    {
+                        double dd[kMaxParts] = {0, 0, 0, 0};
+                        double shared = 0.0, shared_fast = 0.0;
+                        bool fast_ok = false;
+                        switch (kind) {
+#define NTT_SHARED_CASE(K)                                                        \
+    case K:                                                                       \
+        extract_as<K, true>(s, n, dd);                                            \
+        shared = combine_as<K>(s, dd, c.powers[y], pf, pinv);                     \
+        if ((fast_ok = pack_interior<K>(s, n + 1))) {                             \
+            extract_as<K, false>(s, n, dd);                                       \
+            shared_fast = combine_as<K>(s, dd, c.powers[y], pf, pinv);            \
+        }                                                                         \
+        break;
+                            NTT_SHARED_CASE(kPackW64)
+                            NTT_SHARED_CASE(kPack1)
+                            NTT_SHARED_CASE(kPack2)
+                            NTT_SHARED_CASE(kPack3)
+#undef NTT_SHARED_CASE
+                            default: shared = got; break;
+                        }
+                        if (!check_mod(shared, (i128)r, p) || fabs(shared) >= pf) ++fails;
+                        if (fast_ok && shared_fast != shared) ++fails;
+                    }
                     // Parts extracted once and combined per prime (stacked
                     // packing tiles) give the same residue.
                     if (kind >= kPack1) {
@@ -2434,6 +2471,10 @@ int ntt_selftest_host(uint64_t seed) {
                 if (pack_interior<kPackW64Signed>(s, n + 1) &&
                     pack_as<kPackW64Signed, false>(s, n, c.powers[y], pf, pinv) != got)
                     ++fails;
+                double dd[2];
+                extract_as<kPackW64Signed, true>(s, n, dd);
+                const double shared = combine_as<kPackW64Signed>(s, dd, c.powers[y], pf, pinv);
+                if (!check_mod(shared, (i128)(int64_t)limbs[n], p) || fabs(shared) >= pf) ++fails;
             }
         }
     }

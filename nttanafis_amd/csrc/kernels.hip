@@ -598,6 +598,132 @@ __global__ __launch_bounds__(NT) void k_fwd_strided(PassArgs a) {
     store_strided<LOGL, LINES, NT, false>(lds, row + base, S, 0.0, p, pinv);
 }
 
+// Forward rounds from local level L0 to the end of the line, MAXR levels per
+// round (the last one shorter).
+template <int LOGL, int LINES, int NT, int L0, int MAXR>
+__device__ __forceinline__ void fwd_rounds_from(double* lds, Tw tw, double p, double pinv) {
+    if constexpr (L0 < LOGL) {
+        constexpr int RL = LOGL - L0 < MAXR ? LOGL - L0 : MAXR;
+        fwd_round<LOGL, LINES, NT, L0, RL, false>(lds, tw, p, pinv);
+        __syncthreads();
+        fwd_rounds_from<LOGL, LINES, NT, L0 + RL, MAXR>(lds, tw, p, pinv);
+    }
+}
+
+// Levels S0..RL-1 of the transform's first round (radix 2^RL) on the points
+// of one group, in registers (fwd_round's butterflies with ROOT1: the group
+// sits at table node 0, whose level-0 root is 1, and holds packed inputs).
+// Levels below S0 were trivial: their multiplied inputs were all zero.
+template <int RL, int S0>
+__device__ __forceinline__ void first_round_regs(double* x, const Tw& tw, double p, double pinv) {
+    constexpr int RADIX = 1 << RL;
+#pragma unroll
+    for (int s = S0; s < RL; ++s) {
+        const int half = RADIX >> (s + 1);
+#pragma unroll
+        for (int blk = 0; blk < (1 << s); ++blk) {
+            const double w = tw.at(0, s, blk);
+#pragma unroll
+            for (int j = 0; j < half; ++j) {
+                const int k = blk * 2 * half + j;
+                const double u = s == RL - 1 ? reduce(x[k], p, pinv) : x[k];
+                const double t = s == 0 ? x[k + half] : mulmod(x[k + half], w, p, pinv);
+                x[k] = u + t;
+                x[k + half] = u - t;
+            }
+        }
+    }
+}
+
+// Threads of the shared packing pass: eight points of a 4096-point tile each.
+template <int LOGL>
+static constexpr int pack_primes_threads() {
+    return Lay<LOGL>::L * (LOGL <= 6 ? 64 : LOGL == 7 ? 32 : LOGL == 8 ? 16 : 8) / 8;
+}
+
+// First forward pass with packing shared across primes: one workgroup packs
+// one tile of LINES lines of 2^LOGL points (4096 points) for every prime of
+// the operation. Each thread extracts the parts of its 8 coefficients once
+// (elements m0 + r*L/8 of line c: exactly one radix-8 group of the first
+// three levels), then per prime combines them into residues, runs those
+// levels in registers, stores the group to LDS, runs the remaining rounds,
+// and writes the tile to that prime's row. The rows start at table node 0
+// and are not folded (single operations on power-of-two rows or cyclic
+// parts).
+//
+// Zero padding past `digits` prunes the first levels: a group whose upper
+// half is zero has a trivial level 0 (both outputs are the lower input),
+// and one whose upper three quarters are zero also a trivial level 1 (the
+// pruned first stages of src/arch/schedule/expression/stages.rs:109-163).
+// Bounds: packed inputs are below p, level 0 (root 1) gives 2p, level 1 adds
+// 15p/16 (multiplied inputs below 4p, products below 2p^2), level 2 reduces
+// the added input and ends below 9p/16 + 15p/16 = 3p/2, and every later round
+// is entered below 31p/16 (fwd_round's schedule).
+template <int LOGL, int LINES, int NT, int KIND>
+__global__ __launch_bounds__(NT) void k_pack_primes(PassArgs a) {
+    constexpr int L = 1 << LOGL;
+    constexpr int LS = Lay<LOGL>::STRIDE;
+    constexpr int D = L / 8;
+    constexpr int P = PackParts<KIND>::value;
+    static_assert(LINES * L == 8 * NT, "one radix-8 group of the first levels per thread");
+    __shared__ double lds[LINES * LS];
+    const uint64_t S = a.line_stride;
+    Job j;
+    resolve<false>(a, S / LINES, false, &j);
+    const int c = threadIdx.x % LINES, m0 = threadIdx.x / LINES;
+    const uint64_t base = j.tile * LINES;
+    const uint64_t n0 = base + c + (uint64_t)m0 * S;  // element m0 + r*D at n0 + r*D*S
+    const uint64_t step = (uint64_t)D * S;
+    const Source& src = j.src_a;
+    const uint64_t digits = src.digits;
+    const uint64_t safe = pack_safe<KIND>(src);
+    double d[8][P];
+    int fill = 0;  // leading elements below digits (the rest are zero)
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const uint64_t n = n0 + (uint64_t)r * step;
+        if (n < safe) {
+            extract_as<KIND, false>(src, n, d[r]);
+        } else {
+            extract_as<KIND, true>(src, n, d[r]);
+        }
+        fill += n < digits ? 1 : 0;
+    }
+#pragma unroll 1
+    for (int y = 0; y < a.primes; ++y) {
+        const double p = a.field.p[y], pinv = a.field.pinv[y];
+        const double* pw = a.powers[y];
+        double x[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) x[r] = combine_as<KIND>(src, d[r], pw, p, pinv);
+        const Tw tw{a.fwd + (uint64_t)y * a.table_stride, 0, 0, 0, 64, p, pinv};
+        if (fill <= 2) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                x[k + 2] = x[k];
+                x[k + 4] = x[k];
+                x[k + 6] = x[k];
+            }
+            first_round_regs<3, 2>(x, tw, p, pinv);
+        } else if (fill <= 4) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) x[k + 4] = x[k];
+            first_round_regs<3, 1>(x, tw, p, pinv);
+        } else {
+            first_round_regs<3, 0>(x, tw, p, pinv);
+        }
+        double* line = lds + c * LS;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) line[pidx(m0 + k * D)] = x[k];
+        __syncthreads();
+        // Radix-8 rounds: one group per thread, and eight values in flight
+        // beside the extracted parts.
+        fwd_rounds_from<LOGL, LINES, NT, 3, 3>(lds, tw, p, pinv);
+        store_strided<LOGL, LINES, NT, false>(lds, j.data + (uint64_t)y * a.row_stride + base, S, 0.0, p, pinv);
+        __syncthreads();  // the tile is read out before the next prime overwrites it
+    }
+}
+
 // Inverse strided pass; CANON (level_start == 0) multiplies by T^-1 and stores
 // canonical residues in [0, p).
 template <int LOGL, int LINES, int NT, bool CANON, bool BATCH>
@@ -625,8 +751,11 @@ __global__ __launch_bounds__(NT) void k_inv_strided(PassArgs a) {
     store_strided<LOGL, LINES, NT, CANON>(lds, row + base, S, a.tinv[y], p, pinv);
 }
 
-// Row passes form the roots of their last four levels as products.
-__host__ __device__ constexpr int row_split(int logl) { return logl > 8 ? logl - 4 : 64; }
+// Row passes form the roots of their last four levels as products (every
+// row is its own node, so reading them would cost one table entry per point
+// at the last level; the products read W[x] from the short, cache-resident
+// prefix of the table and one W[H 2^l] per row and level).
+__host__ __device__ constexpr int row_split(int logl) { return logl > 4 ? logl - 4 : 0; }
 
 // ---------------------------------------------------------------------------
 // Row pass: row H of 2^LOGL contiguous points holds the last levels of the
@@ -1184,48 +1313,54 @@ __global__ __launch_bounds__(kSchoolThreads) void k_school(CrtArgs a) {
 // F_{b-1}(F_{b-2}(... F_0(0))) for the published carry functions F (the block
 // merge of src/limb/reconstruction.rs:211-247).
 //
-// An output of more than kCarryLookBackBlocks blocks first runs k_carry_scan:
-// one workgroup composes the functions of consecutive chunks of blocks (one
-// chunk per thread), scans the chunk aggregates, and walks each chunk again
-// replacing every block's function base by its carry in — O(blocks) work for
-// any carry pattern, all-ones products included. A shorter output skips the
-// scan: each block looks back to the nearest block whose function has no
-// threshold (its output is fixed whatever comes in) and applies the
-// functions after it, at most kCarryLookBackBlocks steps.
+// An output of more than kCarryLookBackBlocks blocks first runs k_carry_scan,
+// one workgroup per chunk of kCarryChunk blocks (one block per thread): it
+// replaces each block's function by the composition of the functions before
+// it in its chunk (the first block of a chunk gets the identity, marked by a
+// zero threshold, which no carry function has) and publishes the chunk's
+// composed function. The apply kernel then composes the functions of the
+// chunks before its own (at most a few per lane of its wave) and applies its
+// block's prefix: O(blocks) work for any carry pattern, all-ones products
+// included. A shorter output skips the scan: each block looks back to the
+// nearest block whose function has no threshold (its output is fixed whatever
+// comes in) and applies the functions after it, at most kCarryLookBackBlocks
+// steps.
 //
 // Then the first limb of the block takes the carry, and the limbs up to the
 // recorded ripple end take one more when the carry reaches the first limb's
 // threshold.
 constexpr int kApplyThreads = 64;
 
-__global__ __launch_bounds__(kCarryScanThreads) void k_carry_scan(CrtArgs a) {
-    constexpr int WAVES = kCarryScanThreads / 64;
+// Carry bookkeeping of the product a scan or apply workgroup works on.
+struct CarryView {
+    uint32_t* words;  // carry_words(blocks) words
+    uint32_t blocks;
+};
+
+__global__ __launch_bounds__(kCarryChunk) void k_carry_scan(CrtArgs a) {
+    constexpr int WAVES = kCarryChunk / 64;
     __shared__ CarryFn wave_total[WAVES];
-    uint32_t *base, *thresh;
-    uint32_t n;
+    CarryView v;
+    uint32_t chunk;
     if (a.batch == nullptr) {
-        base = a.agg_base;
-        thresh = a.agg_thresh;
-        n = a.blocks;
+        v = CarryView{a.agg_base, a.blocks};
+        chunk = blockIdx.x;
     } else {
-        const BatchProduct& P = a.batch[a.scan_items[blockIdx.x].product];
-        base = P.carry;
-        thresh = P.carry + P.blocks;
-        n = P.blocks;
+        const BatchItem it = a.scan_items[blockIdx.x];
+        const BatchProduct& P = a.batch[it.product];
+        v = CarryView{P.carry, P.blocks};
+        chunk = it.sub;
     }
-    const uint32_t per = (n + kCarryScanThreads - 1) / kCarryScanThreads;
-    const uint32_t b0 = threadIdx.x * per;
-    const uint32_t b1 = b0 + per < n ? b0 + per : n;
-    // The threads whose chunk is not empty are a prefix of the workgroup, so
-    // the placeholder of an empty chunk only reaches threads after it, whose
-    // chunks are empty too.
-    CarryFn agg{0u, kInf};
-    for (uint32_t b = b0; b < b1; ++b) {
-        const CarryFn f{base[b], thresh[b]};
-        agg = b == b0 ? f : compose(agg, f);
-    }
+    uint32_t* base = v.words;
+    uint32_t* thresh = v.words + v.blocks;
+    const uint32_t first = chunk * (uint32_t)kCarryChunk;
+    const uint32_t b = first + threadIdx.x;
+    const bool have = b < v.blocks;
+    // The threads of blocks past the output are a suffix of the workgroup, so
+    // their placeholder only reaches threads after them.
+    const CarryFn f = have ? CarryFn{base[b], thresh[b]} : CarryFn{0u, kInf};
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    CarryFn inc = agg;
+    CarryFn inc = f;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
         const CarryFn prev = shfl_up_fn(inc, d);
@@ -1234,23 +1369,25 @@ __global__ __launch_bounds__(kCarryScanThreads) void k_carry_scan(CrtArgs a) {
     if (lane == 63) wave_total[wave] = inc;
     const CarryFn lane_excl = shfl_up_fn(inc, 1);
     __syncthreads();
-    bool have = false;
+    bool pre = false;
     CarryFn g{0u, kInf};
     for (int u = 0; u < wave; ++u) {
-        g = have ? compose(g, wave_total[u]) : wave_total[u];
-        have = true;
+        g = pre ? compose(g, wave_total[u]) : wave_total[u];
+        pre = true;
     }
     if (lane > 0) {
-        g = have ? compose(g, lane_excl) : lane_excl;
-        have = true;
+        g = pre ? compose(g, lane_excl) : lane_excl;
+        pre = true;
     }
-    // Carry into the chunk: its exclusive prefix applied to the zero carry
-    // into block 0 (every threshold is at least 1, so F(0) is F's base).
-    uint32_t c = have ? g.base : 0u;
-    for (uint32_t b = b0; b < b1; ++b) {
-        const uint32_t fb = base[b], ft = thresh[b];
-        base[b] = c;
-        c = fb + (c >= ft ? 1u : 0u);
+    if (!have) return;
+    base[b] = pre ? g.base : 0u;
+    thresh[b] = pre ? g.thresh : 0u;  // 0: the identity (first block of the chunk)
+    const uint32_t count = v.blocks - first < (uint32_t)kCarryChunk ? v.blocks - first : (uint32_t)kCarryChunk;
+    if (threadIdx.x == count - 1) {
+        const CarryFn whole = pre ? compose(g, f) : f;
+        const uint32_t chunks = carry_chunks(v.blocks);
+        v.words[4ull * v.blocks + chunk] = whole.base;
+        v.words[4ull * v.blocks + chunks + chunk] = whole.thresh;
     }
 }
 
@@ -1263,8 +1400,35 @@ __global__ __launch_bounds__(kApplyThreads) void k_carry_apply(CrtArgs a) {
     if (b == 0) return;
     uint32_t c = 0;
     if (carry_scanned(J.blocks)) {
-        // k_carry_scan left the carry in of every block in its base word.
-        c = J.agg_base[b];
+        // Carry into this block's chunk: the chunk functions before it,
+        // composed in order, lane l taking a contiguous slice (the lanes with
+        // a slice are a prefix of the wave).
+        const uint32_t j = (uint32_t)b / (uint32_t)kCarryChunk;
+        uint32_t cin_chunk = 0;
+        if (j > 0) {
+            const uint32_t chunks = carry_chunks(J.blocks);
+            const uint32_t* cb = J.agg_base + 4ull * J.blocks;
+            const uint32_t* ct = cb + chunks;
+            const uint32_t per = (j + kApplyThreads - 1) / kApplyThreads;
+            const uint32_t s0 = threadIdx.x * per;
+            const uint32_t s1 = s0 + per < j ? s0 + per : j;
+            CarryFn agg{0u, kInf};
+            for (uint32_t q = s0; q < s1; ++q) {
+                const CarryFn fq{cb[q], ct[q]};
+                agg = q == s0 ? fq : compose(agg, fq);
+            }
+            const int lane = threadIdx.x;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const CarryFn prev = shfl_up_fn(agg, d);
+                if (lane >= d) agg = compose(prev, agg);
+            }
+            const int last = (int)((j + per - 1) / per) - 1;
+            cin_chunk = __shfl(agg.base, last, 64);  // the composition applied to 0
+        }
+        // The block's prefix within its chunk, from k_carry_scan.
+        const uint32_t pb = J.agg_base[b], pt = J.agg_thresh[b];
+        c = pt == 0 ? cin_chunk : pb + (cin_chunk >= pt ? 1u : 0u);
     } else {
         // b <= kCarryLookBackBlocks = kApplyThreads: one look-back step.
         if (threadIdx.x == 0) found = 0xFFFFFFFFu;
@@ -1356,6 +1520,32 @@ static bool launch_fwd_strided_t(const PassArgs& a, int units, int operands, boo
     return NTT_CHECK_LAUNCH();
 }
 
+// First forward pass of a single operation with packing shared across its
+// `primes` primes: one workgroup per tile and operand (grid z as in
+// k_fwd_strided). Whole-limb and one- to three-part coefficients.
+template <int LOGL>
+static bool launch_pack_primes_t(const PassArgs& a, int primes, int operands, hipStream_t s) {
+    constexpr int LINES = strided_lines<LOGL>();
+    constexpr int NT = pack_primes_threads<LOGL>();
+    static_assert(LINES * (1 << LOGL) == 8 * NT, "one radix-8 group per thread");
+    if (a.line_stride < (uint64_t)LINES || a.level_start != 0 || a.fold_terms != 1 || a.node_base != 0 || a.batch)
+        return false;
+    PassArgs b = a;
+    b.primes = primes;
+    const dim3 grid((unsigned)(a.line_stride / LINES), 1, (unsigned)operands);
+    switch (pack_kind(a.src_a)) {
+        case kPackW64: NTT_LAUNCH(nullptr, (k_pack_primes<LOGL, LINES, NT, kPackW64>), grid, dim3(NT), s, b); break;
+        case kPackW64Signed:
+            NTT_LAUNCH(nullptr, (k_pack_primes<LOGL, LINES, NT, kPackW64Signed>), grid, dim3(NT), s, b);
+            break;
+        case kPack1: NTT_LAUNCH(nullptr, (k_pack_primes<LOGL, LINES, NT, kPack1>), grid, dim3(NT), s, b); break;
+        case kPack2: NTT_LAUNCH(nullptr, (k_pack_primes<LOGL, LINES, NT, kPack2>), grid, dim3(NT), s, b); break;
+        case kPack3: NTT_LAUNCH(nullptr, (k_pack_primes<LOGL, LINES, NT, kPack3>), grid, dim3(NT), s, b); break;
+        default: return false;
+    }
+    return NTT_CHECK_LAUNCH();
+}
+
 template <int LOGL>
 static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, hipStream_t s) {
     constexpr int LINES = strided_lines<LOGL>();
@@ -1434,6 +1624,21 @@ bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, b
     }
 }
 
+// False when the pass cannot take the shared form (the caller then packs
+// per prime): a batch, a folded or tail part, four-part coefficients, a
+// source kind other than the second operand's, or lines of other than 2^6 to
+// 2^9 points (4096-point tiles).
+bool launch_pack_primes(int logl, const PassArgs& a, int primes, int operands, hipStream_t s) {
+    if (pack_kind(a.src_a) != pack_kind(a.src_b) || pack_kind(a.src_a) == kPack4) return false;
+    switch (logl) {
+        case 6: return launch_pack_primes_t<6>(a, primes, operands, s);
+        case 7: return launch_pack_primes_t<7>(a, primes, operands, s);
+        case 8: return launch_pack_primes_t<8>(a, primes, operands, s);
+        case 9: return launch_pack_primes_t<9>(a, primes, operands, s);
+        default: return false;
+    }
+}
+
 bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, hipStream_t s) {
     switch (logl) {
         case 5: return launch_inv_strided_t<5>(a, primes, canon, s);
@@ -1475,7 +1680,8 @@ bool launch_tables(const TableArgs& a, int primes, hipStream_t s) {
 template <int BL>
 static bool launch_carries(const CrtArgs& a, hipStream_t s) {
     if (a.batch ? a.scan_count > 0 : carry_scanned(a.blocks)) {
-        NTT_LAUNCH("carry_scan", k_carry_scan, dim3(a.batch ? a.scan_count : 1), dim3(kCarryScanThreads), s, a);
+        NTT_LAUNCH("carry_scan", k_carry_scan, dim3(a.batch ? a.scan_count : carry_chunks(a.blocks)),
+                   dim3(kCarryChunk), s, a);
         if (!NTT_CHECK_LAUNCH()) return false;
     }
     if (a.blocks > 1 || a.batch)

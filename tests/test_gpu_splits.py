@@ -41,7 +41,7 @@ def device():
 
 
 def row_candidates(K):
-    """Admissible row sizes, largest first (at most four), as the engine's
+    """Admissible row sizes, largest first (at most eight), as the engine's
     autotuner enumerates them."""
     out = []
     for r in range(12, 4, -1):
@@ -53,7 +53,7 @@ def row_candidates(K):
             ok = k1 <= 11 and rest - k1 >= 5
         if ok:
             out.append(r)
-        if len(out) == 4:
+        if len(out) == 8:
             break
     return out
 
