@@ -64,6 +64,15 @@ struct Lay {
 // the root is formed as W[x] * W[(h0 + c*hstep) << l] instead: the table is
 // bit-reversed, so W[j 2^l + x] = W[j 2^l] W[x] for x < 2^l, and W[x] comes
 // from the short, cache-resident prefix of the table.
+//
+// A product root is left unreduced: both factors are centered (at most p/2),
+// so the product is at most p^2/4 and its rounded quotient is within
+// 1/2 + 1/16 of the exact one, which leaves |root| < 9p/16 + p/16 = 5p/8.
+// Product roots serve only the last forward round of a row (entered below
+// 31p/16, its multiplied inputs stay below 5.2p and their products below
+// 3.3p^2 < 4p^2, and it still ends below 9p/16 + 11p/8 = 31p/16) and the
+// inverse levels (differences below 11p/2 times 5p/8 stay below 4p^2, and
+// below 2p^2 for differences below 11p/4), so every bound below holds.
 struct Tw {
     const double* table;
     uint64_t h0;
@@ -73,7 +82,7 @@ struct Tw {
     double p, pinv;
     __device__ __forceinline__ double at(int c, int lvl, int node) const {
         const uint64_t h = (h0 + (uint64_t)((c & cmask) * hstep)) << lvl;
-        if (lvl >= split) return reduce(mulmod(table[node], table[h], p, pinv), p, pinv);
+        if (lvl >= split) return mulmod(table[node], table[h], p, pinv);
         return table[h + node];
     }
 };
@@ -635,10 +644,15 @@ __device__ __forceinline__ void first_round_regs(double* x, const Tw& tw, double
     }
 }
 
-// Threads of the shared packing pass: eight points of a 4096-point tile each.
+// Tiles of the shared packing pass: 4096 points (its extracted parts stay in
+// registers), eight points per thread.
+template <int LOGL>
+static constexpr int pack_primes_lines() {
+    return 4096 >> LOGL;
+}
 template <int LOGL>
 static constexpr int pack_primes_threads() {
-    return Lay<LOGL>::L * (LOGL <= 6 ? 64 : LOGL == 7 ? 32 : LOGL == 8 ? 16 : 8) / 8;
+    return 4096 / 8;
 }
 
 // First forward pass with packing shared across primes: one workgroup packs
@@ -1477,11 +1491,15 @@ __global__ __launch_bounds__(256) void k_crt_disjoint(CrtArgs a) {
 
 #define NTT_CHECK_LAUNCH() (hipGetLastError() == hipSuccess)
 
-// Lines per strided tile: short lines are taken 64 or 32 at a time, so every
-// point of the tile moves in runs of at least 256 bytes.
+// Lines per strided tile: lines of up to 2^7 points are taken 64 at a time
+// (512-byte runs), lines of 2^8 points 32 at a time (256-byte runs); longer
+// lines fill 8192-point tiles. Measured with a plain tile copy at 2^24
+// points x 6 rows: 64 x 128 moves 5.6 TB/s at a first pass's line stride
+// and 6.0 TB/s at a middle pass's, against 5.1 and 5.35 for 32 x 128; 32 x
+// 256 moves 5.3-5.5 TB/s against 4.6-4.8 for 16 x 256.
 template <int LOGL>
 static constexpr int strided_lines() {
-    return LOGL <= 6 ? 64 : LOGL == 7 ? 32 : LOGL == 8 ? 16 : (LOGL == 9 ? 8 : (LOGL == 10 ? 8 : 4));
+    return LOGL <= 7 ? 64 : LOGL == 8 ? 32 : (LOGL == 9 ? 8 : (LOGL == 10 ? 8 : 4));
 }
 
 // Threads of a strided tile: one radix-16 group per thread and round.
@@ -1525,7 +1543,7 @@ static bool launch_fwd_strided_t(const PassArgs& a, int units, int operands, boo
 // k_fwd_strided). Whole-limb and one- to three-part coefficients.
 template <int LOGL>
 static bool launch_pack_primes_t(const PassArgs& a, int primes, int operands, hipStream_t s) {
-    constexpr int LINES = strided_lines<LOGL>();
+    constexpr int LINES = pack_primes_lines<LOGL>();
     constexpr int NT = pack_primes_threads<LOGL>();
     static_assert(LINES * (1 << LOGL) == 8 * NT, "one radix-8 group per thread");
     if (a.line_stride < (uint64_t)LINES || a.level_start != 0 || a.fold_terms != 1 || a.node_base != 0 || a.batch)
