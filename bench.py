@@ -6,11 +6,20 @@ limbs, seeds 1 and 2, as the reference benchmarks use), operands and result
 resident in HBM. A step is one MulWorkspace product on the device; the metric
 is operand limbs per second (Mlimbs/s). Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--workload mul_2p20|sqr_2p24|conv16_2p22|conv16_5x2p20|conv32s_2p22|batch4096|served]
+    python bench.py [--steps K] [--warmup W] [--workload mul_2p20|sqr_2p24|ones_mul_2p20|ones_sqr_2p24|
+                                                          conv16_2p22|conv16_5x2p20|conv32s_2p22|
+                                                          mul_1024|sqr_1024|batch4096|served|small]
 
-The line also carries the batched form ("batched": 4,096 uneven products in
-one graph launch against the plain one-call-per-product loop) unless
---no-batch is given.
+Besides `value` (K steps between device synchronizations), the line carries:
+per kernel kind, the launch duration bracketed by HIP events and the same
+launch replayed back to back (`kernels`); the roofline of the dominant kernel
+from its replayed duration; the NTT passes' algorithmic bytes over their
+summed durations (`ntt_passes`); the other configurations of the workload
+set (`configs`: 1,024-limb products per call, the 2^24-limb square, all-ones
+operands, both 2^22-coefficient convolutions); the host-buffer path; small
+products; the 4,096-product batch against the per-call loop; the served
+stream; and CPU baselines (the C restatement of the reference single-threaded
+and on every usable host core, and GMP's mpn_mul). --no-* flags drop legs.
 """
 from __future__ import annotations
 
@@ -44,11 +53,21 @@ def splitmix(length: int, seed: int) -> np.ndarray:
 
 
 def workload(name: str):
-    """Returns (operands, step builder description)."""
+    """Operands of a named workload: kind (mul, sqr, conv), a, b, and the
+    coefficient bounds of a convolution."""
+    ones = lambda n: np.full(n, np.uint64((1 << 64) - 1), dtype=np.uint64)  # noqa: E731
     if name == "mul_2p20":
         return dict(kind="mul", a=splitmix(1 << 20, 1), b=splitmix(1 << 20, 2))
+    if name == "ones_mul_2p20":
+        return dict(kind="mul", a=ones(1 << 20), b=ones(1 << 20))
+    if name == "mul_1024":
+        return dict(kind="mul", a=splitmix(1024, 1), b=splitmix(1024, 2))
+    if name == "sqr_1024":
+        return dict(kind="sqr", a=splitmix(1024, 1))
     if name == "sqr_2p24":
         return dict(kind="sqr", a=splitmix(1 << 24, 1))
+    if name == "ones_sqr_2p24":
+        return dict(kind="sqr", a=ones(1 << 24))
     if name == "conv16_2p22":
         n = 1 << 22
         return dict(kind="conv", a=splitmix(n, 1) & np.uint64(0xFFFF), b=splitmix(n, 2) & np.uint64(0xFFFF),
@@ -116,77 +135,83 @@ def step_bytes(name: str, primes: int, log_len: int, tail: int, la: int, lb: int
     return total
 
 
-def measured_valu(workload: str) -> dict:
-    """VALU wave-instructions per launch of each kernel kind (SQ_INSTS_VALU
-    from the committed rocprofv3 counter summary of this workload), lines of
-    the form `VALU <kind> <instructions>`; empty when none are committed."""
+def committed_counters(workload: str) -> dict:
+    """The committed rocprofv3 counter summary of a workload
+    (profiles/<workload>_counters.txt): `VALU <kind> <instructions>` and
+    `TRAFFIC <kind> <bytes>` lines (per launch, FETCH_SIZE and WRITE_SIZE each
+    scaled by what the counter reads for the plain copy kernel), and the
+    `SPLIT <log_len> <pass levels>` the profiled runs used."""
     path = os.path.join(ROOT, "profiles", f"{workload}_counters.txt")
-    out = {}
+    out = {"valu": {}, "traffic": {}, "split": None}
     if os.path.exists(path):
         for line in open(path):
             parts = line.split()
             if len(parts) == 3 and parts[0] == "VALU":
-                out[parts[1]] = int(float(parts[2]))
+                out["valu"][parts[1]] = int(float(parts[2]))
+            elif len(parts) == 3 and parts[0] == "TRAFFIC":
+                out["traffic"][parts[1]] = int(float(parts[2]))
+            elif len(parts) == 3 and parts[0] == "SPLIT":
+                out["split"] = [int(parts[1]), [int(x) for x in parts[2].split(",")]]
     return out
 
 
-def measured_traffic(workload: str) -> dict:
-    """HBM bytes per launch of each kernel kind, from the committed rocprofv3
-    counter summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled
-    by what the counter reads for the plain copy kernel). Lines of the form
-    `TRAFFIC <kind> <bytes>`; empty when no summary is committed."""
-    path = os.path.join(ROOT, "profiles", f"{workload}_counters.txt")
-    out = {}
-    if os.path.exists(path):
-        for line in open(path):
-            parts = line.split()
-            if len(parts) == 3 and parts[0] == "TRAFFIC":
-                out[parts[1]] = int(float(parts[2]))
-    return out
+def _median_time(fn, runs: int, budget_s: float) -> tuple:
+    """Median wall time of fn over `runs` runs after one warm-up (fewer runs
+    when the budget is spent, at least three)."""
+    fn()
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < runs:
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+        if len(times) >= 3 and time.perf_counter() - t_start > budget_s:
+            break
+    return float(np.median(times)), len(times)
 
 
-def cpu_baseline(wl, sample_seconds: float, threads: int = 1):
-    """Times the oracle (the C restatement of the reference CPU path) on a
-    bounded sample of the same workload, with `threads` host threads (the
-    oracle parallelises over primes and tiles, as the reference's rayon
-    schedule does over primes)."""
+def cpu_baselines(a, b, budget_s: float) -> dict:
+    """The reference CPU path restated in C (oracle/ntt_oracle.c, built with
+    -march=native on this host when gcc is here), timed on the same 2^20 x
+    2^20-limb operands single-threaded and on every host core this process
+    may run on (the oracle parallelises over primes and tiles, as the
+    reference's rayon schedule does); and GMP's mpn_mul (libgmp.so.10, one
+    thread) when the library loads. Median of at least five runs each after
+    one warm-up, within `budget_s` seconds per leg."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    a = wl["a"]
-    t0 = time.perf_counter()
-    runs = 0
-    tag = f"{threads} thread" + ("s" if threads > 1 else "")
-    if wl["kind"] == "mul":
-        b = wl["b"]
-        while True:
-            oracle.mul(a, b, threads=threads)
-            runs += 1
-            if time.perf_counter() - t0 > sample_seconds:
-                break
-        per = (time.perf_counter() - t0) / runs
-        return {"value": len(a) / per / 1e6, "unit": "Mlimbs/s", "cores": threads, "kind": "port",
-                "sample": f"{runs} products of 2^20 x 2^20 limbs, oracle/ntt_oracle.c, {tag}"}
-    if wl["kind"] == "sqr":
-        n = 1 << 20
-        sub = a[:n]
-        while True:
-            oracle.sqr(sub, threads=threads)
-            runs += 1
-            if time.perf_counter() - t0 > sample_seconds:
-                break
-        per = (time.perf_counter() - t0) / runs
-        return {"value": n / per / 1e6, "unit": "Mlimbs/s", "cores": threads, "kind": "port",
-                "sample": f"{runs} squares of 2^20 limbs (sub-sample of the 2^24 square), {tag}"}
-    b = wl["b"]
-    while True:
-        oracle.convolve(a, b, wl["max"], wl["max"], wl["signed"], threads=threads)
-        runs += 1
-        if time.perf_counter() - t0 > sample_seconds:
-            break
-    per = (time.perf_counter() - t0) / runs
-    return {"value": len(a) / per / 1e6, "unit": "Mcoeff/s", "cores": threads, "kind": "port",
-            "sample": f"{runs} convolutions of {len(a)} coefficients, {tag}"}
+    native = oracle.use_native()
+    build = "-march=native" if native else "-march=x86-64-v3"
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    n = len(a)
+    t1, r1 = _median_time(lambda: oracle.mul(a, b, threads=1), 5, budget_s)
+    tn, rn = _median_time(lambda: oracle.mul(a, b, threads=cores), 7, budget_s)
+    single = {"value": n / t1 / 1e6, "unit": "Mlimbs/s", "cores": 1, "kind": "port",
+              "sample": f"median of {r1} products of 2^20 x 2^20 limbs, oracle/ntt_oracle.c ({build}), 1 thread",
+              "ms_per_product": round(1000 * t1, 2),
+              "vs_reference_published": round(n / t1 / 1e6 / PUBLISHED["mul_2p20"]["value"], 3)}
+    threads = {"value": n / tn / 1e6, "unit": "Mlimbs/s", "cores": cores, "kind": "port",
+               "sample": f"median of {rn} products of 2^20 x 2^20 limbs, oracle/ntt_oracle.c ({build}), "
+                         f"{cores} threads", "ms_per_product": round(1000 * tn, 2),
+               "host_cpu_count": os.cpu_count(), "host_cores_usable": cores}
+    out = {"cpu_baseline": single, "cpu_baseline_threads": threads}
+    try:
+        import ctypes
+
+        gmp = ctypes.CDLL("libgmp.so.10")
+        dst = np.zeros(2 * n, dtype=np.uint64)
+        P = ctypes.POINTER(ctypes.c_uint64)
+        pa, pb, pd = a.ctypes.data_as(P), b.ctypes.data_as(P), dst.ctypes.data_as(P)
+        tg, rg = _median_time(lambda: gmp.__gmpn_mul(pd, pa, ctypes.c_long(n), pb, ctypes.c_long(n)), 5, budget_s)
+        out["cpu_gmp"] = {"value": n / tg / 1e6, "unit": "Mlimbs/s", "cores": 1, "kind": "gmp mpn_mul",
+                          "sample": f"median of {rg} products of 2^20 x 2^20 limbs, libgmp.so.10, 1 thread",
+                          "ms_per_product": round(1000 * tg, 2)}
+        want = oracle.mul(a, b, threads=cores)
+        out["cpu_gmp"]["matches_oracle"] = bool((dst == want).all())
+    except OSError as e:
+        out["cpu_gmp"] = {"value": None, "unavailable": str(e)}
+    return out
 
 
 def mod_q61(limbs) -> int:
@@ -470,17 +495,226 @@ def bench_served(count: int, load: float, hi: int = 4194304) -> dict:
     }
 
 
+NTT_KINDS = ("fused_single", "fwd_pack_strided", "fwd_strided", "fused_rows", "inv_strided", "inv_strided_canon")
+
+
+def load_digests() -> dict:
+    path = os.path.join(ROOT, "tests", "golden", "digests.json")
+    return json.load(open(path)) if os.path.exists(path) else {}
+
+
+def check_result(name: str, wl: dict, got: np.ndarray) -> str:
+    """Checks a device result: against the committed golden digest of the
+    workload (GMP products and Kronecker convolutions), the all-ones closed
+    form (B^n - 1)(B^m - 1), or the product modulo 2^61 - 1. Returns the
+    check that passed; raises on a mismatch."""
+    import hashlib
+
+    rec = load_digests().get(name)
+    if rec is not None:
+        h = hashlib.sha256(np.ascontiguousarray(got, dtype="<u8").tobytes()).hexdigest()
+        if h != rec["sha256"]:
+            raise SystemExit(f"{name}: result differs from the golden digest")
+        return "golden digest"
+    a = wl["a"]
+    b = a if wl["kind"] == "sqr" else wl["b"]
+    if name.startswith("ones_"):
+        n, m = max(len(a), len(b)), min(len(a), len(b))
+        mx = np.uint64((1 << 64) - 1)
+        ok = (got[0] == 1 and not got[1:m].any() and (got[m:n] == mx).all() and got[n] == mx - np.uint64(1)
+              and (got[n + 1:] == mx).all())
+        if not ok:
+            raise SystemExit(f"{name}: result differs from the all-ones closed form")
+        return "all-ones closed form"
+    if wl["kind"] != "conv":
+        q = (1 << 61) - 1
+        if mod_q61(got) != mod_q61(a) * mod_q61(b) % q:
+            raise SystemExit(f"{name}: result fails its residue modulo 2^61 - 1")
+        return "residue modulo 2^61 - 1"
+    return "none"
+
+
+def gpu_workload(name: str, steps: int, warmup: int, replay_reps: int = 20, runs: int = 5) -> dict:
+    """One workload on the device path with operands resident in HBM.
+
+    - timed region: exactly `steps` steps between device synchronizations
+      (ms_per_step), repeated `runs` times for a median;
+    - the same steps again with every launch bracketed by HIP events (per
+      kernel kind: event_us, the launch time including the event packets);
+    - per kernel kind, the last launch re-issued `replay_reps` times back to
+      back between two events on its stream (replay_us: the kernel's own
+      duration, what rocprofv3 reports);
+    - the result checked (golden digest, closed form or residue)."""
+    wl = workload(name)
+    a = np.ascontiguousarray(wl["a"])
+    kind = wl["kind"]
+    b = a if kind == "sqr" else np.ascontiguousarray(wl["b"])
+    la, lb = len(a), len(b)
+    da = ntt.DeviceBuffer.from_host(a)
+    db = da if kind == "sqr" else ntt.DeviceBuffer.from_host(b)
+    if kind == "conv":
+        plan = ntt.ConvPlan.new(la, lb, wl["max"], wl["max"], wl["signed"])
+        out_len = plan.output_len()
+        ws = ntt.ConvWorkspace()
+        ws.prepare(plan)
+        primes, log_len, tail, chunk = plan.primes, plan.log_len, plan.row_tail, 64
+        dout = ntt.DeviceBuffer(out_len * 8)
+
+        def step():
+            return ws.convolve_device(dout.ptr, out_len, da.ptr, la, db.ptr, lb)
+    else:
+        plan = ntt.MulPlan.new(la, lb)
+        out_len = la + lb
+        ws = ntt.MulWorkspace()
+        ws.prepare(plan)
+        primes, log_len, tail, chunk = plan.primes, plan.log_len, plan.row_tail, plan.chunk_bits
+        dout = ntt.DeviceBuffer(out_len * 8)
+        if kind == "sqr":
+            def step():
+                return ws.sqr_device(dout.ptr, out_len, da.ptr, la)
+        else:
+            def step():
+                return ws.mul_device(dout.ptr, out_len, da.ptr, la, db.ptr, lb)
+    for _ in range(warmup):
+        if not step():
+            raise SystemExit(f"{name}: step failed: {ntt.last_error()}")
+    assert ntt.synchronize()
+    region_ms = []
+    for _ in range(runs):
+        assert ntt.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if not step():
+                raise SystemExit(f"{name}: step failed: {ntt.last_error()}")
+        assert ntt.synchronize()
+        region_ms.append(1000.0 * (time.perf_counter() - t0) / steps)
+    ntt.Profile.reset()
+    ntt.Profile.enable(True)
+    assert ntt.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    assert ntt.synchronize()
+    events_ms = 1000.0 * (time.perf_counter() - t0) / steps
+    ntt.Profile.enable(False)
+    prof = ntt.Profile.read()
+    checked = check_result(name, wl, dout.download(out_len))
+    kernels = {}
+    for k, (ms, count) in prof.items():
+        per_step = max(1, int(round(count / steps)))
+        rep = ntt.Profile.replay(k, replay_reps)
+        nbytes = step_bytes(k, primes, log_len, tail, la, lb, out_len, kind == "sqr") / per_step
+        ev_us = 1000.0 * ms / max(count, 1)
+        row = {"event_us": round(ev_us, 2), "launches_per_step": per_step}
+        if rep is not None:
+            row["replay_us"] = round(1000.0 * rep, 2)
+        if nbytes:
+            row["bytes_per_launch"] = int(nbytes)
+            if rep:
+                row["GBps"] = round(nbytes / (rep * 1e-3) / 1e9, 1)
+        kernels[k] = row
+    # The NTT passes' algorithmic bytes over their summed launch durations in
+    # the timed steps (events), and over their replayed durations.
+    passes = [k for k in kernels if k in NTT_KINDS and kernels[k].get("bytes_per_launch")]
+    pass_bytes = sum(kernels[k]["bytes_per_launch"] * kernels[k]["launches_per_step"] for k in passes)
+    pass_us = sum(kernels[k]["event_us"] * kernels[k]["launches_per_step"] for k in passes)
+    pass_us_rep = sum(kernels[k].get("replay_us", 0.0) * kernels[k]["launches_per_step"] for k in passes)
+    split = ntt.transform_split(log_len, "square" if kind == "sqr" else "product") if tail == 0 else None
+    units = la if kind != "sqr" else la
+    return {
+        "workload": name,
+        "kind": kind,
+        "len_a": la,
+        "len_b": lb,
+        "primes": primes,
+        "chunk_bits": chunk,
+        "log_len": log_len,
+        "row_tail": tail,
+        "transform_split": split,
+        "steps": steps,
+        "ms_per_step": round(region_ms[0], 4),
+        "ms_per_step_runs": [round(x, 4) for x in region_ms],
+        "ms_per_step_median": round(float(np.median(region_ms)), 4),
+        "ms_per_step_with_events": round(events_ms, 4),
+        "rate": round(units / (float(np.median(region_ms)) * 1e-3) / 1e6, 2),
+        "rate_unit": "Mcoeff/s" if kind == "conv" else "Mlimbs/s",
+        "kernels": kernels,
+        "ntt_passes": {"kinds": passes, "bytes_per_step": int(pass_bytes), "us_per_step": round(pass_us, 1),
+                       "TBps": round(pass_bytes / (pass_us * 1e-6) / 1e12, 3) if pass_us else None,
+                       "frac_of_8TBps": round(pass_bytes / (pass_us * 1e-6) / 8e12, 4) if pass_us else None,
+                       "us_per_step_replayed": round(pass_us_rep, 1),
+                       "frac_of_8TBps_replayed": round(pass_bytes / (pass_us_rep * 1e-6) / 8e12, 4) if pass_us_rep else None},
+        "check": checked,
+    }
+
+
+def bench_config1(reps: int = 200) -> dict:
+    """Config 1: MulPlan::mul of two 1,024-limb operands and MulPlan::sqr of
+    the first, per call. Device path (operands in HBM): `reps` calls queued
+    back to back, and `reps` calls each waited for; host path (MulPlan with
+    host arrays, PCIe included): calls each waited for. Results are checked
+    against the committed GMP fixture."""
+    gold = np.load(os.path.join(ROOT, "tests", "golden", "config1.npz"), allow_pickle=False)
+    a, b = splitmix(1024, 1), splitmix(1024, 2)
+    out = {}
+    for name, square in (("mul", False), ("sqr", True)):
+        bb = a if square else b
+        want = gold["sqr" if square else "mul"]
+        da, db = ntt.DeviceBuffer.from_host(a), ntt.DeviceBuffer.from_host(bb)
+        dout = ntt.DeviceBuffer(8 * 2048)
+        ws = ntt.MulWorkspace()
+        ws.prepare(ntt.MulPlan.new(1024, 1024))
+        if square:
+            call = lambda: ws.sqr_device(dout.ptr, 2048, da.ptr, 1024)  # noqa: E731
+        else:
+            call = lambda: ws.mul_device(dout.ptr, 2048, da.ptr, 1024, db.ptr, 1024)  # noqa: E731
+        for _ in range(10):
+            assert call(), ntt.last_error()
+        assert ntt.synchronize()
+        if not (dout.download(2048) == want).all():
+            raise SystemExit(f"config 1 {name}: device result differs from the GMP fixture")
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        assert ntt.synchronize()
+        queued = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+            ntt.synchronize()
+        waited = (time.perf_counter() - t0) / reps
+        plan = ntt.MulPlan.new(1024, 1024)
+        dst = np.zeros(2048, dtype=np.uint64)
+        host = (lambda: plan.sqr(dst, a)) if square else (lambda: plan.mul(dst, a, b))
+        for _ in range(10):
+            assert host(), ntt.last_error()
+        if not (dst == want).all():
+            raise SystemExit(f"config 1 {name}: host-path result differs from the GMP fixture")
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            host()
+        host_s = (time.perf_counter() - t0) / reps
+        out[name] = {"device_us_per_call": round(queued * 1e6, 2), "device_us_waited": round(waited * 1e6, 2),
+                     "host_us_per_call": round(host_s * 1e6, 2),
+                     "schoolbook": bool(ntt.mul_schoolbook(1024, 1024)),
+                     "geometry": list(ntt.mul_geometry(1024, 1024))}
+    out["reference_us"] = {"value": 23.1, "source": "reference README.md:25, mul 1,024 x 1,024, one Zen 5 core"}
+    out["reps"] = reps
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="mul_2p20")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="time budget of each CPU-baseline leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-small", action="store_true", help="skip the small-product latency table")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer API timing")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other configurations' legs")
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=240)
@@ -525,123 +759,51 @@ def main() -> int:
                           "config": {"workload": "batch4096"}, "batched": res}), flush=True)
         return 0
 
-    wl = workload(args.workload)
-    a = np.ascontiguousarray(wl["a"])
-    kind = wl["kind"]
-    b = np.ascontiguousarray(wl["b"]) if kind != "sqr" else a
-    la, lb = len(a), len(b)
-    da = ntt.DeviceBuffer.from_host(a)
-    db = ntt.DeviceBuffer.from_host(b) if kind != "sqr" else da
-    if kind == "conv":
-        plan = ntt.ConvPlan.new(la, lb, wl["max"], wl["max"], wl["signed"])
-        out_len = plan.output_len()
-        ws = ntt.ConvWorkspace()
-        ws.prepare(plan)
-        primes, log_len, tail = plan.primes, plan.log_len, plan.row_tail
-        chunk = 64
-
-        def step():
-            return ws.convolve_device(dout.ptr, out_len, da.ptr, la, db.ptr, lb)
-    else:
-        plan = ntt.MulPlan.new(la, lb)
-        out_len = la + lb
-        ws = ntt.MulWorkspace()
-        ws.prepare(plan)
-        primes, log_len, chunk = plan.primes, plan.log_len, plan.chunk_bits
-        tail = plan.row_tail
-
-        if kind == "sqr":
-            def step():
-                return ws.sqr_device(dout.ptr, out_len, da.ptr, la)
-        else:
-            def step():
-                return ws.mul_device(dout.ptr, out_len, da.ptr, la, db.ptr, lb)
-    dout = ntt.DeviceBuffer(out_len * 8)
-    T = ntt.row_len(log_len, tail)
-
-    for _ in range(args.warmup):
-        assert step(), ntt.last_error()
-    assert ntt.synchronize()
-
-    # Timed region: exactly K steps, synchronized on both sides. Every kernel
-    # launch inside is bracketed by HIP events on its own stream.
-    ntt.Profile.reset()
-    ntt.Profile.enable(True)
-    assert ntt.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        if not step():
-            raise SystemExit("step failed: " + ntt.last_error())
-    assert ntt.synchronize()
-    elapsed = time.perf_counter() - t0
-    ntt.Profile.enable(False)
-    prof = ntt.Profile.read()
-
-    # The same K steps without events, for the event overhead.
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    assert ntt.synchronize()
-    clean = time.perf_counter() - t1
-
+    # The timed workload (default: the headline, 2^20 x 2^20 limbs).
+    head = gpu_workload(args.workload, args.steps, args.warmup)
     if args.copy_bytes:
         n = (args.copy_bytes // 16) * 16
         src, dst = ntt.DeviceBuffer(n), ntt.DeviceBuffer(n)
         assert ntt.load().ntt_debug_copy(dst.ptr, src.ptr, n)
         src.free()
         dst.free()
-
-    # Result check on the last step: the product modulo a 61-bit prime.
-    got = dout.download(out_len)
-    if kind != "conv":
-        q = (1 << 61) - 1
-        ia = int.from_bytes(a.astype("<u8").tobytes(), "little") % q
-        ib = ia if kind == "sqr" else int.from_bytes(b.astype("<u8").tobytes(), "little") % q
-        ig = int.from_bytes(got.astype("<u8").tobytes(), "little") % q
-        assert ig == ia * ib % q, "device product failed its modular check"
-
-    ms_step = 1000.0 * elapsed / args.steps
-    ms_clean = 1000.0 * clean / args.steps
-    per_kernel = {}
-    dominant, dom_total = None, -1.0
-    for name, (ms, count) in prof.items():
-        avg_ms = ms / max(count, 1)
-        # Bytes of one step's launches of this kind over their time in a step.
-        nbytes = step_bytes(name, primes, log_len, tail, la, lb, out_len, kind == "sqr")
-        step_ms = ms / args.steps
-        per_kernel[name] = {"avg_us": round(avg_ms * 1000, 2), "launches": int(count),
-                            "GBps": round(nbytes / (step_ms * 1e-3) / 1e9, 1) if nbytes else None}
-        if ms > dom_total and nbytes:
-            dominant, dom_total = name, ms
-    traffic = measured_traffic(args.workload)
-    roofline = None
-    result_valu = None
+    kernels = head["kernels"]
+    # Roofline of the dominant kernel kind: its algorithmic bytes per launch
+    # over its launch duration inside the timed steps (HIP events bracketing
+    # each launch on its own stream); beside it, the same launch replayed
+    # back to back between two events.
+    dominant, dom_us = None, -1.0
+    for k, row in kernels.items():
+        us = row["event_us"] * row["launches_per_step"]
+        if row.get("bytes_per_launch") and us > dom_us:
+            dominant, dom_us = k, us
+    roofline, valu = None, None
+    committed = committed_counters(args.workload)
+    split_matches = committed["split"] == [head["log_len"], head["transform_split"]]
     if dominant:
-        ms, count = prof[dominant]
-        per_step = count // args.steps
-        nbytes = step_bytes(dominant, primes, log_len, tail, la, lb, out_len, kind == "sqr")
-        achieved = nbytes / (ms / args.steps * 1e-3) / 1e9
-        # A kind launched once per step: bytes and traffic per launch. A
-        # composite row launches some kinds once per part; then the figures
-        # cover the step's launches of that kind together.
+        row = kernels[dominant]
+        achieved = row["bytes_per_launch"] / (row["event_us"] * 1e-6) / 1e9
+        replay = row.get("replay_us")
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic.get(dominant) if per_step == 1 else None,
-                    "algorithmic_bytes": nbytes, "launches_per_step": per_step}
-        # The FP64 vector issue rate of the same kernel: its VALU instructions
-        # per launch (committed counters) over the live launch duration,
-        # against 1024 SIMDs issuing one wave64 FP64 instruction per 4 cycles
-        # at 2.4 GHz (78.6 TFLOP/s of FMA).
-        insts = measured_valu(args.workload).get(dominant)
-        if insts and per_step == 1:
-            rate = insts / (prof[dominant][0] / prof[dominant][1] * 1e-3) / 1e9
-            result_valu = {"kernel": dominant, "insts_per_launch": insts, "achieved": round(rate, 1),
-                           "peak": VALU_PEAK_G, "unit": "G wave-instr/s", "frac": round(rate / VALU_PEAK_G, 4)}
-        else:
-            result_valu = None
+                    "traffic": committed["traffic"].get(dominant) if split_matches else None,
+                    "algorithmic_bytes": row["bytes_per_launch"], "launch_us": row["event_us"],
+                    "duration_source": "HIP events around each launch on its stream, timed steps",
+                    "launch_us_replayed": replay,
+                    "achieved_replayed": round(row["bytes_per_launch"] / (replay * 1e-6) / 1e9, 1) if replay else None,
+                    "counters_split": committed["split"], "counters_match_split": split_matches}
+        insts = committed["valu"].get(dominant) if split_matches else None
+        if insts:
+            rate = insts / (row["event_us"] * 1e-6) / 1e9
+            valu = {"kernel": dominant, "insts_per_launch": insts, "achieved": round(rate, 1), "peak": VALU_PEAK_G,
+                    "unit": "G wave-instr/s", "frac": round(rate / VALU_PEAK_G, 4)}
 
-    unit = "Mcoeff/s" if kind == "conv" else "Mlimbs/s"
-    value = la / (ms_clean * 1e-3) / 1e6
+    kind = head["kind"]
+    unit = head["rate_unit"]
+    # The workload's K steps are timed five times (each between device
+    # synchronizations); the line reports the median run.
+    ms_step = head["ms_per_step_median"]
+    value = head["len_a"] / (ms_step * 1e-3) / 1e6
     published = PUBLISHED.get(args.workload)
     result = {
         "metric": "Mlimbs/s at 2^20-limb mul" if args.workload == "mul_2p20" else f"{unit} ({args.workload})",
@@ -650,33 +812,51 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_clean, 4),
-        "ms_per_step_with_events": round(ms_step, 4),
+        "ms_per_step": ms_step,
+        "ms_per_step_runs": head["ms_per_step_runs"],
+        "ms_per_step_with_events": head["ms_per_step_with_events"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": round(value / published["value"], 1) if published else None,
         "baseline": published,
         "dtype": "f64",
         "data": "synthetic (SplitMix64 limbs, seeds 1 and 2), operands resident in HBM",
-        "config": {"workload": args.workload, "len_a": la, "len_b": lb, "primes": primes,
-                   "chunk_bits": chunk, "transform_len": T, "cyclic_len": 1 << log_len},
-        "kernels": per_kernel,
+        "config": {"workload": args.workload, "len_a": head["len_a"], "len_b": head["len_b"],
+                   "primes": head["primes"], "chunk_bits": head["chunk_bits"],
+                   "transform_len": ntt.row_len(head["log_len"], head["row_tail"]),
+                   "cyclic_len": 1 << head["log_len"], "transform_split": head["transform_split"]},
+        "kernels": kernels,
+        "ntt_passes": head["ntt_passes"],
+        "check": head["check"],
         "roofline": roofline,
-        "valu": result_valu,
+        "valu": valu,
     }
     if kind == "mul" and not args.no_host:
-        result["host_buffers"] = bench_host_path(a, b, max(3, args.steps // 5))
+        wl = workload(args.workload)
+        result["host_buffers"] = bench_host_path(wl["a"], wl["b"], max(3, args.steps // 5))
+    if not args.no_configs and args.workload == "mul_2p20":
+        configs = {"config1_1024": bench_config1()}
+        for name, steps in (("sqr_2p24", 10), ("ones_mul_2p20", 20), ("ones_sqr_2p24", 10), ("conv16_2p22", 10),
+                            ("conv32s_2p22", 10)):
+            r = gpu_workload(name, steps, 2)
+            pub = PUBLISHED.get(name)
+            if pub:
+                r["vs_reference_published"] = round(r["rate"] / pub["value"], 1)
+            configs[name] = r
+        configs["ones_vs_random"] = {
+            "mul_2p20": round(configs["ones_mul_2p20"]["ms_per_step_median"] / head["ms_per_step_median"], 3),
+            "sqr_2p24": round(configs["ones_sqr_2p24"]["ms_per_step_median"] / configs["sqr_2p24"]["ms_per_step_median"], 3),
+        }
+        result["configs"] = configs
     if not args.no_small:
         result["small"] = bench_small()
     if not args.no_batch:
         result["batched"] = bench_batch(args.batch_steps, 2, 2)
     if not args.no_serve:
         result["served"] = bench_served(args.serve_requests, args.serve_load)
-    if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
-        # The same oracle with the host threads one GPU's share of the box
-        # offers (16), as a second reported baseline.
-        result["cpu_baseline_threads"] = cpu_baseline(wl, args.cpu_seconds / 2, min(16, os.cpu_count() or 1))
+    if not args.no_cpu and args.workload == "mul_2p20":
+        wl = workload(args.workload)
+        result.update(cpu_baselines(np.ascontiguousarray(wl["a"]), np.ascontiguousarray(wl["b"]), args.cpu_seconds))
     print(json.dumps(result), flush=True)
     return 0
 

@@ -357,6 +357,26 @@ void ntt_profile_enable(int on);
 int ntt_profile_read(char *names, size_t names_len, double *ms, uint64_t *counts, int cap);
 void ntt_profile_reset(void);
 
+/* Re-issues the last launch of kernel kind `kind` recorded while profiling
+ * (same kernel, grid and argument block, on the stream it ran on) `reps`
+ * times back to back between two HIP events on that stream, after one warm
+ * launch, and writes the average duration per launch in milliseconds: the
+ * kernel's own time without per-launch event overhead. The replays rewrite
+ * that launch's output buffers, so run the operation again before reading
+ * its result. false when no launch of that kind was recorded. No
+ * counterpart in the reference (measurement). */
+bool ntt_profile_replay(const char *kind, int reps, double *avg_ms);
+
+/* Pass split the engine runs for a transform of 2^log_len points as a
+ * single operation's product (use 0), its square (use 1) or a batch bucket
+ * (use 2): returns the number of passes and writes their level counts
+ * (strided passes first, the row pass last) to pass_logs[0..n); the split is
+ * the one the autotuner chose for this length and use, the one
+ * NTTAMD_ROW_LOG / NTTAMD_PASSES force, or the default. 0 for lengths out of
+ * range. No counterpart in the reference (the reference's split lives in its
+ * Geometry). */
+int ntt_transform_split(uint32_t log_len, int use, uint32_t *pass_logs);
+
 /* ---------------------------------------------------------------------------
  * Prime fields and two-prime CRT: the public scalar API of the reference
  * (src/field/prime_field.rs:91-442). Host functions (no GPU). A residue is a

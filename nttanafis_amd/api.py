@@ -570,3 +570,25 @@ class Profile:
         n = _lib().ntt_profile_read(names, len(names), ms, counts, cap)
         raw = names.raw.split(b"\0")
         return {raw[i].decode(): (ms[i], counts[i]) for i in range(n)}
+
+    @staticmethod
+    def replay(kind: str, reps: int = 20) -> Optional[float]:
+        """Average duration (ms) of `reps` back-to-back re-issues of the last
+        recorded launch of `kind` on its own stream, or None when no launch of
+        that kind was recorded. The replays rewrite that launch's outputs."""
+        out = ctypes.c_double(0.0)
+        if not _lib().ntt_profile_replay(kind.encode(), int(reps), ctypes.byref(out)):
+            return None
+        return out.value
+
+
+SPLIT_USES = {"product": 0, "square": 1, "batch": 2}
+
+
+def transform_split(log_len: int, use: str = "product"):
+    """Level counts of the passes the engine runs for a transform of
+    2^log_len points (strided passes, then the row pass) for a single
+    operation's product or square, or a batch bucket, as tuned."""
+    logs = (ctypes.c_uint32 * 3)()
+    n = _lib().ntt_transform_split(log_len, SPLIT_USES[use], logs)
+    return [int(logs[i]) for i in range(n)]

@@ -183,7 +183,7 @@ struct CrtArgs {
     const BatchItem* items;
     const BatchItem* scan_items;
     uint32_t scan_count;
-    uint32_t pad2;
+    uint32_t wide;  // single operation: one 512-thread block covers the whole output
     // Schoolbook products: the operand limbs (single operation; a batch
     // reads them from src_a and src_b of each product).
     const uint64_t* op_a;
@@ -220,6 +220,14 @@ struct RecombineArgs {
     double t[kMaxPrimes][3];  // W3, W3^2, W3^3
     double half[kMaxPrimes];  // 2^-1
 };
+
+// Lines per strided tile of lines of 2^logl points: up to 2^7 points 64 at a
+// time (512-byte runs), 2^8 points 32 at a time (256-byte runs); longer
+// lines fill 8192-point tiles. A pass needs its line stride to be at least
+// this many points.
+__host__ __device__ constexpr int strided_lines(int logl) {
+    return logl <= 7 ? 64 : logl == 8 ? 32 : (logl <= 10 ? 8 : 4);
+}
 
 constexpr uint32_t kInf = 0xFFFFFFFFu;
 // Carries across the blocks of an output: up to kCarryLookBackBlocks blocks

@@ -169,7 +169,10 @@ struct Split {
     int logs[3];  // strided passes, then the row pass (last)
 };
 
-Split split_of(int K);
+// Which tuned split a transform takes: a single operation's product (two
+// operands), its square (one operand), or a batch bucket.
+enum SplitUse { kUseProduct = 0, kUseSquare = 1, kUseBatch = 2 };
+Split split_of(int K, int use);
 
 struct BatchLaunch {
     enum Kind { kFwdStrided, kFused, kInvStrided, kCrt, kSchool } kind;

@@ -162,6 +162,34 @@ void prof_record(const char* kind, hipEvent_t start, hipEvent_t stop) {
     g_prof_pending.push_back({kind, start, stop});
 }
 
+// The last launch of each kind while profiling, for replays.
+struct ProfLaunch {
+    std::string kind;
+    const void* fn;
+    dim3 grid, block;
+    hipStream_t stream;
+    std::vector<unsigned char> arg;
+};
+static std::vector<ProfLaunch> g_prof_last;
+
+void prof_keep(const char* kind, const void* fn, dim3 grid, dim3 block, hipStream_t s, const void* arg,
+               size_t size) {
+    std::lock_guard<std::mutex> g(g_prof_lock);
+    ProfLaunch* slot = nullptr;
+    for (auto& l : g_prof_last)
+        if (l.kind == kind) slot = &l;
+    if (!slot) {
+        g_prof_last.push_back(ProfLaunch{kind, nullptr, dim3(), dim3(), nullptr, {}});
+        slot = &g_prof_last.back();
+    }
+    slot->fn = fn;
+    slot->grid = grid;
+    slot->block = block;
+    slot->stream = s;
+    const unsigned char* b = static_cast<const unsigned char*>(arg);
+    slot->arg.assign(b, b + size);
+}
+
 static void prof_drain() {
     std::lock_guard<std::mutex> g(g_prof_lock);
     for (auto& e : g_prof_pending) {
@@ -496,78 +524,104 @@ Executor* plan_executor() {
 
 
 static std::mutex g_tune_lock;
-static int g_tuned_row_log[kMaxLog + 1] = {0};  // 0: not tuned yet
 
-static Split split_with_rows(int K, int rows) {
+// Splits of a transform of 2^K points, K > 12: a row pass of 2^rows points
+// (5 <= rows <= 12) after one strided pass (2^13 .. 2^23 points) or two
+// (from 2^18 points on), each of 5 to 11 levels. A strided pass needs a line
+// stride of at least its tile's lines (strided_lines).
+static Split split_rows(int K, int rows, int passes) {
     Split s{};
-    if (K <= kSinglePassMaxLog) {
-        s.n = 1;
-        s.logs[0] = K;
-        return s;
-    }
-    if (K <= 23) {
+    if (passes == 2) {
         s.n = 2;
         s.logs[0] = K - rows;
         s.logs[1] = rows;
-        return s;
+    } else {
+        const int k1 = (K - rows + 1) / 2;
+        s.n = 3;
+        s.logs[0] = k1;
+        s.logs[1] = K - rows - k1;
+        s.logs[2] = rows;
     }
-    const int k1 = (K - rows + 1) / 2;
-    s.n = 3;
-    s.logs[0] = k1;
-    s.logs[1] = K - rows - k1;
-    s.logs[2] = rows;
     return s;
 }
 
-// Row-pass sizes whose strided passes stay within 5..11 levels, largest
-// first (at most eight), in a fixed array (split_of runs on the
-// allocation-free path too).
-struct RowCandidates {
+static bool split_ok(int K, const Split& s) {
+    int level = 0;
+    for (int i = 0; i + 1 < s.n; ++i) {
+        if (s.logs[i] < kMinLog || s.logs[i] > 11) return false;
+        const int stride_log = K - level - s.logs[i];
+        if (stride_log < 0 || ((uint64_t)1 << stride_log) < (uint64_t)strided_lines(s.logs[i])) return false;
+        level += s.logs[i];
+    }
+    const int rows = s.logs[s.n - 1];
+    return rows >= kMinLog && rows <= kSinglePassMaxLog && level + rows == K;
+}
+
+static bool same_split(const Split& a, const Split& b) {
+    if (a.n != b.n) return false;
+    for (int i = 0; i < a.n; ++i)
+        if (a.logs[i] != b.logs[i]) return false;
+    return true;
+}
+
+// Admissible splits, in a fixed array (split_of runs on the allocation-free
+// path too): two-pass splits first, each form from the largest row down.
+struct SplitCandidates {
     int n = 0;
-    int r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int* begin() const { return r; }
-    const int* end() const { return r + n; }
+    Split s[16];
+    const Split* begin() const { return s; }
+    const Split* end() const { return s + n; }
     bool empty() const { return n == 0; }
     size_t size() const { return (size_t)n; }
-    int operator[](int i) const { return r[i]; }
-    void push_back(int v) { r[n++] = v; }
+    const Split& operator[](int i) const { return s[i]; }
 };
 
-static RowCandidates row_candidates(int K) {
-    RowCandidates c;
+static SplitCandidates split_candidates(int K) {
+    SplitCandidates c;
     if (K <= kSinglePassMaxLog) return c;
-    for (int r = 12; r >= 5 && c.size() < 8; --r) {
-        const int rest = K - r;
-        if (K <= 23) {
-            if (rest < 5 || rest > 11) continue;
-        } else {
-            const int k1 = (rest + 1) / 2;
-            if (k1 > 11 || rest - k1 < 5) continue;
+    for (int passes = 2; passes <= 3; ++passes) {
+        if ((passes == 2 && K > 23) || (passes == 3 && K < 18)) continue;
+        for (int r = kSinglePassMaxLog; r >= kMinLog && c.n < 16; --r) {
+            const Split sp = split_rows(K, r, passes);
+            if (split_ok(K, sp)) c.s[c.n++] = sp;
         }
-        c.push_back(r);
     }
     return c;
 }
 
-// Split of a transform of 2^K points: a single pass up to 2^12; otherwise a
-// row pass of 2^rows points after one or two strided passes, with rows from
-// NTTAMD_ROW_LOG when set and admissible, else from the autotuner once it
-// has timed this length, else 12 (as close to 12 as the strided passes
-// allow).
-static thread_local int t_force_rows = 0;  // set while the autotuner times a candidate
+// Tuned splits per use and length (n == 0: not tuned yet).
+static Split g_tuned[3][kMaxLog + 1];
 
-Split split_of(int K) {
-    if (K <= kSinglePassMaxLog) return split_with_rows(K, K);
-    const RowCandidates cand = row_candidates(K);
-    int rows = t_force_rows;
-    if (!rows)
-        if (const char* e = getenv("NTTAMD_ROW_LOG")) rows = atoi(e);
-    if (!rows && K <= kMaxLog) {
-        std::lock_guard<std::mutex> g(g_tune_lock);
-        rows = g_tuned_row_log[K];
+// Split of a transform of 2^K points: a single pass up to 2^12; otherwise
+// the autotuner's candidate while it times one, the candidate that
+// NTTAMD_ROW_LOG (and NTTAMD_PASSES, 2 or 3) name when set and admissible,
+// the tuned split of this use once the autotuner timed this length, else
+// the first candidate (two passes with the largest row below 2^24 points,
+// three from there on).
+static thread_local Split t_force{};  // set while the autotuner times a candidate
+
+Split split_of(int K, int use) {
+    if (K <= kSinglePassMaxLog) {
+        Split s{};
+        s.n = 1;
+        s.logs[0] = K;
+        return s;
     }
-    if (std::find(cand.begin(), cand.end(), rows) == cand.end()) rows = cand.empty() ? 12 : cand[0];
-    return split_with_rows(K, rows);
+    if (t_force.n) return t_force;
+    const SplitCandidates cand = split_candidates(K);
+    const char* er = getenv("NTTAMD_ROW_LOG");
+    const char* ep = getenv("NTTAMD_PASSES");
+    if (er || ep) {
+        const int rows = er ? atoi(er) : 0, passes = ep ? atoi(ep) : 0;
+        for (const Split& s : cand)
+            if ((!rows || s.logs[s.n - 1] == rows) && (!passes || s.n == passes)) return s;
+    }
+    if (K <= kMaxLog && use >= 0 && use < 3) {
+        std::lock_guard<std::mutex> g(g_tune_lock);
+        if (g_tuned[use][K].n) return g_tuned[use][K];
+    }
+    if (!cand.empty()) return cand[0];
+    return split_rows(K, kSinglePassMaxLog, K <= 23 ? 2 : 3);
 }
 
 // Pass arguments of a transform of 2^log_n points over rows of row_len(g)
@@ -716,7 +770,7 @@ static bool part_residues(hipStream_t s, const Geometry& g, const RowPart& part,
     const int np = (int)g.primes;
     double* A = buf.spec_a + part.offset;
     double* B = (square ? buf.spec_a : buf.spec_b) + part.offset;
-    const Split sp = split_of(part.log_n);
+    const Split sp = split_of(part.log_n, square ? kUseSquare : kUseProduct);
     if (sp.n == 1) {
         // One workgroup per prime packs both operands, transforms, multiplies
         // and stores the product rows into B.
@@ -791,7 +845,7 @@ static bool split_residues(hipStream_t s, const Geometry& g, const Source& sa, c
     a.other = buf.spec_b;
     a.src_a = lng;
     a.src_b = sht;
-    const Split sp = split_of(g.log_t);
+    const Split sp = split_of(g.log_t, kUseProduct);
     *rows_out = buf.spec_a;
     if (sp.n == 1) {
         Timed t("fused_single", s);
@@ -871,6 +925,13 @@ static bool crt_product_from(hipStream_t s, const Geometry& g, const double* row
     a.coefficients = std::min<uint64_t>(g.conv_len, (out_len * 64 + g.width - 1) / g.width);
     const uint64_t blocks = (out_len + kCrtBlockLimbs - 1) / kCrtBlockLimbs;
     a.blocks = (uint32_t)blocks;
+    // An output of at most two blocks whose coefficients fit one 512-thread
+    // block's words is reconstructed by that one block, carries included.
+    const uint64_t wide_span = (64 * 2 * (uint64_t)kCrtBlockLimbs + g.bound_bits) / g.width + 2;
+    if (blocks == 2 && wide_span * (uint64_t)crt_words((int)g.primes) <= 2 * 4096) {
+        a.wide = 1;
+        a.blocks = 1;
+    }
     a.agg_base = carry;
     a.agg_thresh = carry + blocks;
     a.first_thresh = carry + 2 * blocks;
@@ -959,18 +1020,20 @@ static bool device_product(Executor& ex, const Geometry& g, const uint64_t* d_a,
 }
 
 // Plan-time autotuning of a single operation's split: the first prepare of a
-// transform length times every admissible row size on the executor's own
-// buffers (operands of the plan's shape, contents irrelevant to the timing)
-// and records the fastest for every later call of that length.
-static void tune_split(Executor& ex, int K, const std::function<bool()>& run) {
-    if (K < 0 || K > kMaxLog) return;
-    if (getenv("NTTAMD_ROW_LOG")) return;  // a forced split needs no trials
+// transform length times every admissible split on the executor's own
+// buffers (operands of the plan's shape, contents irrelevant to the timing),
+// for the product and, when the operands are as long, for the square, and
+// records the fastest for every later call of that length and use.
+static void tune_split(Executor& ex, int K, int use, const std::function<bool()>& run) {
+    if (K < 0 || K > kMaxLog || use < 0 || use > 2) return;
+    if (getenv("NTTAMD_ROW_LOG") || getenv("NTTAMD_PASSES")) return;  // a forced split needs no trials
     {
         std::lock_guard<std::mutex> g(g_tune_lock);
-        if (g_tuned_row_log[K]) return;
+        if (g_tuned[use][K].n) return;
     }
-    const RowCandidates cand = row_candidates(K);
-    int best = cand.empty() ? K : cand[0];
+    const SplitCandidates cand = split_candidates(K);
+    if (cand.empty()) return;
+    Split best = cand[0];
     if (cand.size() > 1) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
@@ -978,8 +1041,8 @@ static void tune_split(Executor& ex, int K, const std::function<bool()>& run) {
             return;
         }
         float best_ms = 1e30f;
-        for (int rows : cand) {
-            t_force_rows = rows;
+        for (const Split& sp : cand) {
+            t_force = sp;
             bool ok = run();  // warm
             hipEventRecord(e0, ex.stream);
             for (int r = 0; r < 3 && ok; ++r) ok = run();
@@ -988,17 +1051,17 @@ static void tune_split(Executor& ex, int K, const std::function<bool()>& run) {
             if (ok && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
             if (ok && ms < best_ms) {
                 best_ms = ms;
-                best = rows;
+                best = sp;
             }
         }
-        t_force_rows = 0;
+        t_force = Split{};
         hipStreamSynchronize(ex.stream);
         hipEventDestroy(e0);
         hipEventDestroy(e1);
         clear_error();
     }
     std::lock_guard<std::mutex> g(g_tune_lock);
-    if (!g_tuned_row_log[K]) g_tuned_row_log[K] = best;
+    if (!g_tuned[use][K].n) g_tuned[use][K] = best;
 }
 
 // ---------------------------------------------------------------------------
@@ -1245,7 +1308,7 @@ static void build_program(Batch& B) {
     const BatchItem* I = B.items.as<BatchItem>();
     for (const BatchBucket& r : B.buckets) {
         B.seg.push_back(B.program.size());
-        bucket_program(B.program, bucket_base(B, r.K), r.K, split_of(r.K), I, r);
+        bucket_program(B.program, bucket_base(B, r.K), r.K, split_of(r.K, kUseBatch), I, r);
     }
     // The schoolbook launch (np == 0) depends on no transform: its own
     // segment; the reconstructions follow every bucket.
@@ -1461,22 +1524,23 @@ static bool batch_autotune(Batch& B) {
     for (const BatchBucket& r : B.buckets) {
         {
             std::lock_guard<std::mutex> g(g_tune_lock);
-            if (g_tuned_row_log[r.K]) continue;
+            if (g_tuned[kUseBatch][r.K].n) continue;
         }
-        const RowCandidates cand = row_candidates(r.K);
-        int best = cand.empty() ? r.K : cand[0];
+        const SplitCandidates cand = split_candidates(r.K);
+        if (cand.empty()) continue;
+        Split best = cand[0];
         if (cand.size() > 1) {
             float best_ms = 1e30f;
-            for (int rows : cand) {
+            for (const Split& sp : cand) {
                 // The bucket's whole transform work under this split (it
                 // recomputes everything from the operands, so repeats are
                 // harmless).
                 std::vector<BatchLaunch> prog;
-                bucket_program(prog, bucket_base(B, r.K), r.K, split_with_rows(r.K, rows), I, r);
+                bucket_program(prog, bucket_base(B, r.K), r.K, sp, I, r);
                 const float ms = time_launches(prog, B.ex.stream, 3);
                 if (ms < best_ms) {
                     best_ms = ms;
-                    best = rows;
+                    best = sp;
                 }
             }
             if (hipStreamSynchronize(B.ex.stream) != hipSuccess) {
@@ -1485,8 +1549,8 @@ static bool batch_autotune(Batch& B) {
             }
         }
         std::lock_guard<std::mutex> g(g_tune_lock);
-        if (!g_tuned_row_log[r.K]) {
-            g_tuned_row_log[r.K] = best;
+        if (!g_tuned[kUseBatch][r.K].n) {
+            g_tuned[kUseBatch][r.K] = best;
             changed = true;
         }
     }
@@ -1902,10 +1966,15 @@ void ntt_mul_workspace_prepare(ntt_mul_workspace* ws, const ntt_mul_plan* plan) 
     if (g.log_t > kSinglePassMaxLog && ex.in_a.ensure(la * 8) && ex.in_b.ensure(lb * 8) &&
         ex.out.ensure((la + lb) * 8) && hipMemsetAsync(ex.in_a.ptr, 0x5a, la * 8, ex.stream) == hipSuccess &&
         hipMemsetAsync(ex.in_b.ptr, 0xa5, lb * 8, ex.stream) == hipSuccess) {
-        tune_split(ex, g.log_t, [&] {
+        tune_split(ex, g.log_t, kUseProduct, [&] {
             return device_product(ex, g, ex.in_a.as<uint64_t>(), la, ex.in_b.as<uint64_t>(), lb, false,
                                   ex.out.as<uint64_t>());
         });
+        if (la == lb && !g.blocks)
+            tune_split(ex, g.log_t, kUseSquare, [&] {
+                return device_product(ex, g, ex.in_a.as<uint64_t>(), la, ex.in_a.as<uint64_t>(), la, true,
+                                      ex.out.as<uint64_t>());
+            });
     }
 }
 
@@ -2041,11 +2110,17 @@ void ntt_conv_workspace_prepare(ntt_conv_workspace* ws, const ntt_conv_plan* pla
         hipMemsetAsync(ex.in_a.ptr, 0, la * 8, ex.stream) == hipSuccess &&
         hipMemsetAsync(ex.in_b.ptr, 0, lb * 8, ex.stream) == hipSuccess) {
         OpBuffers buf;
-        if (exec_buffers(ex, g, false, 0, &buf))
-            tune_split(ex, g.log_t, [&] {
+        if (exec_buffers(ex, g, false, 0, &buf)) {
+            tune_split(ex, g.log_t, kUseProduct, [&] {
                 return convolution_on(ex.stream, g, plan, ex.in_a.as<uint64_t>(), ex.in_b.as<uint64_t>(), false, buf,
                                       ex.out.as<uint64_t>());
             });
+            if (la == lb && !g.blocks)
+                tune_split(ex, g.log_t, kUseSquare, [&] {
+                    return convolution_on(ex.stream, g, plan, ex.in_a.as<uint64_t>(), ex.in_a.as<uint64_t>(), true,
+                                          buf, ex.out.as<uint64_t>());
+                });
+        }
     }
 }
 
@@ -2306,6 +2381,45 @@ void ntt_profile_reset(void) {
     prof_drain();
     std::lock_guard<std::mutex> g(g_prof_lock);
     g_prof_totals.clear();
+    g_prof_last.clear();
+}
+
+bool ntt_profile_replay(const char* kind, int reps, double* avg_ms) {
+    if (!kind || reps < 1 || !avg_ms) return false;
+    ProfLaunch l;
+    {
+        std::lock_guard<std::mutex> g(g_prof_lock);
+        bool found = false;
+        for (auto& x : g_prof_last)
+            if (x.kind == kind) {
+                l = x;
+                found = true;
+            }
+        if (!found) return false;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) hipEventDestroy(e0);
+        return false;
+    }
+    void* args[1] = {l.arg.data()};
+    bool ok = hipLaunchKernel(l.fn, l.grid, l.block, args, 0, l.stream) == hipSuccess &&  // warm
+              hipEventRecord(e0, l.stream) == hipSuccess;
+    for (int r = 0; r < reps && ok; ++r) ok = hipLaunchKernel(l.fn, l.grid, l.block, args, 0, l.stream) == hipSuccess;
+    ok = ok && hipEventRecord(e1, l.stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+    float ms = 0;
+    if (ok) ok = hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (ok) *avg_ms = ms / reps;
+    return ok;
+}
+
+int ntt_transform_split(uint32_t log_len, int use, uint32_t* pass_logs) {
+    if (log_len < (uint32_t)kMinLog || log_len > (uint32_t)kMaxLog || use < 0 || use > 2) return 0;
+    const Split sp = split_of((int)log_len, use);
+    for (int i = 0; i < sp.n && pass_logs; ++i) pass_logs[i] = (uint32_t)sp.logs[i];
+    return sp.n;
 }
 
 int ntt_profile_read(char* names, size_t names_len, double* ms, uint64_t* counts, int cap) {

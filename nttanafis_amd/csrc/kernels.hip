@@ -27,6 +27,16 @@ namespace nttamd {
 bool prof_enabled();
 const char* prof_kind();
 void prof_record(const char* kind, hipEvent_t start, hipEvent_t stop);
+// Keeps the last launch of a kind (kernel, shape, stream, argument block) for
+// ntt_profile_replay: kernels whose only argument is one argument block.
+void prof_keep(const char* kind, const void* fn, dim3 grid, dim3 block, hipStream_t s, const void* arg,
+               size_t size);
+
+template <class... A>
+inline void prof_remember(const char* kind, const void* fn, dim3 grid, dim3 block, hipStream_t s,
+                          const A&... args) {
+    if constexpr (sizeof...(A) == 1) (prof_keep(kind, fn, grid, block, s, &args, sizeof(args)), ...);
+}
 
 #define NTT_LAUNCH(KIND, KERNEL, GRID, BLOCK, STREAM, ...)                                 \
     do {                                                                                   \
@@ -34,8 +44,9 @@ void prof_record(const char* kind, hipEvent_t start, hipEvent_t stop);
         if (prof_enabled() && hipEventCreate(&e0_) == hipSuccess &&                        \
             hipEventCreate(&e1_) == hipSuccess) {                                          \
             hipExtLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, STREAM, e0_, e1_, 0, __VA_ARGS__); \
-            const char* k_ = (KIND);                                                       \
-            prof_record(k_ ? k_ : prof_kind(), e0_, e1_);                                  \
+            const char* k_ = (KIND) ? (KIND) : prof_kind();                                \
+            prof_record(k_, e0_, e1_);                                                     \
+            prof_remember(k_, reinterpret_cast<const void*>(KERNEL), GRID, BLOCK, STREAM, __VA_ARGS__); \
         } else {                                                                           \
             hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, STREAM, __VA_ARGS__);              \
         }                                                                                  \
@@ -912,8 +923,15 @@ __global__ __launch_bounds__(256) void k_recombine(RecombineArgs a) {
 // order. Carries inside the block then resolve by a scan of carry functions
 // (per wave with lane shuffles, then across the block's waves). Blocks
 // publish their carry function for k_carry_apply (the block merge of
-// src/limb/reconstruction.rs:211-247).
-constexpr int kCrtLdsWords = 4096;
+// src/limb/reconstruction.rs:211-247). A workgroup of NT threads covers
+// 4*NT limbs (kCrtBlockLimbs for the 256-thread form); the 512-thread form
+// reconstructs a whole output of up to 2048 limbs in one block, which needs
+// no carry pass after it.
+template <int NT>
+struct CrtShape {
+    static constexpr int BLOCK = NT * kCrtLimbsPerThread;  // limbs per block
+    static constexpr int LDS_WORDS = 16 * NT;               // coefficient words per block
+};
 
 __device__ __forceinline__ CarryFn shfl_up_fn(CarryFn f, int d) {
     CarryFn r;
@@ -991,9 +1009,10 @@ __device__ __forceinline__ CrtJob crt_job(const CrtArgs& a) {
 // Second half of a reconstruction block: acc[j] holds limb j's sum mod 2^64
 // and wraps[j] the carries it sends on. Resolves the carries inside the
 // block, stores the limbs and publishes the block's carry function.
+template <int NT>
 __device__ __forceinline__ void block_finish(const CrtJob& J, unsigned long long* acc, uint32_t* wraps,
                                              CarryFn* wave_total, uint32_t* ripple_s) {
-    const uint64_t L0 = (uint64_t)J.block * kCrtBlockLimbs;
+    const uint64_t L0 = (uint64_t)J.block * CrtShape<NT>::BLOCK;
     uint64_t v[kCrtLimbsPerThread];
     CarryFn f[kCrtLimbsPerThread];
 #pragma unroll
@@ -1045,10 +1064,10 @@ __device__ __forceinline__ void block_finish(const CrtJob& J, unsigned long long
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kCrtLimbsPerThread; ++q) {
-        const uint64_t j = L0 + threadIdx.x + q * kCrtThreads;
-        if (j < J.out_len) J.out[j] = acc[threadIdx.x + q * kCrtThreads];
+        const uint64_t j = L0 + threadIdx.x + q * NT;
+        if (j < J.out_len) J.out[j] = acc[threadIdx.x + q * NT];
     }
-    if (threadIdx.x == kCrtThreads - 1) {
+    if (threadIdx.x == NT - 1) {
         J.agg_base[J.block] = g.base;
         J.agg_thresh[J.block] = g.thresh;
     }
@@ -1058,22 +1077,23 @@ __device__ __forceinline__ void block_finish(const CrtJob& J, unsigned long long
     }
 }
 
-template <int NP>
-__global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
+template <int NP, int NT>
+__global__ __launch_bounds__(NT) void k_crt_product(CrtArgs a) {
+    constexpr int BLOCK = CrtShape<NT>::BLOCK;
     constexpr int NW = CrtWords<NP>::value;
-    constexpr int RMAX = (kCrtLdsWords / NW + kCrtThreads - 1) / kCrtThreads;
-    constexpr int WAVES = kCrtThreads / 64;
-    __shared__ unsigned long long acc[kCrtBlockLimbs];
-    __shared__ uint32_t wraps[kCrtBlockLimbs];
+    constexpr int RMAX = (CrtShape<NT>::LDS_WORDS / NW + NT - 1) / NT;
+    constexpr int WAVES = NT / 64;
+    __shared__ unsigned long long acc[BLOCK];
+    __shared__ uint32_t wraps[BLOCK];
     __shared__ CarryFn wave_total[WAVES];
     __shared__ uint32_t ripple;
     const CrtJob J = crt_job(a);
-    const uint64_t L0 = (uint64_t)J.block * kCrtBlockLimbs;
+    const uint64_t L0 = (uint64_t)J.block * BLOCK;
     const uint64_t w = J.width;
     const uint64_t xb = J.value_bits;
     const uint64_t lo_bit = L0 * 64;
     const uint64_t i_lo = lo_bit > xb ? (lo_bit - xb) / w : 0;
-    uint64_t i_hi = (lo_bit + 64 * (uint64_t)kCrtBlockLimbs - 1) / w + 1;
+    uint64_t i_hi = (lo_bit + 64 * (uint64_t)BLOCK - 1) / w + 1;
     if (i_hi > J.coefficients) i_hi = J.coefficients;
     const int count = i_hi > i_lo ? (int)(i_hi - i_lo) : 0;
     // Residues of this thread's coefficients, every load issued first.
@@ -1081,7 +1101,7 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
     if (J.split_blocks == 0) {
 #pragma unroll
         for (int q = 0; q < RMAX; ++q) {
-            const int idx = threadIdx.x + q * kCrtThreads;
+            const int idx = threadIdx.x + q * NT;
 #pragma unroll
             for (int k = 0; k < NP; ++k)
                 r[q][k] = idx < count ? J.residues[(uint64_t)k * J.row_stride + i_lo + idx] : 0.0;
@@ -1094,7 +1114,7 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
         const uint64_t D = J.split_digits;
 #pragma unroll
         for (int q = 0; q < RMAX; ++q) {
-            const int idx = threadIdx.x + q * kCrtThreads;
+            const int idx = threadIdx.x + q * NT;
             const uint64_t i = i_lo + idx;
             uint64_t z1 = i / D;
             if (z1 >= J.split_blocks) z1 = J.split_blocks - 1;
@@ -1118,19 +1138,19 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
     }
 #pragma unroll
     for (int q = 0; q < kCrtLimbsPerThread; ++q) {
-        acc[threadIdx.x + q * kCrtThreads] = 0;
-        wraps[threadIdx.x + q * kCrtThreads] = 0;
+        acc[threadIdx.x + q * NT] = 0;
+        wraps[threadIdx.x + q * NT] = 0;
     }
     if (threadIdx.x == 0) ripple = 0;
     __syncthreads();
     // Coefficient i starts at bit i*w - 64*L0 of the block, which lies in
-    // [-xb, 64*kCrtBlockLimbs); offsets from bit i_lo*w keep the arithmetic
+    // [-xb, 64*BLOCK); offsets from bit i_lo*w keep the arithmetic
     // in 32 bits. Word m of X_i << s lands in limb qrel + m.
     const uint32_t wr = (uint32_t)w;
     const uint32_t rel0 = (uint32_t)(lo_bit - i_lo * w);  // bit of limb L0 from coefficient i_lo
 #pragma unroll
     for (int q = 0; q < RMAX; ++q) {
-        const int idx = threadIdx.x + q * kCrtThreads;
+        const int idx = threadIdx.x + q * NT;
         if (idx < count) {
             uint64_t X[NW];
             garner_r<NP>(a, r[q], X, false);
@@ -1144,7 +1164,7 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
                 const uint64_t word = sh ? (cur << sh) | (prev >> (64 - sh)) : cur;
                 prev = cur;
                 const int32_t j = qrel + m;
-                if (word != 0 && j >= 0 && j < kCrtBlockLimbs) {
+                if (word != 0 && j >= 0 && j < BLOCK) {
                     const unsigned long long old = atomicAdd(&acc[j], (unsigned long long)word);
                     if (old + word < old) atomicAdd(&wraps[j], 1u);
                 }
@@ -1152,7 +1172,7 @@ __global__ __launch_bounds__(kCrtThreads) void k_crt_product(CrtArgs a) {
         }
     }
     __syncthreads();
-    block_finish(J, acc, wraps, wave_total, &ripple);
+    block_finish<NT>(J, acc, wraps, wave_total, &ripple);
 }
 
 // Schoolbook product, no transform, for a shorter operand of at most
@@ -1343,8 +1363,6 @@ __global__ __launch_bounds__(kSchoolThreads) void k_school(CrtArgs a) {
 // Then the first limb of the block takes the carry, and the limbs up to the
 // recorded ripple end take one more when the carry reaches the first limb's
 // threshold.
-constexpr int kApplyThreads = 64;
-
 // Carry bookkeeping of the product a scan or apply workgroup works on.
 struct CarryView {
     uint32_t* words;  // carry_words(blocks) words
@@ -1405,50 +1423,59 @@ __global__ __launch_bounds__(kCarryChunk) void k_carry_scan(CrtArgs a) {
     }
 }
 
+// Threads of an apply workgroup: one wave finds the carry into the block,
+// then all of them add the ripple (up to BL - 1 limbs).
 template <int BL>
-__global__ __launch_bounds__(kApplyThreads) void k_carry_apply(CrtArgs a) {
+__host__ __device__ constexpr int apply_threads() {
+    return BL >= 1024 ? 256 : 64;
+}
+
+template <int BL>
+__global__ __launch_bounds__(apply_threads<BL>()) void k_carry_apply(CrtArgs a) {
+    constexpr int NT = apply_threads<BL>();
     __shared__ uint32_t found;
     __shared__ uint32_t cin_s;
     const CrtJob J = crt_job(a);
     const int b = (int)J.block;
     if (b == 0) return;
-    uint32_t c = 0;
+    const int lane = threadIdx.x;  // wave 0 only below
     if (carry_scanned(J.blocks)) {
-        // Carry into this block's chunk: the chunk functions before it,
-        // composed in order, lane l taking a contiguous slice (the lanes with
-        // a slice are a prefix of the wave).
-        const uint32_t j = (uint32_t)b / (uint32_t)kCarryChunk;
-        uint32_t cin_chunk = 0;
-        if (j > 0) {
-            const uint32_t chunks = carry_chunks(J.blocks);
-            const uint32_t* cb = J.agg_base + 4ull * J.blocks;
-            const uint32_t* ct = cb + chunks;
-            const uint32_t per = (j + kApplyThreads - 1) / kApplyThreads;
-            const uint32_t s0 = threadIdx.x * per;
-            const uint32_t s1 = s0 + per < j ? s0 + per : j;
-            CarryFn agg{0u, kInf};
-            for (uint32_t q = s0; q < s1; ++q) {
-                const CarryFn fq{cb[q], ct[q]};
-                agg = q == s0 ? fq : compose(agg, fq);
-            }
-            const int lane = threadIdx.x;
+        if (threadIdx.x < 64) {
+            // Carry into this block's chunk: the chunk functions before it,
+            // composed in order, lane l taking a contiguous slice (the lanes
+            // with a slice are a prefix of the wave).
+            const uint32_t j = (uint32_t)b / (uint32_t)kCarryChunk;
+            uint32_t cin_chunk = 0;
+            if (j > 0) {
+                const uint32_t chunks = carry_chunks(J.blocks);
+                const uint32_t* cb = J.agg_base + 4ull * J.blocks;
+                const uint32_t* ct = cb + chunks;
+                const uint32_t per = (j + 63) / 64;
+                const uint32_t s0 = lane * per;
+                const uint32_t s1 = s0 + per < j ? s0 + per : j;
+                CarryFn agg{0u, kInf};
+                for (uint32_t q = s0; q < s1; ++q) {
+                    const CarryFn fq{cb[q], ct[q]};
+                    agg = q == s0 ? fq : compose(agg, fq);
+                }
 #pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const CarryFn prev = shfl_up_fn(agg, d);
-                if (lane >= d) agg = compose(prev, agg);
+                for (int d = 1; d < 64; d <<= 1) {
+                    const CarryFn prev = shfl_up_fn(agg, d);
+                    if (lane >= d) agg = compose(prev, agg);
+                }
+                const int last = (int)((j + per - 1) / per) - 1;
+                cin_chunk = __shfl(agg.base, last, 64);  // the composition applied to 0
             }
-            const int last = (int)((j + per - 1) / per) - 1;
-            cin_chunk = __shfl(agg.base, last, 64);  // the composition applied to 0
+            // The block's prefix within its chunk, from k_carry_scan.
+            const uint32_t pb = J.agg_base[b], pt = J.agg_thresh[b];
+            if (lane == 0) cin_s = pt == 0 ? cin_chunk : pb + (cin_chunk >= pt ? 1u : 0u);
         }
-        // The block's prefix within its chunk, from k_carry_scan.
-        const uint32_t pb = J.agg_base[b], pt = J.agg_thresh[b];
-        c = pt == 0 ? cin_chunk : pb + (cin_chunk >= pt ? 1u : 0u);
     } else {
-        // b <= kCarryLookBackBlocks = kApplyThreads: one look-back step.
+        // b <= kCarryLookBackBlocks = 64: one look-back step by wave 0.
         if (threadIdx.x == 0) found = 0xFFFFFFFFu;
         __syncthreads();
-        const int k = b - 1 - (int)threadIdx.x;
-        if (k >= 0 && J.agg_thresh[k] == kInf) atomicMin(&found, (uint32_t)threadIdx.x);
+        const int k = b - 1 - lane;
+        if (lane < 64 && k >= 0 && J.agg_thresh[k] == kInf) atomicMin(&found, (uint32_t)lane);
         __syncthreads();
         const uint32_t t = found;
         // Block whose output is fixed, or -1: begin with a zero carry into block 0.
@@ -1458,15 +1485,15 @@ __global__ __launch_bounds__(kApplyThreads) void k_carry_apply(CrtArgs a) {
             for (int q = start + 1; q < b; ++q) cc = J.agg_base[q] + (cc >= J.agg_thresh[q] ? 1u : 0u);
             cin_s = cc;
         }
-        __syncthreads();
-        c = cin_s;
     }
+    __syncthreads();
+    const uint32_t c = cin_s;
     if (c == 0) return;
     const uint64_t L0 = (uint64_t)b * BL;
     if (threadIdx.x == 0 && L0 < J.out_len) J.out[L0] += c;
     if (c >= J.first_thresh[b]) {
         const uint32_t rend = J.ripple_end[b];
-        for (uint32_t rel = 1 + threadIdx.x; rel <= rend; rel += kApplyThreads) {
+        for (uint32_t rel = 1 + threadIdx.x; rel <= rend; rel += NT) {
             const uint64_t j = L0 + rel;
             if (j < J.out_len) J.out[j] += 1;
         }
@@ -1491,28 +1518,22 @@ __global__ __launch_bounds__(256) void k_crt_disjoint(CrtArgs a) {
 
 #define NTT_CHECK_LAUNCH() (hipGetLastError() == hipSuccess)
 
-// Lines per strided tile: lines of up to 2^7 points are taken 64 at a time
-// (512-byte runs), lines of 2^8 points 32 at a time (256-byte runs); longer
-// lines fill 8192-point tiles. Measured with a plain tile copy at 2^24
-// points x 6 rows: 64 x 128 moves 5.6 TB/s at a first pass's line stride
-// and 6.0 TB/s at a middle pass's, against 5.1 and 5.35 for 32 x 128; 32 x
-// 256 moves 5.3-5.5 TB/s against 4.6-4.8 for 16 x 256.
-template <int LOGL>
-static constexpr int strided_lines() {
-    return LOGL <= 7 ? 64 : LOGL == 8 ? 32 : (LOGL == 9 ? 8 : (LOGL == 10 ? 8 : 4));
-}
+// Lines per strided tile (strided_lines, device.h). Measured with a plain
+// tile copy at 2^24 points x 6 rows: 64 x 128 moves 5.6 TB/s at a first
+// pass's line stride and 6.0 TB/s at a middle pass's, against 5.1 and 5.35
+// for 32 x 128; 32 x 256 moves 5.3-5.5 TB/s against 4.6-4.8 for 16 x 256.
 
 // Threads of a strided tile: one radix-16 group per thread and round.
 template <int LOGL>
 static constexpr int strided_threads() {
-    return strided_lines<LOGL>() * (1 << LOGL) / 16 >= 512 ? 512 : 256;
+    return strided_lines(LOGL) * (1 << LOGL) / 16 >= 512 ? 512 : 256;
 }
 
 // In a batched launch (a.batch set) `units` counts work items (product,
 // prime); otherwise it counts primes.
 template <int LOGL>
 static bool launch_fwd_strided_t(const PassArgs& a, int units, int operands, bool pack, hipStream_t s) {
-    constexpr int LINES = strided_lines<LOGL>();
+    constexpr int LINES = strided_lines(LOGL);
     constexpr int NT = strided_threads<LOGL>();
     if (a.line_stride < (uint64_t)LINES) return false;
     const uint64_t nodes = (uint64_t)1 << a.level_start;
@@ -1566,7 +1587,7 @@ static bool launch_pack_primes_t(const PassArgs& a, int primes, int operands, hi
 
 template <int LOGL>
 static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, hipStream_t s) {
-    constexpr int LINES = strided_lines<LOGL>();
+    constexpr int LINES = strided_lines(LOGL);
     constexpr int NT = strided_threads<LOGL>();
     if (a.line_stride < (uint64_t)LINES) return false;
     const uint64_t nodes = (uint64_t)1 << a.level_start;
@@ -1703,25 +1724,35 @@ static bool launch_carries(const CrtArgs& a, hipStream_t s) {
         if (!NTT_CHECK_LAUNCH()) return false;
     }
     if (a.blocks > 1 || a.batch)
-        NTT_LAUNCH("carry_apply", k_carry_apply<BL>, dim3(a.blocks), dim3(kApplyThreads), s, a);
+        NTT_LAUNCH("carry_apply", k_carry_apply<BL>, dim3(a.blocks), dim3(apply_threads<BL>()), s, a);
     return NTT_CHECK_LAUNCH();
 }
 
-// In a batched launch a.blocks counts (product, block) items.
-bool launch_crt_product(const CrtArgs& a, hipStream_t s) {
-    dim3 grid(a.blocks);
+// In a batched launch a.blocks counts (product, block) items. a.wide: one
+// 512-thread block covers the whole output (at most 2048 limbs), so there
+// are no carries between blocks.
+template <int NT>
+static bool launch_crt_t(const CrtArgs& a, dim3 grid, hipStream_t s) {
     switch (a.primes) {
-        case 1: NTT_LAUNCH("crt_product", k_crt_product<1>, grid, dim3(kCrtThreads), s, a); break;
-        case 2: NTT_LAUNCH("crt_product", k_crt_product<2>, grid, dim3(kCrtThreads), s, a); break;
-        case 3: NTT_LAUNCH("crt_product", k_crt_product<3>, grid, dim3(kCrtThreads), s, a); break;
-        case 4: NTT_LAUNCH("crt_product", k_crt_product<4>, grid, dim3(kCrtThreads), s, a); break;
-        case 5: NTT_LAUNCH("crt_product", k_crt_product<5>, grid, dim3(kCrtThreads), s, a); break;
-        case 6: NTT_LAUNCH("crt_product", k_crt_product<6>, grid, dim3(kCrtThreads), s, a); break;
-        case 7: NTT_LAUNCH("crt_product", k_crt_product<7>, grid, dim3(kCrtThreads), s, a); break;
-        case 8: NTT_LAUNCH("crt_product", k_crt_product<8>, grid, dim3(kCrtThreads), s, a); break;
+        case 1: NTT_LAUNCH("crt_product", (k_crt_product<1, NT>), grid, dim3(NT), s, a); break;
+        case 2: NTT_LAUNCH("crt_product", (k_crt_product<2, NT>), grid, dim3(NT), s, a); break;
+        case 3: NTT_LAUNCH("crt_product", (k_crt_product<3, NT>), grid, dim3(NT), s, a); break;
+        case 4: NTT_LAUNCH("crt_product", (k_crt_product<4, NT>), grid, dim3(NT), s, a); break;
+        case 5: NTT_LAUNCH("crt_product", (k_crt_product<5, NT>), grid, dim3(NT), s, a); break;
+        case 6: NTT_LAUNCH("crt_product", (k_crt_product<6, NT>), grid, dim3(NT), s, a); break;
+        case 7: NTT_LAUNCH("crt_product", (k_crt_product<7, NT>), grid, dim3(NT), s, a); break;
+        case 8: NTT_LAUNCH("crt_product", (k_crt_product<8, NT>), grid, dim3(NT), s, a); break;
         default: return false;
     }
-    if (!NTT_CHECK_LAUNCH()) return false;
+    return NTT_CHECK_LAUNCH();
+}
+
+bool launch_crt_product(const CrtArgs& a, hipStream_t s) {
+    if (a.wide) {
+        if (a.batch || a.blocks != 1 || a.out_len > (uint64_t)CrtShape<2 * kCrtThreads>::BLOCK) return false;
+        return launch_crt_t<2 * kCrtThreads>(a, dim3(1), s);
+    }
+    if (!launch_crt_t<kCrtThreads>(a, dim3(a.blocks), s)) return false;
     return launch_carries<kCrtBlockLimbs>(a, s);
 }
 

@@ -41,14 +41,29 @@ class ConvPlan(ctypes.Structure):
 
 
 _lib = None
+_lib_path = LIB_PATH
+
+
+def use_native() -> bool:
+    """Switches this module to liboracle_native.so (the same source built
+    with -march=native on the calling host, as the CPU baseline wants);
+    False, keeping the portable build, when it cannot be built here. Call it
+    before the first use of the module."""
+    global _lib_path
+    native = os.path.join(HERE, "liboracle_native.so")
+    proc = subprocess.run(["make", "-s", "-C", HERE, "liboracle_native.so"], capture_output=True, text=True)
+    if proc.returncode != 0 or not os.path.exists(native):
+        return False
+    _lib_path = native
+    return True
 
 
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if _lib_path == LIB_PATH and not os.path.exists(LIB_PATH):
             subprocess.run(["make", "-s", "-C", HERE], check=True)
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(_lib_path)
         L.oracle_prime.restype = u64
         L.oracle_generator.restype = u64
         L.oracle_mulmod.restype = ctypes.c_double

@@ -118,7 +118,8 @@ def test_squares_match_oracle(oracle, n):
     assert (engine_mul(a, a) == expect).all()
 
 
-@pytest.mark.parametrize("n,m", [(1, 1), (7, 3), (1024, 1024), (5000, 17), (65536, 65536), (300000, 200000)])
+@pytest.mark.parametrize("n,m", [(1, 1), (7, 3), (700, 701), (1024, 1024), (1500, 548), (5000, 17), (65536, 65536),
+                                 (300000, 200000)])
 def test_all_ones_carry_chains(n, m):
     say(f"all ones {n}x{m}")
     a = np.full(n, MASK, dtype=np.uint64)

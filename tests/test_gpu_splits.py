@@ -1,9 +1,9 @@
 """Every admissible pass split gives the same product.
 
-The autotuner may pick any admissible row size for a transform length, and
-NTTAMD_ROW_LOG forces one, so each split (one strided pass and a row pass of
-2^5..2^12 points, or two strided passes and a row pass) must reproduce the
-default split's limbs bit for bit; the default split itself is checked against
+The autotuner may pick any admissible split for a transform length, and
+NTTAMD_PASSES / NTTAMD_ROW_LOG force one, so each split (one strided pass and
+a row pass of 2^5..2^12 points, or two strided passes and a row pass) must
+reproduce the default split's limbs bit for bit; the default split itself is checked against
 the oracle (up to 2^21-point transforms) or, for the largest shapes, against
 the product's residue modulo 2^61 - 1.
 """
@@ -38,23 +38,36 @@ def device():
     assert ntt.device_available(), "no gfx950 device visible"
     yield
     os.environ.pop("NTTAMD_ROW_LOG", None)
+    os.environ.pop("NTTAMD_PASSES", None)
 
 
-def row_candidates(K):
-    """Admissible row sizes, largest first (at most eight), as the engine's
-    autotuner enumerates them."""
+def strided_lines(logl):
+    """Lines per strided tile (device.h strided_lines)."""
+    return 64 if logl <= 7 else 32 if logl == 8 else 8 if logl <= 10 else 4
+
+
+def split_candidates(K):
+    """Admissible (passes, row size) splits as the engine's autotuner
+    enumerates them: two passes up to 2^23 points, three from 2^18 on, row
+    pass 2^5..2^12, strided passes of 5..11 levels whose line stride holds
+    their tile's lines."""
     out = []
-    for r in range(12, 4, -1):
-        rest = K - r
-        if K <= 23:
-            ok = 5 <= rest <= 11
-        else:
-            k1 = (rest + 1) // 2
-            ok = k1 <= 11 and rest - k1 >= 5
-        if ok:
-            out.append(r)
-        if len(out) == 8:
-            break
+    for passes in (2, 3):
+        if (passes == 2 and K > 23) or (passes == 3 and K < 18):
+            continue
+        for r in range(12, 4, -1):
+            if passes == 2:
+                logs = [K - r]
+            else:
+                k1 = (K - r + 1) // 2
+                logs = [k1, K - r - k1]
+            level, ok = 0, True
+            for lg in logs:
+                stride = K - level - lg
+                ok = ok and 5 <= lg <= 11 and stride >= 0 and (1 << stride) >= strided_lines(lg)
+                level += lg
+            if ok:
+                out.append((passes, r))
     return out
 
 
@@ -84,11 +97,14 @@ def test_every_split_matches_the_default(oracle, monkeypatch, la, lb, force_prim
         assert (ref == oracle.mul(a, b, threads=8)).all(), "default split differs from the oracle"
     else:
         assert mod_q(ref) == mod_q(a) * mod_q(b) % Q61, "default split fails its residue check"
-    cands = row_candidates(K)
+    cands = split_candidates(K)
     assert cands, f"no admissible split for 2^{K}"
-    for r in cands:
+    for passes, r in cands:
+        os.environ["NTTAMD_PASSES"] = str(passes)
         os.environ["NTTAMD_ROW_LOG"] = str(r)
-        say(f"  rows 2^{r}")
+        assert ntt.transform_split(K)[-1] == r and len(ntt.transform_split(K)) == passes
+        say(f"  {passes} passes, rows 2^{r}")
         got = product(a, b)
-        assert (got == ref).all(), f"rows 2^{r} differ from the default split"
+        assert (got == ref).all(), f"{passes} passes with rows 2^{r} differ from the default split"
     os.environ.pop("NTTAMD_ROW_LOG", None)
+    os.environ.pop("NTTAMD_PASSES", None)
