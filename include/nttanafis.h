@@ -244,9 +244,11 @@ bool ntt_batch_mul_host(ntt_batch *batch, const uint64_t *const *a, const uint64
 /* Request server (no counterpart in the reference). Serves a stream of
  * products, squares and convolutions from host threads with continuous
  * batching: worker slots with their own streams take work whenever they are
- * free, small products are grouped into one batched launch sequence, and
- * every other request is admitted in arrival order once its device working
- * set fits in the memory budget. Root tables stay resident. Each request
+ * free, in arrival order: the small products queued ahead of the oldest
+ * other request run as one batched launch sequence, and that request is
+ * admitted as soon as its device working set fits in the memory budget
+ * (small products behind it run only while it waits for memory, so it is
+ * never starved). Root tables stay resident. Each request
  * computes what MulWorkspace::mul / sqr (src/plan.rs:164-179) or
  * ConvPlan::convolve (src/convolve.rs:148-162) computes for it alone. */
 

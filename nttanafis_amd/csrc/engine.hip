@@ -514,9 +514,24 @@ bool ensure_tables(uint64_t need, hipStream_t s, const double** fwd, const doubl
 
 
 
+// Executors of the plan path (MulPlan::mul / sqr, ConvPlan::convolve take
+// &self and run concurrently in the reference, src/plan.rs:94-114): each
+// calling thread gets its own executor (stream and device arenas) the first
+// time it calls, from a process-wide pool, so calls from different threads
+// overlap on the device. Executors live until the process ends: work queued
+// from a thread that has since exited stays valid.
 Executor* plan_executor() {
-    static Executor* ex = new Executor();
-    return ex;
+    static std::mutex pool_lock;
+    static std::vector<Executor*>* pool = new std::vector<Executor*>();
+    thread_local Executor* mine = nullptr;
+    if (!mine) {
+        Executor* ex = new (std::nothrow) Executor();
+        if (!ex) return nullptr;
+        std::lock_guard<std::mutex> g(pool_lock);
+        pool->push_back(ex);
+        mine = ex;
+    }
+    return mine;
 }
 
 // ---------------------------------------------------------------------------
@@ -1930,13 +1945,17 @@ size_t ntt_mul_plan_sqr_scratch_len(const ntt_mul_plan*) { return 0; }
 bool ntt_mul_plan_mul(const ntt_mul_plan* plan, uint64_t* dst, size_t dst_len, const uint64_t* a,
                       size_t a_len, const uint64_t* b, size_t b_len, uint64_t*, size_t) {
     const bool square = a == b && a_len == b_len;
-    return mul_host(*plan_executor(), plan, dst, dst_len, a, a_len, b, b_len, square);
+    Executor* ex = plan_executor();
+    if (!ex) return false;
+    return mul_host(*ex, plan, dst, dst_len, a, a_len, b, b_len, square);
 }
 
 bool ntt_mul_plan_sqr(const ntt_mul_plan* plan, uint64_t* dst, size_t dst_len, const uint64_t* a,
                       size_t a_len, uint64_t*, size_t) {
     if (!plan || plan->len_a != plan->len_b) return false;
-    return mul_host(*plan_executor(), plan, dst, dst_len, a, a_len, a, a_len, true);
+    Executor* ex = plan_executor();
+    if (!ex) return false;
+    return mul_host(*ex, plan, dst, dst_len, a, a_len, a, a_len, true);
 }
 
 ntt_mul_workspace* ntt_mul_workspace_new(void) { return new (std::nothrow) ntt_mul_workspace(); }
@@ -2080,7 +2099,9 @@ size_t ntt_conv_plan_scratch_len(const ntt_conv_plan*) { return 0; }
 
 bool ntt_conv_plan_convolve(const ntt_conv_plan* plan, uint64_t* dst, size_t dst_len, const uint64_t* a,
                             size_t a_len, const uint64_t* b, size_t b_len, uint64_t*, size_t) {
-    return conv_host(*plan_executor(), plan, dst, dst_len, a, a_len, b, b_len);
+    Executor* ex = plan_executor();
+    if (!ex) return false;
+    return conv_host(*ex, plan, dst, dst_len, a, a_len, b, b_len);
 }
 
 ntt_conv_workspace* ntt_conv_workspace_new(void) { return new (std::nothrow) ntt_conv_workspace(); }
@@ -2625,6 +2646,7 @@ bool ntt_debug_copy(void* d_dst, const void* d_src, uint64_t bytes) {
 
 bool ntt_debug_tables(uint64_t len, double* fwd, double* inv) {
     Executor* ex = plan_executor();
+    if (!ex) return false;
     std::lock_guard<std::mutex> guard(ex->lock);
     if (!ex->init()) return false;
     const double *f = nullptr, *i = nullptr;
@@ -2658,6 +2680,7 @@ bool ntt_debug_residues(const uint64_t* a, size_t la, const uint64_t* b, size_t 
     if (!out) return true;
     if (out_len < g.primes * T) return false;
     Executor* ex = plan_executor();
+    if (!ex) return false;
     std::lock_guard<std::mutex> guard(ex->lock);
     if (!ex->init() || !ex->in_a.ensure(la * 8) || !ex->in_b.ensure(lb * 8)) return false;
     if (hipMemcpy(ex->in_a.ptr, a, la * 8, hipMemcpyHostToDevice) != hipSuccess) return false;
@@ -2677,6 +2700,7 @@ bool ntt_debug_crt(size_t la, size_t lb, const double* rows, uint64_t* out) {
     g.school = false;
     const uint64_t T = row_len(g);
     Executor* ex = plan_executor();
+    if (!ex) return false;
     std::lock_guard<std::mutex> guard(ex->lock);
     const uint64_t out_len = la + lb;
     if (!ex->init() || !ex->spec_b.ensure(g.primes * T * 8) || !ex->out.ensure(out_len * 8)) return false;

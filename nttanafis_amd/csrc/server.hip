@@ -110,6 +110,7 @@ struct Request {
     uint64_t offset = UINT64_MAX;
     double t_submit = 0, t_start = 0, t_done = 0;
     int status = 0;  // 0 queued or running, 1 done, -1 failed
+    bool waited = false;  // has waited for device memory (counted once)
 };
 
 // Pinned host staging of one slot (grow-only).
@@ -168,38 +169,55 @@ struct Server {
         for (auto& kv : requests) delete kv.second;
     }
 
-    // Picks work for a free slot (lock held): a group of small products, or
-    // the first large request in queue order when it fits in the budget.
+    // Picks work for a free slot (lock held), in arrival order: the small
+    // products queued ahead of the oldest large request go as one group;
+    // when that large request is the oldest, it is admitted as soon as its
+    // working set fits the budget. While it waits for memory, small products
+    // queued behind it may run (they use the slot's own arena, not the
+    // budget), but none is taken ahead of it once it fits, so a steady
+    // stream of small products cannot starve it.
     bool take(std::vector<Request*>* group) {
         group->clear();
         uint64_t spectra = 0;
-        for (auto it = queue.begin(); it != queue.end();) {
+        auto add_small = [&](std::deque<Request*>::iterator& it) -> bool {
             Request* q = *it;
-            if (!q->small || (!group->empty() && q->device != (*group)[0]->device)) {
+            if (!group->empty() && q->device != (*group)[0]->device) {
                 ++it;
-                continue;
+                return true;
             }
             const uint64_t sb = op_spectra_bytes(q->g) + (q->square ? 0 : op_spectra_bytes_b(q->g));
-            if (!group->empty() && (group->size() >= kGroupMax || spectra + sb > kGroupSpectra)) break;
+            if (!group->empty() && (group->size() >= kGroupMax || spectra + sb > kGroupSpectra)) return false;
             spectra += sb;
             group->push_back(q);
             it = queue.erase(it);
-        }
-        if (!group->empty()) return true;
-        for (auto it = queue.begin(); it != queue.end(); ++it) {
-            Request* q = *it;
-            if (q->small) continue;
-            const uint64_t off = arena.alloc(q->bytes);
-            if (off == UINT64_MAX) {
-                ++stats.waits_for_memory;
-                return false;  // the oldest large request waits for memory
-            }
-            q->offset = off;
-            group->push_back(q);
+            return true;
+        };
+        auto it = queue.begin();
+        while (it != queue.end() && (*it)->small)
+            if (!add_small(it)) return true;
+        if (!group->empty() || it == queue.end()) return !group->empty();
+        // The oldest request is large.
+        Request* big = *it;
+        const uint64_t off = arena.alloc(big->bytes);
+        if (off != UINT64_MAX) {
+            big->offset = off;
+            group->push_back(big);
             queue.erase(it);
             return true;
         }
-        return false;
+        if (!big->waited) {
+            big->waited = true;
+            ++stats.waits_for_memory;
+        }
+        // It waits for memory: small products behind it may run meanwhile.
+        for (++it; it != queue.end();) {
+            if (!(*it)->small) {
+                ++it;
+                continue;
+            }
+            if (!add_small(it)) break;
+        }
+        return !group->empty();
     }
 
     void finish(const std::vector<Request*>& group, bool ok) {

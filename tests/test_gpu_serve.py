@@ -61,7 +61,11 @@ def test_served_mix_equals_alone(pools, oracle, monkeypatch):
     # included) exceed the budget together; the schoolbook runs in the other
     # served tests.
     monkeypatch.setenv("NTTAMD_SCHOOL", "0")
-    reqs = served_stream(96, 0.0, lo=256, hi=1 << 18, seed=21)
+    # Two of the largest products first: the second cannot be admitted while
+    # the first holds its working set (each needs about 33 MiB of the 48).
+    big = [{"kind": MUL, "arrival": 0.0, "len_a": 1 << 18, "len_b": 1 << 18, "fill_a": "random", "fill_b": "random",
+            "off_a": 0, "off_b": 0} for _ in range(2)]
+    reqs = big + served_stream(96, 0.0, lo=256, hi=1 << 18, seed=21)
     say(f"served mix: {len(reqs)} requests")
     # A budget of 48 MiB holds only a few large working sets at once, so
     # admissions wait for memory while small products run in groups.
@@ -90,7 +94,7 @@ def test_served_mix_equals_alone(pools, oracle, monkeypatch):
         req = reqs[i]
         a, b = request_operands(req, pools)
         if req["kind"] == CONV:
-            want = oracle.convolve(a, b, 0xFFFF, 0xFFFF, False)
+            want = oracle.convolve(a, b, 0xFFFF, 0xFFFF, False, threads=8)
         elif req["kind"] == SQR:
             want = oracle.sqr(a)
         else:
@@ -221,4 +225,34 @@ def test_served_full_size_range(oracle):
             want = oracle.mul(a, b, threads=16)
         assert (outs[i] == want).all(), f"request {i} ({req['kind']} {req['len_a']}x{req['len_b']}) vs oracle"
         say(f"oracle ok: request {i} {req['kind']} {req['len_a']}x{req['len_b']}")
+    server.close()
+
+
+def test_large_request_not_starved_by_small_traffic(pools):
+    """Arrival order: with one slot, a large request queued behind a run of
+    small products and ahead of another run starts before every small
+    product submitted after it (small products are grouped only from those
+    queued ahead of it)."""
+    server = ntt.Server(budget_bytes=256 << 20, slots=1)
+    small = [{"kind": MUL, "arrival": 0.0, "len_a": 700 + k, "len_b": 900, "fill_a": "random", "fill_b": "random",
+              "off_a": k, "off_b": 2 * k} for k in range(120)]
+    large = {"kind": MUL, "arrival": 0.0, "len_a": 1 << 17, "len_b": 1 << 17, "fill_a": "random",
+             "fill_b": "random", "off_a": 0, "off_b": 0}
+    reqs = small[:60] + [large] + small[60:]
+    outs, tickets = [], []
+    for req in reqs:
+        a, b = request_operands(req, pools)
+        dst = np.zeros(output_len(req), dtype=np.uint64)
+        t = submit(server, req, a, b, dst)
+        assert t >= 0, ntt.last_error()
+        outs.append(dst)
+        tickets.append(t)
+    assert server.drain()
+    t_large = server.times(tickets[60])[1]
+    later = [server.times(t)[1] for t in tickets[61:]]
+    say(f"large request started at {t_large:.6f}; first later small start {min(later):.6f}")
+    assert all(t_large <= t for t in later), "a small product submitted later started before the large request"
+    for i in (0, 60, 61, len(reqs) - 1):
+        a, b = request_operands(reqs[i], pools)
+        assert (outs[i] == alone(reqs[i], a, b)).all(), f"request {i}"
     server.close()
