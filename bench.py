@@ -358,7 +358,19 @@ def bench_batch(steps: int, warmup: int, plain_steps: int) -> dict:
         "first_run_s": round(tune_s, 3),
         "kernels_ms": {k: round(v[0], 3) for k, v in prof.items()},
         "squares": int(sum(sq)),
+        "tuned_buckets": tuned_forms("batch"),
     }
+
+
+def tuned_forms(use: str) -> dict:
+    """Split and kernel forms the autotuner chose, per tuned transform length
+    ("2^K": {"passes": [...], "forms": [...]})."""
+    out = {}
+    for k in range(5, 31):
+        f = ntt.transform_form(k, use)
+        if f["tuned"]:
+            out[f"2^{k}"] = {"passes": f["passes"], "forms": f["forms"]}
+    return out
 
 
 def bench_host_path(a, b, steps: int) -> dict:
@@ -619,7 +631,8 @@ def gpu_workload(name: str, steps: int, warmup: int, replay_reps: int = 20, runs
     pass_bytes = sum(kernels[k]["bytes_per_launch"] * kernels[k]["launches_per_step"] for k in passes)
     pass_us = sum(kernels[k]["event_us"] * kernels[k]["launches_per_step"] for k in passes)
     pass_us_rep = sum(kernels[k].get("replay_us", 0.0) * kernels[k]["launches_per_step"] for k in passes)
-    split = ntt.transform_split(log_len, "square" if kind == "sqr" else "product") if tail == 0 else None
+    form = ntt.transform_form(log_len, "square" if kind == "sqr" else "product") if tail == 0 else None
+    split = form["passes"] if form else None
     units = la if kind != "sqr" else la
     return {
         "workload": name,
@@ -631,6 +644,7 @@ def gpu_workload(name: str, steps: int, warmup: int, replay_reps: int = 20, runs
         "log_len": log_len,
         "row_tail": tail,
         "transform_split": split,
+        "transform_forms": form["forms"] if form else None,
         "steps": steps,
         "ms_per_step": round(region_ms[0], 4),
         "ms_per_step_runs": [round(x, 4) for x in region_ms],
@@ -697,7 +711,9 @@ def bench_config1(reps: int = 200) -> dict:
         out[name] = {"device_us_per_call": round(queued * 1e6, 2), "device_us_waited": round(waited * 1e6, 2),
                      "host_us_per_call": round(host_s * 1e6, 2),
                      "schoolbook": bool(ntt.mul_schoolbook(1024, 1024)),
-                     "geometry": list(ntt.mul_geometry(1024, 1024))}
+                     "geometry": list(ntt.mul_geometry(1024, 1024)),
+                     "transform_form": ntt.transform_form(ntt.mul_geometry(1024, 1024)[2],
+                                                          "square" if square else "product")}
     out["reference_us"] = {"value": 23.1, "source": "reference README.md:25, mul 1,024 x 1,024, one Zen 5 core"}
     out["reps"] = reps
     return out

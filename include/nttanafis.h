@@ -372,12 +372,18 @@ bool ntt_profile_replay(const char *kind, int reps, double *avg_ms);
 /* Pass split the engine runs for a transform of 2^log_len points as a
  * single operation's product (use 0), its square (use 1) or a batch bucket
  * (use 2): returns the number of passes and writes their level counts
- * (strided passes first, the row pass last) to pass_logs[0..n); the split is
- * the one the autotuner chose for this length and use, the one
- * NTTAMD_ROW_LOG / NTTAMD_PASSES force, or the default. 0 for lengths out of
- * range. No counterpart in the reference (the reference's split lives in its
- * Geometry). */
-int ntt_transform_split(uint32_t log_len, int use, uint32_t *pass_logs);
+ * (strided passes first, the row pass last) to pass_logs[0..n) and, when
+ * `flags` is not NULL, the kernel forms chosen with it (NTT_FORM_*); the split
+ * is the one the autotuner chose for this length and use, the one
+ * NTTAMD_ROW_LOG / NTTAMD_PASSES (and NTTAMD_ALT_TILES, NTTAMD_INV16,
+ * NTTAMD_PER_PRIME_PACK set to 0 or 1) force, or the default. 0 for lengths
+ * out of range. No counterpart in the reference (the reference's split lives
+ * in its Geometry). */
+#define NTT_FORM_ALT_TILES 1u        /* strided passes use the alternative tiles */
+#define NTT_FORM_INV16 2u            /* product rows run the inverse in radix 16 (else 8) */
+#define NTT_FORM_PER_PRIME_PACK 4u   /* first pass packs per (tile, prime), not shared across primes */
+#define NTT_FORM_TUNED 256u          /* the autotuner has timed this length and use */
+int ntt_transform_split(uint32_t log_len, int use, uint32_t *pass_logs, uint32_t *flags);
 
 /* ---------------------------------------------------------------------------
  * Prime fields and two-prime CRT: the public scalar API of the reference

@@ -24,6 +24,7 @@ from .api import (  # noqa: F401
     row_len,
     row_shape,
     synchronize,
+    transform_form,
     transform_split,
 )
 from ._ffi import LIB_PATH, load  # noqa: F401
@@ -51,6 +52,7 @@ __all__ = [
     "row_len",
     "row_shape",
     "synchronize",
+    "transform_form",
     "transform_split",
     "load",
     "LIB_PATH",

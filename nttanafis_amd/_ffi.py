@@ -208,7 +208,7 @@ SIGNATURES = {
     ),
     "ntt_profile_reset": (None, []),
     "ntt_profile_replay": (ctypes.c_bool, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
-    "ntt_transform_split": (ctypes.c_int, [u32, ctypes.c_int, ctypes.POINTER(u32)]),
+    "ntt_transform_split": (ctypes.c_int, [u32, ctypes.c_int, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
     "ntt_row_shape": (ctypes.c_bool, [u64, u64, P64, P64, P64]),
     "ntt_mul_split": (ctypes.c_bool, [size_t, size_t, ctypes.POINTER(u32), P64]),
     "ntt_mul_schoolbook": (ctypes.c_int, [size_t, size_t]),

@@ -585,10 +585,26 @@ class Profile:
 SPLIT_USES = {"product": 0, "square": 1, "batch": 2}
 
 
+FORM_FLAGS = {"alt_tiles": 1, "inv16": 2, "per_prime_pack": 4}
+FORM_TUNED = 256
+
+
 def transform_split(log_len: int, use: str = "product"):
     """Level counts of the passes the engine runs for a transform of
     2^log_len points (strided passes, then the row pass) for a single
     operation's product or square, or a batch bucket, as tuned."""
+    return transform_form(log_len, use)["passes"]
+
+
+def transform_form(log_len: int, use: str = "product") -> dict:
+    """The split of transform_split with the kernel forms chosen beside it:
+    {"passes": [...], "forms": [names of FORM_FLAGS set], "tuned": bool}
+    ("tuned": the autotuner has timed this length and use)."""
     logs = (ctypes.c_uint32 * 3)()
-    n = _lib().ntt_transform_split(log_len, SPLIT_USES[use], logs)
-    return [int(logs[i]) for i in range(n)]
+    flags = ctypes.c_uint32(0)
+    n = _lib().ntt_transform_split(log_len, SPLIT_USES[use], logs, ctypes.byref(flags))
+    return {
+        "passes": [int(logs[i]) for i in range(n)],
+        "forms": [k for k, bit in FORM_FLAGS.items() if flags.value & bit],
+        "tuned": bool(flags.value & FORM_TUNED),
+    }

@@ -228,6 +228,13 @@ struct RecombineArgs {
 __host__ __device__ constexpr int strided_lines(int logl) {
     return logl <= 7 ? 64 : logl == 8 ? 32 : (logl <= 10 ? 8 : 4);
 }
+// The alternative tiles the autotuner may choose instead: half the lines up
+// to 2^8 points and for 2^10-2^11 points (4096-point tiles), twice the lines
+// for 2^9 points (8192 points).
+__host__ __device__ constexpr int alt_strided_lines(int logl) {
+    return logl <= 7 ? 32 : logl == 8 ? 16 : logl == 9 ? 16 : logl == 10 ? 4 : 2;
+}
+__host__ __device__ constexpr int tile_lines(int logl, bool alt) { return alt ? alt_strided_lines(logl) : strided_lines(logl); }
 
 constexpr uint32_t kInf = 0xFFFFFFFFu;
 // Carries across the blocks of an output: up to kCarryLookBackBlocks blocks

@@ -167,18 +167,26 @@ bool convolution_on(hipStream_t s, const Geometry& g, const ntt_conv_plan* p, co
 struct Split {
     int n;
     int logs[3];  // strided passes, then the row pass (last)
+    // Kernel forms the autotuner may choose beside the levels: the
+    // alternative strided tiles (alt_strided_lines), radix-16 inverse rows
+    // in the product row kernel, and packing per (tile, prime) instead of
+    // packing shared across the primes of a tile.
+    uint8_t alt_tiles, inv16, per_prime_pack, pad;
 };
 
 // Which tuned split a transform takes: a single operation's product (two
 // operands), its square (one operand), or a batch bucket.
 enum SplitUse { kUseProduct = 0, kUseSquare = 1, kUseBatch = 2 };
+constexpr uint32_t kFormAltTiles = NTT_FORM_ALT_TILES, kFormInv16 = NTT_FORM_INV16,
+                   kFormPerPrimePack = NTT_FORM_PER_PRIME_PACK, kFormTuned = NTT_FORM_TUNED;
 Split split_of(int K, int use);
 
 struct BatchLaunch {
-    enum Kind { kFwdStrided, kFused, kInvStrided, kCrt, kSchool } kind;
+    enum Kind { kFwdStrided, kPackPrimes, kFused, kInvStrided, kCrt, kSchool } kind;
     int logl;
     bool flag;  // pack (forward strided), canonical output (inverse strided), square (fused)
     bool pack;  // fused single pass: pack the operands
+    bool alt;   // strided passes: alternative tiles; fused: radix-16 inverse rows
     int units;
     const char* name;
     PassArgs pass;
@@ -188,6 +196,8 @@ struct BatchLaunch {
 struct BatchBucket {
     int K;
     uint32_t fwd, n_fwd;  // forward items: every first operand, then every second one
+    uint32_t pk, n_pk;    // the same with prime 0 only: shared packing items
+    bool shared_ok;       // every operand's coefficients fit the shared packing pass
     uint32_t ns, n_ns;    // product items of products with two operands
     uint32_t sq, n_sq;    // product items of squares (follow ns: ns..sq is every product)
 };

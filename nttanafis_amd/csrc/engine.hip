@@ -20,10 +20,12 @@ namespace nttamd {
 
 typedef unsigned __int128 u128;
 
-bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, bool pack, hipStream_t s);
+bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, bool pack, bool alt, hipStream_t s);
 bool launch_pack_primes(int logl, const PassArgs& a, int primes, int operands, hipStream_t s);
-bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, hipStream_t s);
-bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, hipStream_t s);
+bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, bool alt, hipStream_t s);
+bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, bool inv16, hipStream_t s);
+bool pack_primes_batch_fits(int logl, uint64_t line_stride);
+bool launch_pack_primes_batch(int logl, const PassArgs& a, int items, hipStream_t s);
 bool launch_crt_product(const CrtArgs& a, hipStream_t s);
 bool launch_crt_disjoint(const CrtArgs& a, hipStream_t s);
 bool launch_school(const CrtArgs& a, hipStream_t s);
@@ -543,7 +545,8 @@ static std::mutex g_tune_lock;
 // Splits of a transform of 2^K points, K > 12: a row pass of 2^rows points
 // (5 <= rows <= 12) after one strided pass (2^13 .. 2^23 points) or two
 // (from 2^18 points on), each of 5 to 11 levels. A strided pass needs a line
-// stride of at least its tile's lines (strided_lines).
+// stride of at least its tile's lines (tile_lines: the default or the
+// alternative tiles).
 static Split split_rows(int K, int rows, int passes) {
     Split s{};
     if (passes == 2) {
@@ -565,7 +568,8 @@ static bool split_ok(int K, const Split& s) {
     for (int i = 0; i + 1 < s.n; ++i) {
         if (s.logs[i] < kMinLog || s.logs[i] > 11) return false;
         const int stride_log = K - level - s.logs[i];
-        if (stride_log < 0 || ((uint64_t)1 << stride_log) < (uint64_t)strided_lines(s.logs[i])) return false;
+        if (stride_log < 0 || ((uint64_t)1 << stride_log) < (uint64_t)tile_lines(s.logs[i], s.alt_tiles != 0))
+            return false;
         level += s.logs[i];
     }
     const int rows = s.logs[s.n - 1];
@@ -573,10 +577,52 @@ static bool split_ok(int K, const Split& s) {
 }
 
 static bool same_split(const Split& a, const Split& b) {
-    if (a.n != b.n) return false;
+    if (a.n != b.n || a.alt_tiles != b.alt_tiles || a.inv16 != b.inv16 || a.per_prime_pack != b.per_prime_pack)
+        return false;
     for (int i = 0; i < a.n; ++i)
         if (a.logs[i] != b.logs[i]) return false;
     return true;
+}
+
+// Kernel-form flags forced from the environment (tests, A/B runs):
+// NTTAMD_ALT_TILES, NTTAMD_INV16 and NTTAMD_PER_PRIME_PACK set to 0 or 1.
+// -1: not forced.
+static int env_flag(const char* name) {
+    const char* e = getenv(name);
+    return e && *e ? (atoi(e) != 0) : -1;
+}
+static bool flags_forced() {
+    return env_flag("NTTAMD_ALT_TILES") >= 0 || env_flag("NTTAMD_INV16") >= 0 || env_flag("NTTAMD_PER_PRIME_PACK") >= 0;
+}
+static Split with_env_flags(int K, Split s) {
+    const int alt = env_flag("NTTAMD_ALT_TILES"), inv = env_flag("NTTAMD_INV16"),
+              ppp = env_flag("NTTAMD_PER_PRIME_PACK");
+    if (alt >= 0 && s.n > 1) {
+        Split t = s;
+        t.alt_tiles = (uint8_t)alt;
+        if (split_ok(K, t)) s = t;
+    }
+    if (inv >= 0) s.inv16 = (uint8_t)inv;
+    if (ppp >= 0 && s.n > 1) s.per_prime_pack = (uint8_t)ppp;
+    return s;
+}
+
+// The flag variants the autotuner tries after choosing the levels, in order
+// (each kept when it beats the best so far): alternative strided tiles,
+// radix-16 inverse rows (products; a square's rows already run radix 16),
+// packing per prime instead of the shared first packing pass.
+static int flag_variants(int K, int use, const Split& best, Split out[3]) {
+    int n = 0;
+    Split t = best;
+    t.alt_tiles = 1;
+    if (best.n > 1 && !best.alt_tiles && split_ok(K, t)) out[n++] = t;
+    t = best;
+    t.inv16 = 1;
+    if (use != kUseSquare && !best.inv16) out[n++] = t;
+    t = best;
+    t.per_prime_pack = 1;
+    if (best.n > 1 && !best.per_prime_pack) out[n++] = t;
+    return n;
 }
 
 // Admissible splits, in a fixed array (split_of runs on the allocation-free
@@ -607,36 +653,40 @@ static SplitCandidates split_candidates(int K) {
 // Tuned splits per use and length (n == 0: not tuned yet).
 static Split g_tuned[3][kMaxLog + 1];
 
-// Split of a transform of 2^K points: a single pass up to 2^12; otherwise
-// the autotuner's candidate while it times one, the candidate that
+// Split of a transform of 2^K points: the autotuner's candidate while it
+// times one; otherwise a single pass up to 2^12, the candidate that
 // NTTAMD_ROW_LOG (and NTTAMD_PASSES, 2 or 3) name when set and admissible,
 // the tuned split of this use once the autotuner timed this length, else
 // the first candidate (two passes with the largest row below 2^24 points,
-// three from there on).
+// three from there on); then the flags the environment forces.
 static thread_local Split t_force{};  // set while the autotuner times a candidate
 
-Split split_of(int K, int use) {
+static Split split_default(int K, int use) {
+    const SplitCandidates cand = split_candidates(K);
+    const char* er = getenv("NTTAMD_ROW_LOG");
+    const char* ep = getenv("NTTAMD_PASSES");
+    if (K > kSinglePassMaxLog && (er || ep)) {
+        const int rows = er ? atoi(er) : 0, passes = ep ? atoi(ep) : 0;
+        for (const Split& s : cand)
+            if ((!rows || s.logs[s.n - 1] == rows) && (!passes || s.n == passes)) return s;
+    }
+    if (K >= 0 && K <= kMaxLog && use >= 0 && use < 3) {
+        std::lock_guard<std::mutex> g(g_tune_lock);
+        if (g_tuned[use][K].n) return g_tuned[use][K];
+    }
     if (K <= kSinglePassMaxLog) {
         Split s{};
         s.n = 1;
         s.logs[0] = K;
         return s;
     }
-    if (t_force.n) return t_force;
-    const SplitCandidates cand = split_candidates(K);
-    const char* er = getenv("NTTAMD_ROW_LOG");
-    const char* ep = getenv("NTTAMD_PASSES");
-    if (er || ep) {
-        const int rows = er ? atoi(er) : 0, passes = ep ? atoi(ep) : 0;
-        for (const Split& s : cand)
-            if ((!rows || s.logs[s.n - 1] == rows) && (!passes || s.n == passes)) return s;
-    }
-    if (K <= kMaxLog && use >= 0 && use < 3) {
-        std::lock_guard<std::mutex> g(g_tune_lock);
-        if (g_tuned[use][K].n) return g_tuned[use][K];
-    }
     if (!cand.empty()) return cand[0];
     return split_rows(K, kSinglePassMaxLog, K <= 23 ? 2 : 3);
+}
+
+Split split_of(int K, int use) {
+    if (t_force.n) return t_force;
+    return with_env_flags(K, split_default(K, use));
 }
 
 // Pass arguments of a transform of 2^log_n points over rows of row_len(g)
@@ -734,8 +784,9 @@ static bool forward_head(const Split& sp, const PassArgs& a0, const Source& sa, 
         a.level_start = level;
         a.line_stride = (uint64_t)1 << (a.log_t - level - sp.logs[i]);
         Timed t(i == 0 ? "fwd_pack_strided" : "fwd_strided", s);
-        const bool shared = i == 0 && pack_shared() && launch_pack_primes(sp.logs[i], a, primes, operands, s);
-        if (!shared && !launch_fwd_strided(sp.logs[i], a, primes, operands, i == 0, s)) return false;
+        const bool shared = i == 0 && pack_shared() && !sp.per_prime_pack &&
+                            launch_pack_primes(sp.logs[i], a, primes, operands, s);
+        if (!shared && !launch_fwd_strided(sp.logs[i], a, primes, operands, i == 0, sp.alt_tiles, s)) return false;
         level += sp.logs[i];
     }
     return true;
@@ -749,7 +800,7 @@ static bool inverse_tail(const Split& sp, const PassArgs& a0, int primes, hipStr
         a.level_start = level;
         a.line_stride = (uint64_t)1 << (a.log_t - level - sp.logs[i]);
         Timed t(i == 0 ? "inv_strided_canon" : "inv_strided", s);
-        if (!launch_inv_strided(sp.logs[i], a, primes, i == 0, s)) return false;
+        if (!launch_inv_strided(sp.logs[i], a, primes, i == 0, sp.alt_tiles, s)) return false;
     }
     return true;
 }
@@ -795,7 +846,7 @@ static bool part_residues(hipStream_t s, const Geometry& g, const RowPart& part,
         a.src_b = sb;
         a.level_start = 0;
         Timed t("fused_single", s);
-        return launch_fused(part.log_n, a, np, true, square, s);
+        return launch_fused(part.log_n, a, np, true, square, sp.inv16, s);
     }
     const int row_log = sp.logs[sp.n - 1];
     const int row_level = part.log_n - row_log;
@@ -812,7 +863,7 @@ static bool part_residues(hipStream_t s, const Geometry& g, const RowPart& part,
     pf.line_stride = 1;
     {
         Timed t("fused_rows", s);
-        if (!launch_fused(row_log, pf, np, false, square, s)) return false;
+        if (!launch_fused(row_log, pf, np, false, square, sp.inv16, s)) return false;
     }
     return inverse_tail(sp, pf, np, s);
 }
@@ -864,7 +915,7 @@ static bool split_residues(hipStream_t s, const Geometry& g, const Source& sa, c
     *rows_out = buf.spec_a;
     if (sp.n == 1) {
         Timed t("fused_single", s);
-        return launch_fused(g.log_t, a, np, true, false, s);
+        return launch_fused(g.log_t, a, np, true, false, sp.inv16, s);
     }
     if (!forward_head(sp, a, lng, sht, (int)g.blocks + 1, np, s)) return false;
     const int row_log = sp.logs[sp.n - 1];
@@ -872,7 +923,7 @@ static bool split_residues(hipStream_t s, const Geometry& g, const Source& sa, c
     a.line_stride = 1;
     {
         Timed t("fused_rows", s);
-        if (!launch_fused(row_log, a, np, false, false, s)) return false;
+        if (!launch_fused(row_log, a, np, false, false, sp.inv16, s)) return false;
     }
     return inverse_tail(sp, a, np, s);
 }
@@ -1037,44 +1088,69 @@ static bool device_product(Executor& ex, const Geometry& g, const uint64_t* d_a,
 // Plan-time autotuning of a single operation's split: the first prepare of a
 // transform length times every admissible split on the executor's own
 // buffers (operands of the plan's shape, contents irrelevant to the timing),
-// for the product and, when the operands are as long, for the square, and
-// records the fastest for every later call of that length and use.
+// for the product and, when the operands are as long, for the square, then
+// tries the kernel-form flags on the fastest (flag_variants), and records the
+// result for every later call of that length and use. A single-pass length
+// (up to 2^12 points) has one split and tries the flags only.
 static void tune_split(Executor& ex, int K, int use, const std::function<bool()>& run) {
-    if (K < 0 || K > kMaxLog || use < 0 || use > 2) return;
-    if (getenv("NTTAMD_ROW_LOG") || getenv("NTTAMD_PASSES")) return;  // a forced split needs no trials
+    if (K < kMinLog || K > kMaxLog || use < 0 || use > 2) return;
+    if (getenv("NTTAMD_ROW_LOG") || getenv("NTTAMD_PASSES") || flags_forced()) return;  // forced: no trials
     {
         std::lock_guard<std::mutex> g(g_tune_lock);
         if (g_tuned[use][K].n) return;
     }
-    const SplitCandidates cand = split_candidates(K);
-    if (cand.empty()) return;
+    SplitCandidates cand = split_candidates(K);
+    if (cand.empty()) {
+        cand.n = 1;
+        cand.s[0] = Split{};
+        cand.s[0].n = 1;
+        cand.s[0].logs[0] = K;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) hipEventDestroy(e0);
+        return;
+    }
+    auto time_of = [&](const Split& sp) {
+        t_force = sp;
+        bool ok = run();  // warm
+        hipEventRecord(e0, ex.stream);
+        for (int r = 0; r < 3 && ok; ++r) ok = run();
+        hipEventRecord(e1, ex.stream);
+        float ms = 1e30f;
+        if (ok && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
+        return ok ? ms : 1e30f;
+    };
     Split best = cand[0];
-    if (cand.size() > 1) {
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-            if (e0) hipEventDestroy(e0);
-            return;
-        }
-        float best_ms = 1e30f;
+    float best_ms = 1e30f;
+    if (cand.size() > 1)
         for (const Split& sp : cand) {
-            t_force = sp;
-            bool ok = run();  // warm
-            hipEventRecord(e0, ex.stream);
-            for (int r = 0; r < 3 && ok; ++r) ok = run();
-            hipEventRecord(e1, ex.stream);
-            float ms = 1e30f;
-            if (ok && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
-            if (ok && ms < best_ms) {
+            const float ms = time_of(sp);
+            if (ms < best_ms) {
                 best_ms = ms;
                 best = sp;
             }
         }
-        t_force = Split{};
-        hipStreamSynchronize(ex.stream);
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-        clear_error();
+    Split var[3];
+    const int nv = flag_variants(K, use, best, var);
+    if (nv && cand.size() == 1) best_ms = time_of(best);
+    for (int i = 0; i < nv; ++i) {
+        Split t = best;  // the flag of variant i on top of the flags kept so far
+        t.alt_tiles |= var[i].alt_tiles;
+        t.inv16 |= var[i].inv16;
+        t.per_prime_pack |= var[i].per_prime_pack;
+        if (!split_ok(K, t) && t.n > 1) continue;
+        const float ms = time_of(t);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
     }
+    t_force = Split{};
+    hipStreamSynchronize(ex.stream);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    clear_error();
     std::lock_guard<std::mutex> g(g_tune_lock);
     if (!g_tuned[use][K].n) g_tuned[use][K] = best;
 }
@@ -1246,9 +1322,10 @@ namespace nttamd {
 static bool issue_launch(const BatchLaunch& L, hipStream_t s) {
     Timed t(L.name, s);
     switch (L.kind) {
-        case BatchLaunch::kFwdStrided: return launch_fwd_strided(L.logl, L.pass, L.units, 1, L.flag, s);
-        case BatchLaunch::kFused: return launch_fused(L.logl, L.pass, L.units, L.pack, L.flag, s);
-        case BatchLaunch::kInvStrided: return launch_inv_strided(L.logl, L.pass, L.units, L.flag, s);
+        case BatchLaunch::kFwdStrided: return launch_fwd_strided(L.logl, L.pass, L.units, 1, L.flag, L.alt, s);
+        case BatchLaunch::kPackPrimes: return launch_pack_primes_batch(L.logl, L.pass, L.units, s);
+        case BatchLaunch::kFused: return launch_fused(L.logl, L.pass, L.units, L.pack, L.flag, L.alt, s);
+        case BatchLaunch::kInvStrided: return launch_inv_strided(L.logl, L.pass, L.units, L.flag, L.alt, s);
         case BatchLaunch::kCrt: return launch_crt_product(L.crt, s);
         case BatchLaunch::kSchool: return launch_school(L.crt, s);
     }
@@ -1276,6 +1353,7 @@ static void bucket_program(std::vector<BatchLaunch>& prog, const PassArgs& base,
         L.logl = logl;
         L.flag = flag;
         L.pack = pack;
+        L.alt = kind == BatchLaunch::kFused ? sp.inv16 != 0 : sp.alt_tiles != 0;
         L.units = (int)units;
         L.name = name;
         L.pass = base;
@@ -1294,8 +1372,13 @@ static void bucket_program(std::vector<BatchLaunch>& prog, const PassArgs& base,
     const int row_level = K - row_log;
     int level = 0;
     for (int i = 0; i + 1 < sp.n; ++i) {
-        pass(BatchLaunch::kFwdStrided, sp.logs[i], i == 0, false, level, r.fwd, r.n_fwd, kOperandForward,
-             i == 0 ? "batch_fwd_pack_strided" : "batch_fwd_strided");
+        if (i == 0 && !sp.per_prime_pack && r.shared_ok && pack_shared() &&
+            pack_primes_batch_fits(sp.logs[0], (uint64_t)1 << (K - sp.logs[0])))
+            pass(BatchLaunch::kPackPrimes, sp.logs[0], true, false, 0, r.pk, r.n_pk, kOperandForward,
+                 "batch_pack_primes");
+        else
+            pass(BatchLaunch::kFwdStrided, sp.logs[i], i == 0, false, level, r.fwd, r.n_fwd, kOperandForward,
+                 i == 0 ? "batch_fwd_pack_strided" : "batch_fwd_strided");
         level += sp.logs[i];
     }
     pass(BatchLaunch::kFused, row_log, false, false, row_level, r.ns, r.n_ns, kOperandProduct, "batch_fused_rows");
@@ -1456,6 +1539,16 @@ bool batch_plan(Batch& B, size_t count, const size_t* la, const size_t* lb, cons
         if ((uint32_t)items.size() == r.fwd) continue;
         add(K, 1);
         r.n_fwd = (uint32_t)items.size() - r.fwd;
+        // Shared packing items: the forward items of prime 0.
+        r.pk = (uint32_t)items.size();
+        for (uint32_t k = r.fwd; k < r.fwd + r.n_fwd; ++k)
+            if ((items[k].sub & kItemPrimeMask) == 0) items.push_back(items[k]);
+        r.n_pk = (uint32_t)items.size() - r.pk;
+        r.shared_ok = true;
+        for (size_t i = 0; i < count; ++i)
+            if (B.geo[i].log_t == K && !B.geo[i].school &&
+                (pack_kind(B.host[i].src_a) == kPack4 || pack_kind(B.host[i].src_b) == kPack4))
+                r.shared_ok = false;
         r.ns = (uint32_t)items.size();
         add(K, 2);
         r.n_ns = (uint32_t)items.size() - r.ns;
@@ -1530,38 +1623,61 @@ static float time_launches(const std::vector<BatchLaunch>& prog, hipStream_t s, 
     return ms;
 }
 
-// Times the transform work of each untuned bucket under each candidate split
-// and keeps the fastest. The operands must be bound: the timed passes read
-// them.
+// Times the transform work of each untuned bucket under each candidate split,
+// then the kernel-form flags on the fastest (flag_variants), and keeps the
+// fastest. The operands must be bound: the timed passes read them.
 static bool batch_autotune(Batch& B) {
     const BatchItem* I = B.items.as<BatchItem>();
+    const bool forced = getenv("NTTAMD_ROW_LOG") || getenv("NTTAMD_PASSES") || flags_forced();
     bool changed = false;
     for (const BatchBucket& r : B.buckets) {
+        if (forced) break;
         {
             std::lock_guard<std::mutex> g(g_tune_lock);
             if (g_tuned[kUseBatch][r.K].n) continue;
         }
-        const SplitCandidates cand = split_candidates(r.K);
-        if (cand.empty()) continue;
+        SplitCandidates cand = split_candidates(r.K);
+        if (cand.empty()) {
+            cand.n = 1;
+            cand.s[0] = Split{};
+            cand.s[0].n = 1;
+            cand.s[0].logs[0] = r.K;
+        }
+        // The bucket's whole transform work under a split (it recomputes
+        // everything from the operands, so repeats are harmless).
+        auto time_of = [&](const Split& sp) {
+            std::vector<BatchLaunch> prog;
+            bucket_program(prog, bucket_base(B, r.K), r.K, sp, I, r);
+            return time_launches(prog, B.ex.stream, 3);
+        };
         Split best = cand[0];
-        if (cand.size() > 1) {
-            float best_ms = 1e30f;
+        float best_ms = 1e30f;
+        if (cand.size() > 1)
             for (const Split& sp : cand) {
-                // The bucket's whole transform work under this split (it
-                // recomputes everything from the operands, so repeats are
-                // harmless).
-                std::vector<BatchLaunch> prog;
-                bucket_program(prog, bucket_base(B, r.K), r.K, sp, I, r);
-                const float ms = time_launches(prog, B.ex.stream, 3);
+                const float ms = time_of(sp);
                 if (ms < best_ms) {
                     best_ms = ms;
                     best = sp;
                 }
             }
-            if (hipStreamSynchronize(B.ex.stream) != hipSuccess) {
-                set_error("batch: autotuning run failed");
-                return false;
+        Split var[3];
+        const int nv = flag_variants(r.K, kUseBatch, best, var);
+        if (nv && cand.size() == 1) best_ms = time_of(best);
+        for (int i = 0; i < nv; ++i) {
+            Split t = best;
+            t.alt_tiles |= var[i].alt_tiles;
+            t.inv16 |= var[i].inv16;
+            t.per_prime_pack |= var[i].per_prime_pack;
+            if (!split_ok(r.K, t) && t.n > 1) continue;
+            const float ms = time_of(t);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
             }
+        }
+        if (hipStreamSynchronize(B.ex.stream) != hipSuccess) {
+            set_error("batch: autotuning run failed");
+            return false;
         }
         std::lock_guard<std::mutex> g(g_tune_lock);
         if (!g_tuned[kUseBatch][r.K].n) {
@@ -1982,7 +2098,7 @@ void ntt_mul_workspace_prepare(ntt_mul_workspace* ws, const ntt_mul_plan* plan) 
     // Time the admissible splits of this length on operands of this shape.
     const uint64_t la = plan->len_a, lb = plan->len_b;
     Executor& ex = ws->ex;
-    if (g.log_t > kSinglePassMaxLog && ex.in_a.ensure(la * 8) && ex.in_b.ensure(lb * 8) &&
+    if (ex.in_a.ensure(la * 8) && ex.in_b.ensure(lb * 8) &&
         ex.out.ensure((la + lb) * 8) && hipMemsetAsync(ex.in_a.ptr, 0x5a, la * 8, ex.stream) == hipSuccess &&
         hipMemsetAsync(ex.in_b.ptr, 0xa5, lb * 8, ex.stream) == hipSuccess) {
         tune_split(ex, g.log_t, kUseProduct, [&] {
@@ -2126,8 +2242,7 @@ void ntt_conv_workspace_prepare(ntt_conv_workspace* ws, const ntt_conv_plan* pla
     // shape (every coefficient within any bound).
     Executor& ex = ws->ex;
     const uint64_t la = plan->len_a, lb = plan->len_b;
-    if (g.log_t > kSinglePassMaxLog && ex.in_a.ensure(la * 8) &&
-        ex.in_b.ensure(lb * 8) && ex.out.ensure(plan->output_len * 8) &&
+    if (ex.in_a.ensure(la * 8) && ex.in_b.ensure(lb * 8) && ex.out.ensure(plan->output_len * 8) &&
         hipMemsetAsync(ex.in_a.ptr, 0, la * 8, ex.stream) == hipSuccess &&
         hipMemsetAsync(ex.in_b.ptr, 0, lb * 8, ex.stream) == hipSuccess) {
         OpBuffers buf;
@@ -2436,10 +2551,19 @@ bool ntt_profile_replay(const char* kind, int reps, double* avg_ms) {
     return ok;
 }
 
-int ntt_transform_split(uint32_t log_len, int use, uint32_t* pass_logs) {
+int ntt_transform_split(uint32_t log_len, int use, uint32_t* pass_logs, uint32_t* flags) {
     if (log_len < (uint32_t)kMinLog || log_len > (uint32_t)kMaxLog || use < 0 || use > 2) return 0;
     const Split sp = split_of((int)log_len, use);
     for (int i = 0; i < sp.n && pass_logs; ++i) pass_logs[i] = (uint32_t)sp.logs[i];
+    if (flags) {
+        bool tuned;
+        {
+            std::lock_guard<std::mutex> g(g_tune_lock);
+            tuned = g_tuned[use][log_len].n != 0;
+        }
+        *flags = (sp.alt_tiles ? kFormAltTiles : 0u) | (sp.inv16 ? kFormInv16 : 0u) |
+                 (sp.per_prime_pack ? kFormPerPrimePack : 0u) | (tuned ? kFormTuned : 0u);
+    }
     return sp.n;
 }
 

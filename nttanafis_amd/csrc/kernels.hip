@@ -685,16 +685,13 @@ static constexpr int pack_primes_threads() {
 // the added input and ends below 9p/16 + 15p/16 = 3p/2, and every later round
 // is entered below 31p/16 (fwd_round's schedule).
 template <int LOGL, int LINES, int NT, int KIND>
-__global__ __launch_bounds__(NT) void k_pack_primes(PassArgs a) {
+__device__ __forceinline__ void pack_primes_tile(const PassArgs& a, const Job& j, int primes, double* lds) {
     constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
     constexpr int D = L / 8;
     constexpr int P = PackParts<KIND>::value;
     static_assert(LINES * L == 8 * NT, "one radix-8 group of the first levels per thread");
-    __shared__ double lds[LINES * LS];
     const uint64_t S = a.line_stride;
-    Job j;
-    resolve<false>(a, S / LINES, false, &j);
     const int c = threadIdx.x % LINES, m0 = threadIdx.x / LINES;
     const uint64_t base = j.tile * LINES;
     const uint64_t n0 = base + c + (uint64_t)m0 * S;  // element m0 + r*D at n0 + r*D*S
@@ -715,7 +712,7 @@ __global__ __launch_bounds__(NT) void k_pack_primes(PassArgs a) {
         fill += n < digits ? 1 : 0;
     }
 #pragma unroll 1
-    for (int y = 0; y < a.primes; ++y) {
+    for (int y = 0; y < primes; ++y) {
         const double p = a.field.p[y], pinv = a.field.pinv[y];
         const double* pw = a.powers[y];
         double x[8];
@@ -746,6 +743,34 @@ __global__ __launch_bounds__(NT) void k_pack_primes(PassArgs a) {
         fwd_rounds_from<LOGL, LINES, NT, 3, 3>(lds, tw, p, pinv);
         store_strided<LOGL, LINES, NT, false>(lds, j.data + (uint64_t)y * a.row_stride + base, S, 0.0, p, pinv);
         __syncthreads();  // the tile is read out before the next prime overwrites it
+    }
+}
+
+template <int LOGL, int LINES, int NT, int KIND>
+__global__ __launch_bounds__(NT) void k_pack_primes(PassArgs a) {
+    __shared__ double lds[LINES * Lay<LOGL>::STRIDE];
+    Job j;
+    resolve<false>(a, a.line_stride / LINES, false, &j);
+    pack_primes_tile<LOGL, LINES, NT, KIND>(a, j, a.primes, lds);
+}
+
+// Batched form: workgroup b packs tile b % tiles_per_item of the operand (or
+// block) that item b / tiles_per_item names, for every prime of its product.
+// The items are (product, block, operand) with prime 0; the host sends no
+// product whose coefficients need four parts.
+template <int LOGL, int LINES, int NT>
+__global__ __launch_bounds__(NT) void k_pack_primes_batch(PassArgs a) {
+    __shared__ double lds[LINES * Lay<LOGL>::STRIDE];
+    Job j;
+    resolve<true>(a, 0, false, &j);
+    const int primes = (int)a.batch[a.items[blockIdx.x / a.tiles_per_item].product].primes;
+    switch (pack_kind(j.src_a)) {
+        case kPackW64: pack_primes_tile<LOGL, LINES, NT, kPackW64>(a, j, primes, lds); break;
+        case kPackW64Signed: pack_primes_tile<LOGL, LINES, NT, kPackW64Signed>(a, j, primes, lds); break;
+        case kPack1: pack_primes_tile<LOGL, LINES, NT, kPack1>(a, j, primes, lds); break;
+        case kPack2: pack_primes_tile<LOGL, LINES, NT, kPack2>(a, j, primes, lds); break;
+        case kPack3: pack_primes_tile<LOGL, LINES, NT, kPack3>(a, j, primes, lds); break;
+        default: break;
     }
 }
 
@@ -793,7 +818,7 @@ __host__ __device__ constexpr int row_split(int logl) { return logl > 4 ? logl -
 // lds + (t*LINES + c)*LS and is node H0 + c. With PACK the row is the whole
 // transform (level_start == 0, LINES == 1): both operands are packed from
 // limbs into LDS and the result is stored as canonical residues times T^-1.
-template <int LOGL, int LINES, int NT, bool PACK, bool SQUARE, bool BATCH>
+template <int LOGL, int LINES, int NT, bool PACK, bool SQUARE, bool BATCH, bool INV16 = false>
 __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     static_assert(!PACK || LINES == 1, "a packed transform is one row");
     constexpr int L = 1 << LOGL;
@@ -839,8 +864,9 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     }
     __syncthreads();
     // The product is one tile: radix-16 groups would occupy half of the
-    // threads, so a product (not a square) runs its inverse in radix 8.
-    if constexpr (SQUARE)
+    // threads, so a product (not a square) runs its inverse in radix 8
+    // unless the autotuner chose radix 16 (INV16) for this shape.
+    if constexpr (SQUARE || INV16)
         inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, twI, p, pinv, PACK && a.node_base == 0);
     else
         inv_rounds8<LOGL, LINES, NT, Lay8<LOGL>::ROUNDS - 1>(lds, twI, p, pinv, PACK && a.node_base == 0);
@@ -1524,17 +1550,16 @@ __global__ __launch_bounds__(256) void k_crt_disjoint(CrtArgs a) {
 // for 32 x 128; 32 x 256 moves 5.3-5.5 TB/s against 4.6-4.8 for 16 x 256.
 
 // Threads of a strided tile: one radix-16 group per thread and round.
-template <int LOGL>
+template <int LOGL, int LINES>
 static constexpr int strided_threads() {
-    return strided_lines(LOGL) * (1 << LOGL) / 16 >= 512 ? 512 : 256;
+    return LINES * (1 << LOGL) / 16 >= 512 ? 512 : 256;
 }
 
 // In a batched launch (a.batch set) `units` counts work items (product,
 // prime); otherwise it counts primes.
-template <int LOGL>
-static bool launch_fwd_strided_t(const PassArgs& a, int units, int operands, bool pack, hipStream_t s) {
-    constexpr int LINES = strided_lines(LOGL);
-    constexpr int NT = strided_threads<LOGL>();
+template <int LOGL, int LINES>
+static bool launch_fwd_strided_l(const PassArgs& a, int units, int operands, bool pack, hipStream_t s) {
+    constexpr int NT = strided_threads<LOGL, LINES>();
     if (a.line_stride < (uint64_t)LINES) return false;
     const uint64_t nodes = (uint64_t)1 << a.level_start;
     const uint64_t tiles = nodes * (a.line_stride / LINES);
@@ -1557,6 +1582,12 @@ static bool launch_fwd_strided_t(const PassArgs& a, int units, int operands, boo
             NTT_LAUNCH(nullptr, (k_fwd_strided<LOGL, LINES, NT, false, false>), grid, dim3(NT), s, b);
     }
     return NTT_CHECK_LAUNCH();
+}
+
+template <int LOGL>
+static bool launch_fwd_strided_t(const PassArgs& a, int units, int operands, bool pack, bool alt, hipStream_t s) {
+    return alt ? launch_fwd_strided_l<LOGL, alt_strided_lines(LOGL)>(a, units, operands, pack, s)
+               : launch_fwd_strided_l<LOGL, strided_lines(LOGL)>(a, units, operands, pack, s);
 }
 
 // First forward pass of a single operation with packing shared across its
@@ -1585,10 +1616,9 @@ static bool launch_pack_primes_t(const PassArgs& a, int primes, int operands, hi
     return NTT_CHECK_LAUNCH();
 }
 
-template <int LOGL>
-static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, hipStream_t s) {
-    constexpr int LINES = strided_lines(LOGL);
-    constexpr int NT = strided_threads<LOGL>();
+template <int LOGL, int LINES>
+static bool launch_inv_strided_l(const PassArgs& a, int units, bool canon, hipStream_t s) {
+    constexpr int NT = strided_threads<LOGL, LINES>();
     if (a.line_stride < (uint64_t)LINES) return false;
     const uint64_t nodes = (uint64_t)1 << a.level_start;
     const uint64_t tiles = nodes * (a.line_stride / LINES);
@@ -1614,6 +1644,11 @@ static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, hipSt
     return NTT_CHECK_LAUNCH();
 }
 
+template <int LOGL>
+static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, bool alt, hipStream_t s) {
+    return alt ? launch_inv_strided_l<LOGL, alt_strided_lines(LOGL)>(a, units, canon, s)
+               : launch_inv_strided_l<LOGL, strided_lines(LOGL)>(a, units, canon, s);
+}
 
 // Shapes of the fused row pass: about 256 radix-16 groups per workgroup
 // round (enough rows per workgroup to fill it), one thread per group.
@@ -1626,7 +1661,7 @@ struct FusedShape {
     static constexpr int NT = NT_RAW < 64 ? 64 : (NT_RAW > 512 ? 512 : NT_RAW);
 };
 
-template <int LOGL, bool PACK, bool SQUARE>
+template <int LOGL, bool PACK, bool SQUARE, bool INV16>
 static bool launch_fused_v(const PassArgs& a, int units, hipStream_t s) {
     using F = FusedShape<LOGL, PACK, SQUARE>;
     const uint64_t rows = (uint64_t)1 << a.level_start;
@@ -1637,28 +1672,35 @@ static bool launch_fused_v(const PassArgs& a, int units, hipStream_t s) {
     dim3 grid((unsigned)tiles, (unsigned)units, a.split_blocks ? a.split_blocks : 1);
     if (a.batch) {
         grid = dim3((unsigned)(tiles * units), 1);
-        NTT_LAUNCH(nullptr, (k_fused<LOGL, F::LINES, F::NT, PACK, SQUARE, true>), grid, dim3(F::NT), s, b);
+        NTT_LAUNCH(nullptr, (k_fused<LOGL, F::LINES, F::NT, PACK, SQUARE, true, INV16>), grid, dim3(F::NT), s, b);
     } else {
-        NTT_LAUNCH(nullptr, (k_fused<LOGL, F::LINES, F::NT, PACK, SQUARE, false>), grid, dim3(F::NT), s, b);
+        NTT_LAUNCH(nullptr, (k_fused<LOGL, F::LINES, F::NT, PACK, SQUARE, false, INV16>), grid, dim3(F::NT), s, b);
     }
     return NTT_CHECK_LAUNCH();
 }
 
+// inv16: a product (not a square) runs its inverse rows in radix 16.
 template <int LOGL>
-static bool launch_fused_t(const PassArgs& a, int units, bool pack, bool square, hipStream_t s) {
-    if (pack) return square ? launch_fused_v<LOGL, true, true>(a, units, s) : launch_fused_v<LOGL, true, false>(a, units, s);
-    return square ? launch_fused_v<LOGL, false, true>(a, units, s) : launch_fused_v<LOGL, false, false>(a, units, s);
+static bool launch_fused_t(const PassArgs& a, int units, bool pack, bool square, bool inv16, hipStream_t s) {
+    if (square)
+        return pack ? launch_fused_v<LOGL, true, true, false>(a, units, s)
+                    : launch_fused_v<LOGL, false, true, false>(a, units, s);
+    if (inv16)
+        return pack ? launch_fused_v<LOGL, true, false, true>(a, units, s)
+                    : launch_fused_v<LOGL, false, false, true>(a, units, s);
+    return pack ? launch_fused_v<LOGL, true, false, false>(a, units, s)
+                : launch_fused_v<LOGL, false, false, false>(a, units, s);
 }
 
-bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, bool pack, hipStream_t s) {
+bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, bool pack, bool alt, hipStream_t s) {
     switch (logl) {
-        case 5: return launch_fwd_strided_t<5>(a, primes, operands, pack, s);
-        case 6: return launch_fwd_strided_t<6>(a, primes, operands, pack, s);
-        case 7: return launch_fwd_strided_t<7>(a, primes, operands, pack, s);
-        case 8: return launch_fwd_strided_t<8>(a, primes, operands, pack, s);
-        case 9: return launch_fwd_strided_t<9>(a, primes, operands, pack, s);
-        case 10: return launch_fwd_strided_t<10>(a, primes, operands, pack, s);
-        case 11: return launch_fwd_strided_t<11>(a, primes, operands, pack, s);
+        case 5: return launch_fwd_strided_t<5>(a, primes, operands, pack, alt, s);
+        case 6: return launch_fwd_strided_t<6>(a, primes, operands, pack, alt, s);
+        case 7: return launch_fwd_strided_t<7>(a, primes, operands, pack, alt, s);
+        case 8: return launch_fwd_strided_t<8>(a, primes, operands, pack, alt, s);
+        case 9: return launch_fwd_strided_t<9>(a, primes, operands, pack, alt, s);
+        case 10: return launch_fwd_strided_t<10>(a, primes, operands, pack, alt, s);
+        case 11: return launch_fwd_strided_t<11>(a, primes, operands, pack, alt, s);
         default: return false;
     }
 }
@@ -1670,6 +1712,7 @@ bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, b
 bool launch_pack_primes(int logl, const PassArgs& a, int primes, int operands, hipStream_t s) {
     if (pack_kind(a.src_a) != pack_kind(a.src_b) || pack_kind(a.src_a) == kPack4) return false;
     switch (logl) {
+        case 5: return launch_pack_primes_t<5>(a, primes, operands, s);
         case 6: return launch_pack_primes_t<6>(a, primes, operands, s);
         case 7: return launch_pack_primes_t<7>(a, primes, operands, s);
         case 8: return launch_pack_primes_t<8>(a, primes, operands, s);
@@ -1678,30 +1721,59 @@ bool launch_pack_primes(int logl, const PassArgs& a, int primes, int operands, h
     }
 }
 
-bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, hipStream_t s) {
+// Batched shared packing: `items` (product, block, operand) units, every
+// product of kind below kPack4 (pack_primes_batch_fits checks the shape).
+template <int LOGL>
+static bool launch_pack_primes_batch_t(const PassArgs& a, int items, hipStream_t s) {
+    constexpr int LINES = pack_primes_lines<LOGL>();
+    constexpr int NT = pack_primes_threads<LOGL>();
+    if (a.line_stride < (uint64_t)LINES || a.level_start != 0 || !a.batch) return false;
+    PassArgs b = a;
+    b.tiles_per_item = a.line_stride / LINES;
+    const dim3 grid((unsigned)(b.tiles_per_item * (uint64_t)items), 1);
+    NTT_LAUNCH(nullptr, (k_pack_primes_batch<LOGL, LINES, NT>), grid, dim3(NT), s, b);
+    return NTT_CHECK_LAUNCH();
+}
+
+bool pack_primes_batch_fits(int logl, uint64_t line_stride) {
+    return logl >= 5 && logl <= 9 && line_stride >= (uint64_t)(4096 >> logl);
+}
+
+bool launch_pack_primes_batch(int logl, const PassArgs& a, int items, hipStream_t s) {
     switch (logl) {
-        case 5: return launch_inv_strided_t<5>(a, primes, canon, s);
-        case 6: return launch_inv_strided_t<6>(a, primes, canon, s);
-        case 7: return launch_inv_strided_t<7>(a, primes, canon, s);
-        case 8: return launch_inv_strided_t<8>(a, primes, canon, s);
-        case 9: return launch_inv_strided_t<9>(a, primes, canon, s);
-        case 10: return launch_inv_strided_t<10>(a, primes, canon, s);
-        case 11: return launch_inv_strided_t<11>(a, primes, canon, s);
+        case 5: return launch_pack_primes_batch_t<5>(a, items, s);
+        case 6: return launch_pack_primes_batch_t<6>(a, items, s);
+        case 7: return launch_pack_primes_batch_t<7>(a, items, s);
+        case 8: return launch_pack_primes_batch_t<8>(a, items, s);
+        case 9: return launch_pack_primes_batch_t<9>(a, items, s);
+        default: return false;
+    }
+}
+
+bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, bool alt, hipStream_t s) {
+    switch (logl) {
+        case 5: return launch_inv_strided_t<5>(a, primes, canon, alt, s);
+        case 6: return launch_inv_strided_t<6>(a, primes, canon, alt, s);
+        case 7: return launch_inv_strided_t<7>(a, primes, canon, alt, s);
+        case 8: return launch_inv_strided_t<8>(a, primes, canon, alt, s);
+        case 9: return launch_inv_strided_t<9>(a, primes, canon, alt, s);
+        case 10: return launch_inv_strided_t<10>(a, primes, canon, alt, s);
+        case 11: return launch_inv_strided_t<11>(a, primes, canon, alt, s);
         default: return false;
     }
 }
 
 
-bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, hipStream_t s) {
+bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, bool inv16, hipStream_t s) {
     switch (logl) {
-        case 5: return launch_fused_t<5>(a, primes, pack, square, s);
-        case 6: return launch_fused_t<6>(a, primes, pack, square, s);
-        case 7: return launch_fused_t<7>(a, primes, pack, square, s);
-        case 8: return launch_fused_t<8>(a, primes, pack, square, s);
-        case 9: return launch_fused_t<9>(a, primes, pack, square, s);
-        case 10: return launch_fused_t<10>(a, primes, pack, square, s);
-        case 11: return launch_fused_t<11>(a, primes, pack, square, s);
-        case 12: return launch_fused_t<12>(a, primes, pack, square, s);
+        case 5: return launch_fused_t<5>(a, primes, pack, square, inv16, s);
+        case 6: return launch_fused_t<6>(a, primes, pack, square, inv16, s);
+        case 7: return launch_fused_t<7>(a, primes, pack, square, inv16, s);
+        case 8: return launch_fused_t<8>(a, primes, pack, square, inv16, s);
+        case 9: return launch_fused_t<9>(a, primes, pack, square, inv16, s);
+        case 10: return launch_fused_t<10>(a, primes, pack, square, inv16, s);
+        case 11: return launch_fused_t<11>(a, primes, pack, square, inv16, s);
+        case 12: return launch_fused_t<12>(a, primes, pack, square, inv16, s);
         default: return false;
     }
 }
