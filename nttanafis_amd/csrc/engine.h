@@ -169,16 +169,19 @@ struct Split {
     int logs[3];  // strided passes, then the row pass (last)
     // Kernel forms the autotuner may choose beside the levels: the
     // alternative strided tiles (alt_strided_lines), radix-16 inverse rows
-    // in the product row kernel, and packing per (tile, prime) instead of
-    // packing shared across the primes of a tile.
-    uint8_t alt_tiles, inv16, per_prime_pack, pad;
+    // in the product row kernel, packing per (tile, prime) instead of
+    // packing shared across the primes of a tile, and radix-8 row rounds
+    // with twice the threads (taking precedence over inv16).
+    uint8_t alt_tiles, inv16, per_prime_pack, rows8;
+    int fused_form() const { return rows8 ? 2 : (inv16 ? 1 : 0); }
 };
 
 // Which tuned split a transform takes: a single operation's product (two
 // operands), its square (one operand), or a batch bucket.
 enum SplitUse { kUseProduct = 0, kUseSquare = 1, kUseBatch = 2 };
 constexpr uint32_t kFormAltTiles = NTT_FORM_ALT_TILES, kFormInv16 = NTT_FORM_INV16,
-                   kFormPerPrimePack = NTT_FORM_PER_PRIME_PACK, kFormTuned = NTT_FORM_TUNED;
+                   kFormPerPrimePack = NTT_FORM_PER_PRIME_PACK, kFormRows8 = NTT_FORM_ROWS8,
+                   kFormTuned = NTT_FORM_TUNED;
 Split split_of(int K, int use);
 
 struct BatchLaunch {
@@ -186,7 +189,8 @@ struct BatchLaunch {
     int logl;
     bool flag;  // pack (forward strided), canonical output (inverse strided), square (fused)
     bool pack;  // fused single pass: pack the operands
-    bool alt;   // strided passes: alternative tiles; fused: radix-16 inverse rows
+    bool alt;   // strided passes: alternative tiles
+    int form;   // fused passes: Split::fused_form
     int units;
     const char* name;
     PassArgs pass;
@@ -223,6 +227,7 @@ struct Batch {
     // segments (the launches of one bucket, or the schoolbook launch); the
     // reconstruction launches from seg.back() on depend on all of them.
     std::vector<size_t> seg;
+    std::vector<float> seg_ms;  // measured duration of each independent segment (empty: not measured)
     std::vector<hipStream_t> side;  // branches of the captured graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;

@@ -23,7 +23,7 @@ typedef unsigned __int128 u128;
 bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, bool pack, bool alt, hipStream_t s);
 bool launch_pack_primes(int logl, const PassArgs& a, int primes, int operands, hipStream_t s);
 bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, bool alt, hipStream_t s);
-bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, bool inv16, hipStream_t s);
+bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, int form, hipStream_t s);
 bool pack_primes_batch_fits(int logl, uint64_t line_stride);
 bool launch_pack_primes_batch(int logl, const PassArgs& a, int items, hipStream_t s);
 bool launch_crt_product(const CrtArgs& a, hipStream_t s);
@@ -577,7 +577,8 @@ static bool split_ok(int K, const Split& s) {
 }
 
 static bool same_split(const Split& a, const Split& b) {
-    if (a.n != b.n || a.alt_tiles != b.alt_tiles || a.inv16 != b.inv16 || a.per_prime_pack != b.per_prime_pack)
+    if (a.n != b.n || a.alt_tiles != b.alt_tiles || a.inv16 != b.inv16 || a.per_prime_pack != b.per_prime_pack ||
+        a.rows8 != b.rows8)
         return false;
     for (int i = 0; i < a.n; ++i)
         if (a.logs[i] != b.logs[i]) return false;
@@ -585,18 +586,19 @@ static bool same_split(const Split& a, const Split& b) {
 }
 
 // Kernel-form flags forced from the environment (tests, A/B runs):
-// NTTAMD_ALT_TILES, NTTAMD_INV16 and NTTAMD_PER_PRIME_PACK set to 0 or 1.
-// -1: not forced.
+// NTTAMD_ALT_TILES, NTTAMD_INV16, NTTAMD_PER_PRIME_PACK and NTTAMD_ROWS8 set
+// to 0 or 1. -1: not forced.
 static int env_flag(const char* name) {
     const char* e = getenv(name);
     return e && *e ? (atoi(e) != 0) : -1;
 }
 static bool flags_forced() {
-    return env_flag("NTTAMD_ALT_TILES") >= 0 || env_flag("NTTAMD_INV16") >= 0 || env_flag("NTTAMD_PER_PRIME_PACK") >= 0;
+    return env_flag("NTTAMD_ALT_TILES") >= 0 || env_flag("NTTAMD_INV16") >= 0 ||
+           env_flag("NTTAMD_PER_PRIME_PACK") >= 0 || env_flag("NTTAMD_ROWS8") >= 0;
 }
 static Split with_env_flags(int K, Split s) {
     const int alt = env_flag("NTTAMD_ALT_TILES"), inv = env_flag("NTTAMD_INV16"),
-              ppp = env_flag("NTTAMD_PER_PRIME_PACK");
+              ppp = env_flag("NTTAMD_PER_PRIME_PACK"), r8 = env_flag("NTTAMD_ROWS8");
     if (alt >= 0 && s.n > 1) {
         Split t = s;
         t.alt_tiles = (uint8_t)alt;
@@ -604,14 +606,17 @@ static Split with_env_flags(int K, Split s) {
     }
     if (inv >= 0) s.inv16 = (uint8_t)inv;
     if (ppp >= 0 && s.n > 1) s.per_prime_pack = (uint8_t)ppp;
+    if (r8 >= 0) s.rows8 = (uint8_t)r8;
     return s;
 }
 
-// The flag variants the autotuner tries after choosing the levels, in order
-// (each kept when it beats the best so far): alternative strided tiles,
-// radix-16 inverse rows (products; a square's rows already run radix 16),
+// The flag variants the autotuner tries after choosing the levels, in order,
+// each on top of the flags kept so far and kept when it beats the best so
+// far: alternative strided tiles, radix-16 inverse rows (products; a
+// square's rows already run radix 16), radix-8 rows with twice the threads,
 // packing per prime instead of the shared first packing pass.
-static int flag_variants(int K, int use, const Split& best, Split out[3]) {
+constexpr int kMaxVariants = 4;
+static int flag_variants(int K, int use, const Split& best, Split out[kMaxVariants]) {
     int n = 0;
     Split t = best;
     t.alt_tiles = 1;
@@ -620,9 +625,21 @@ static int flag_variants(int K, int use, const Split& best, Split out[3]) {
     t.inv16 = 1;
     if (use != kUseSquare && !best.inv16) out[n++] = t;
     t = best;
+    t.rows8 = 1;
+    if (!best.rows8) out[n++] = t;
+    t = best;
     t.per_prime_pack = 1;
     if (best.n > 1 && !best.per_prime_pack) out[n++] = t;
     return n;
+}
+
+// best with the flag variant v sets on top.
+static Split with_variant(Split best, const Split& v) {
+    best.alt_tiles |= v.alt_tiles;
+    best.inv16 |= v.inv16;
+    best.per_prime_pack |= v.per_prime_pack;
+    best.rows8 |= v.rows8;
+    return best;
 }
 
 // Admissible splits, in a fixed array (split_of runs on the allocation-free
@@ -846,7 +863,7 @@ static bool part_residues(hipStream_t s, const Geometry& g, const RowPart& part,
         a.src_b = sb;
         a.level_start = 0;
         Timed t("fused_single", s);
-        return launch_fused(part.log_n, a, np, true, square, sp.inv16, s);
+        return launch_fused(part.log_n, a, np, true, square, sp.fused_form(), s);
     }
     const int row_log = sp.logs[sp.n - 1];
     const int row_level = part.log_n - row_log;
@@ -863,7 +880,7 @@ static bool part_residues(hipStream_t s, const Geometry& g, const RowPart& part,
     pf.line_stride = 1;
     {
         Timed t("fused_rows", s);
-        if (!launch_fused(row_log, pf, np, false, square, sp.inv16, s)) return false;
+        if (!launch_fused(row_log, pf, np, false, square, sp.fused_form(), s)) return false;
     }
     return inverse_tail(sp, pf, np, s);
 }
@@ -915,7 +932,7 @@ static bool split_residues(hipStream_t s, const Geometry& g, const Source& sa, c
     *rows_out = buf.spec_a;
     if (sp.n == 1) {
         Timed t("fused_single", s);
-        return launch_fused(g.log_t, a, np, true, false, sp.inv16, s);
+        return launch_fused(g.log_t, a, np, true, false, sp.fused_form(), s);
     }
     if (!forward_head(sp, a, lng, sht, (int)g.blocks + 1, np, s)) return false;
     const int row_log = sp.logs[sp.n - 1];
@@ -923,7 +940,7 @@ static bool split_residues(hipStream_t s, const Geometry& g, const Source& sa, c
     a.line_stride = 1;
     {
         Timed t("fused_rows", s);
-        if (!launch_fused(row_log, a, np, false, false, sp.inv16, s)) return false;
+        if (!launch_fused(row_log, a, np, false, false, sp.fused_form(), s)) return false;
     }
     return inverse_tail(sp, a, np, s);
 }
@@ -1131,14 +1148,11 @@ static void tune_split(Executor& ex, int K, int use, const std::function<bool()>
                 best = sp;
             }
         }
-    Split var[3];
+    Split var[kMaxVariants];
     const int nv = flag_variants(K, use, best, var);
     if (nv && cand.size() == 1) best_ms = time_of(best);
     for (int i = 0; i < nv; ++i) {
-        Split t = best;  // the flag of variant i on top of the flags kept so far
-        t.alt_tiles |= var[i].alt_tiles;
-        t.inv16 |= var[i].inv16;
-        t.per_prime_pack |= var[i].per_prime_pack;
+        const Split t = with_variant(best, var[i]);
         if (!split_ok(K, t) && t.n > 1) continue;
         const float ms = time_of(t);
         if (ms < best_ms) {
@@ -1324,7 +1338,7 @@ static bool issue_launch(const BatchLaunch& L, hipStream_t s) {
     switch (L.kind) {
         case BatchLaunch::kFwdStrided: return launch_fwd_strided(L.logl, L.pass, L.units, 1, L.flag, L.alt, s);
         case BatchLaunch::kPackPrimes: return launch_pack_primes_batch(L.logl, L.pass, L.units, s);
-        case BatchLaunch::kFused: return launch_fused(L.logl, L.pass, L.units, L.pack, L.flag, L.alt, s);
+        case BatchLaunch::kFused: return launch_fused(L.logl, L.pass, L.units, L.pack, L.flag, L.form, s);
         case BatchLaunch::kInvStrided: return launch_inv_strided(L.logl, L.pass, L.units, L.flag, L.alt, s);
         case BatchLaunch::kCrt: return launch_crt_product(L.crt, s);
         case BatchLaunch::kSchool: return launch_school(L.crt, s);
@@ -1353,7 +1367,8 @@ static void bucket_program(std::vector<BatchLaunch>& prog, const PassArgs& base,
         L.logl = logl;
         L.flag = flag;
         L.pack = pack;
-        L.alt = kind == BatchLaunch::kFused ? sp.inv16 != 0 : sp.alt_tiles != 0;
+        L.alt = sp.alt_tiles != 0;
+        L.form = sp.fused_form();
         L.units = (int)units;
         L.name = name;
         L.pass = base;
@@ -1623,6 +1638,8 @@ static float time_launches(const std::vector<BatchLaunch>& prog, hipStream_t s, 
     return ms;
 }
 
+static int batch_branches();
+
 // Times the transform work of each untuned bucket under each candidate split,
 // then the kernel-form flags on the fastest (flag_variants), and keeps the
 // fastest. The operands must be bound: the timed passes read them.
@@ -1660,14 +1677,11 @@ static bool batch_autotune(Batch& B) {
                     best = sp;
                 }
             }
-        Split var[3];
+        Split var[kMaxVariants];
         const int nv = flag_variants(r.K, kUseBatch, best, var);
         if (nv && cand.size() == 1) best_ms = time_of(best);
         for (int i = 0; i < nv; ++i) {
-            Split t = best;
-            t.alt_tiles |= var[i].alt_tiles;
-            t.inv16 |= var[i].inv16;
-            t.per_prime_pack |= var[i].per_prime_pack;
+            const Split t = with_variant(best, var[i]);
             if (!split_ok(r.K, t) && t.n > 1) continue;
             const float ms = time_of(t);
             if (ms < best_ms) {
@@ -1689,6 +1703,19 @@ static bool batch_autotune(Batch& B) {
     if (changed) {
         B.drop_graph();
         build_program(B);
+    }
+    // Each independent segment timed alone, for the branch assignment of the
+    // captured graph.
+    B.seg_ms.clear();
+    if (batch_branches() > 1) {
+        for (size_t i = 0; i + 1 < B.seg.size(); ++i) {
+            const std::vector<BatchLaunch> part(B.program.begin() + B.seg[i], B.program.begin() + B.seg[i + 1]);
+            B.seg_ms.push_back(part.empty() ? 0.0f : time_launches(part, B.ex.stream, 2));
+        }
+        if (hipStreamSynchronize(B.ex.stream) != hipSuccess) {
+            set_error("batch: segment timing failed");
+            return false;
+        }
     }
     return true;
 }
@@ -1744,14 +1771,53 @@ static bool issue_branched(Batch& B, hipStream_t s, const std::vector<hipEvent_t
         }
         return true;
     };
-    // Segments in transform-length order, dealt round robin.
+    // Longest segment first onto the least-loaded branch, by the durations
+    // measured at tuning time (round robin in transform-length order when
+    // none were measured); the reconstructions likewise, by blocks times
+    // primes.
+    auto deal = [&](const std::vector<double>& cost, const std::vector<std::pair<size_t, size_t>>& range,
+                    std::vector<std::vector<const BatchLaunch*>>& out) {
+        std::vector<size_t> order(cost.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+        std::vector<double> load(out.size(), 0.0);
+        for (size_t i : order) {
+            const size_t k = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[k] += cost[i];
+            for (size_t l = range[i].first; l < range[i].second; ++l) out[k].push_back(&B.program[l]);
+        }
+    };
     std::vector<std::vector<const BatchLaunch*>> work(B.side.size());
-    for (size_t i = 0; i < nseg; ++i)
-        for (size_t l = B.seg[i]; l < B.seg[i + 1]; ++l) work[i % work.size()].push_back(&B.program[l]);
+    const char* lpt = getenv("NTTAMD_BATCH_LPT");  // 0: round robin (A/B runs)
+    if (B.seg_ms.size() == nseg && !(lpt && lpt[0] == '0')) {
+        std::vector<double> cost(nseg);
+        std::vector<std::pair<size_t, size_t>> range(nseg);
+        for (size_t i = 0; i < nseg; ++i) {
+            cost[i] = B.seg_ms[i];
+            range[i] = {B.seg[i], B.seg[i + 1]};
+        }
+        deal(cost, range, work);
+    } else {
+        for (size_t i = 0; i < nseg; ++i)
+            for (size_t l = B.seg[i]; l < B.seg[i + 1]; ++l) work[i % work.size()].push_back(&B.program[l]);
+    }
     if (!fork_join(work)) return false;
     std::vector<std::vector<const BatchLaunch*>> crt(B.side.size());
     const size_t first = B.seg.empty() ? 0 : B.seg.back();
-    for (size_t l = first; l < B.program.size(); ++l) crt[(l - first) % crt.size()].push_back(&B.program[l]);
+    std::vector<double> cost;
+    std::vector<std::pair<size_t, size_t>> range;
+    for (size_t l = first; l < B.program.size(); ++l) {
+        const BatchLaunch& L = B.program[l];
+        // A reconstruction launch: blocks times primes (the schoolbook one
+        // is not among them); its carry launches follow it on its branch.
+        if (L.kind == BatchLaunch::kCrt || L.kind == BatchLaunch::kSchool || range.empty()) {
+            cost.push_back((double)L.crt.blocks * (L.kind == BatchLaunch::kCrt ? std::max<uint32_t>(1u, L.crt.primes) : 1u));
+            range.push_back({l, l + 1});
+        } else {
+            range.back().second = l + 1;
+        }
+    }
+    deal(cost, range, crt);
     return fork_join(crt);
 }
 
@@ -2562,7 +2628,8 @@ int ntt_transform_split(uint32_t log_len, int use, uint32_t* pass_logs, uint32_t
             tuned = g_tuned[use][log_len].n != 0;
         }
         *flags = (sp.alt_tiles ? kFormAltTiles : 0u) | (sp.inv16 ? kFormInv16 : 0u) |
-                 (sp.per_prime_pack ? kFormPerPrimePack : 0u) | (tuned ? kFormTuned : 0u);
+                 (sp.per_prime_pack ? kFormPerPrimePack : 0u) | (sp.rows8 ? kFormRows8 : 0u) |
+                 (tuned ? kFormTuned : 0u);
     }
     return sp.n;
 }

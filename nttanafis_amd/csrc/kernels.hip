@@ -818,7 +818,11 @@ __host__ __device__ constexpr int row_split(int logl) { return logl > 4 ? logl -
 // lds + (t*LINES + c)*LS and is node H0 + c. With PACK the row is the whole
 // transform (level_start == 0, LINES == 1): both operands are packed from
 // limbs into LDS and the result is stored as canonical residues times T^-1.
-template <int LOGL, int LINES, int NT, bool PACK, bool SQUARE, bool BATCH, bool INV16 = false>
+// FORM 0: radix-16 forward rounds, inverse rounds in radix 16 for a square
+// and radix 8 for a product; FORM 1: radix 16 both ways; FORM 2: radix 8
+// both ways, with one thread per radix-8 group (twice the threads of the
+// radix-16 shapes over the same tile: more waves per CU, one more round).
+template <int LOGL, int LINES, int NT, bool PACK, bool SQUARE, bool BATCH, int FORM = 0>
 __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     static_assert(!PACK || LINES == 1, "a packed transform is one row");
     constexpr int L = 1 << LOGL;
@@ -833,6 +837,7 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     const int y = j.y;
     const double p = a.field.p[y], pinv = a.field.pinv[y];
     const uint64_t H0 = j.tile * LINES;
+    const uint64_t HR = H0 + (a.node_base << a.level_start);
     double* row = j.data + (uint64_t)y * a.row_stride + H0 * L;
     if constexpr (PACK) {
         const Fold fold = fold_of(a, y);
@@ -845,12 +850,14 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     }
     __syncthreads();
     const int split = PACK ? 64 : row_split(LOGL);
-    const uint64_t HR = H0 + (a.node_base << a.level_start);
     const Tw twF{a.fwd + (uint64_t)y * a.table_stride, HR, 1, LINES - 1, split, p, pinv};
     const Tw twI{a.inv + (uint64_t)y * a.table_stride, HR, 1, LINES - 1, split, p, pinv};
     // The lines of both tiles go through one set of rounds; line c of either
     // tile is node H0 + (c mod LINES).
-    fwd_rounds<LOGL, TILES * LINES, NT, 0>(lds, twF, p, pinv, PACK && a.node_base == 0);
+    if constexpr (FORM == 2)
+        fwd_rounds_from<LOGL, TILES * LINES, NT, 0, 3>(lds, twF, p, pinv);
+    else
+        fwd_rounds<LOGL, TILES * LINES, NT, 0>(lds, twF, p, pinv, PACK && a.node_base == 0);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int e = threadIdx.x + r * NT;
@@ -865,8 +872,8 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     __syncthreads();
     // The product is one tile: radix-16 groups would occupy half of the
     // threads, so a product (not a square) runs its inverse in radix 8
-    // unless the autotuner chose radix 16 (INV16) for this shape.
-    if constexpr (SQUARE || INV16)
+    // unless the autotuner chose radix 16 (FORM 1) for this shape.
+    if constexpr (FORM != 2 && (SQUARE || FORM == 1))
         inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, twI, p, pinv, PACK && a.node_base == 0);
     else
         inv_rounds8<LOGL, LINES, NT, Lay8<LOGL>::ROUNDS - 1>(lds, twI, p, pinv, PACK && a.node_base == 0);
@@ -1651,19 +1658,20 @@ static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, bool 
 }
 
 // Shapes of the fused row pass: about 256 radix-16 groups per workgroup
-// round (enough rows per workgroup to fill it), one thread per group.
-template <int LOGL, bool PACK, bool SQUARE>
+// round (enough rows per workgroup to fill it), one thread per group; the
+// radix-8 form (FORM 2) keeps the tile and doubles the threads.
+template <int LOGL, bool PACK, bool SQUARE, int FORM = 0>
 struct FusedShape {
     static constexpr int TILES = SQUARE ? 1 : 2;
     static constexpr int GROUPS = TILES * (1 << LOGL) / 16;  // per row, per round
     static constexpr int LINES = PACK ? 1 : (GROUPS >= 256 ? 1 : (256 / GROUPS > 32 ? 32 : 256 / GROUPS));
-    static constexpr int NT_RAW = GROUPS * LINES;
+    static constexpr int NT_RAW = GROUPS * LINES * (FORM == 2 ? 2 : 1);
     static constexpr int NT = NT_RAW < 64 ? 64 : (NT_RAW > 512 ? 512 : NT_RAW);
 };
 
-template <int LOGL, bool PACK, bool SQUARE, bool INV16>
+template <int LOGL, bool PACK, bool SQUARE, int FORM>
 static bool launch_fused_v(const PassArgs& a, int units, hipStream_t s) {
-    using F = FusedShape<LOGL, PACK, SQUARE>;
+    using F = FusedShape<LOGL, PACK, SQUARE, FORM>;
     const uint64_t rows = (uint64_t)1 << a.level_start;
     if (rows % F::LINES != 0) return false;
     const uint64_t tiles = rows / F::LINES;
@@ -1672,24 +1680,30 @@ static bool launch_fused_v(const PassArgs& a, int units, hipStream_t s) {
     dim3 grid((unsigned)tiles, (unsigned)units, a.split_blocks ? a.split_blocks : 1);
     if (a.batch) {
         grid = dim3((unsigned)(tiles * units), 1);
-        NTT_LAUNCH(nullptr, (k_fused<LOGL, F::LINES, F::NT, PACK, SQUARE, true, INV16>), grid, dim3(F::NT), s, b);
+        NTT_LAUNCH(nullptr, (k_fused<LOGL, F::LINES, F::NT, PACK, SQUARE, true, FORM>), grid, dim3(F::NT), s, b);
     } else {
-        NTT_LAUNCH(nullptr, (k_fused<LOGL, F::LINES, F::NT, PACK, SQUARE, false, INV16>), grid, dim3(F::NT), s, b);
+        NTT_LAUNCH(nullptr, (k_fused<LOGL, F::LINES, F::NT, PACK, SQUARE, false, FORM>), grid, dim3(F::NT), s, b);
     }
     return NTT_CHECK_LAUNCH();
 }
 
-// inv16: a product (not a square) runs its inverse rows in radix 16.
+// form: k_fused's FORM (1, radix-16 inverse rows, applies to products only).
+template <int LOGL, bool SQUARE>
+static bool launch_fused_f(const PassArgs& a, int units, bool pack, int form, hipStream_t s) {
+    if (form == 2)
+        return pack ? launch_fused_v<LOGL, true, SQUARE, 2>(a, units, s)
+                    : launch_fused_v<LOGL, false, SQUARE, 2>(a, units, s);
+    if (form == 1 && !SQUARE)
+        return pack ? launch_fused_v<LOGL, true, SQUARE, 1>(a, units, s)
+                    : launch_fused_v<LOGL, false, SQUARE, 1>(a, units, s);
+    return pack ? launch_fused_v<LOGL, true, SQUARE, 0>(a, units, s)
+                : launch_fused_v<LOGL, false, SQUARE, 0>(a, units, s);
+}
+
 template <int LOGL>
-static bool launch_fused_t(const PassArgs& a, int units, bool pack, bool square, bool inv16, hipStream_t s) {
-    if (square)
-        return pack ? launch_fused_v<LOGL, true, true, false>(a, units, s)
-                    : launch_fused_v<LOGL, false, true, false>(a, units, s);
-    if (inv16)
-        return pack ? launch_fused_v<LOGL, true, false, true>(a, units, s)
-                    : launch_fused_v<LOGL, false, false, true>(a, units, s);
-    return pack ? launch_fused_v<LOGL, true, false, false>(a, units, s)
-                : launch_fused_v<LOGL, false, false, false>(a, units, s);
+static bool launch_fused_t(const PassArgs& a, int units, bool pack, bool square, int form, hipStream_t s) {
+    return square ? launch_fused_f<LOGL, true>(a, units, pack, form, s)
+                  : launch_fused_f<LOGL, false>(a, units, pack, form, s);
 }
 
 bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, bool pack, bool alt, hipStream_t s) {
@@ -1764,16 +1778,16 @@ bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, boo
 }
 
 
-bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, bool inv16, hipStream_t s) {
+bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, int form, hipStream_t s) {
     switch (logl) {
-        case 5: return launch_fused_t<5>(a, primes, pack, square, inv16, s);
-        case 6: return launch_fused_t<6>(a, primes, pack, square, inv16, s);
-        case 7: return launch_fused_t<7>(a, primes, pack, square, inv16, s);
-        case 8: return launch_fused_t<8>(a, primes, pack, square, inv16, s);
-        case 9: return launch_fused_t<9>(a, primes, pack, square, inv16, s);
-        case 10: return launch_fused_t<10>(a, primes, pack, square, inv16, s);
-        case 11: return launch_fused_t<11>(a, primes, pack, square, inv16, s);
-        case 12: return launch_fused_t<12>(a, primes, pack, square, inv16, s);
+        case 5: return launch_fused_t<5>(a, primes, pack, square, form, s);
+        case 6: return launch_fused_t<6>(a, primes, pack, square, form, s);
+        case 7: return launch_fused_t<7>(a, primes, pack, square, form, s);
+        case 8: return launch_fused_t<8>(a, primes, pack, square, form, s);
+        case 9: return launch_fused_t<9>(a, primes, pack, square, form, s);
+        case 10: return launch_fused_t<10>(a, primes, pack, square, form, s);
+        case 11: return launch_fused_t<11>(a, primes, pack, square, form, s);
+        case 12: return launch_fused_t<12>(a, primes, pack, square, form, s);
         default: return false;
     }
 }

@@ -1,11 +1,12 @@
 """Every kernel form the autotuner may choose gives the same limbs.
 
-Beside the pass split, the autotuner picks three kernel forms per transform
+Beside the pass split, the autotuner picks four kernel forms per transform
 length and use: the alternative strided tiles (other line counts per tile),
-radix-16 instead of radix-8 inverse rows in the product row kernel, and
-first-pass packing per (tile, prime) instead of one packing workgroup shared
-by the primes of a tile. NTTAMD_ALT_TILES, NTTAMD_INV16 and
-NTTAMD_PER_PRIME_PACK force them. Each forced form must reproduce the
+radix-16 instead of radix-8 inverse rows in the product row kernel, radix-8
+row rounds with twice the threads, and first-pass packing per (tile, prime)
+instead of one packing workgroup shared by the primes of a tile.
+NTTAMD_ALT_TILES, NTTAMD_INV16, NTTAMD_ROWS8 and NTTAMD_PER_PRIME_PACK force
+them. Each forced form must reproduce the
 default form's product bit for bit under every admissible split, for single
 products, squares and batches; the default itself is checked against the
 oracle (MulPlan::mul, src/plan.rs:112-150) or, for the largest shapes,
@@ -25,7 +26,8 @@ from nttanafis_amd.workloads import batch_operands, operand
 
 pytestmark = pytest.mark.gpu
 
-FORM_ENV = ("NTTAMD_ALT_TILES", "NTTAMD_INV16", "NTTAMD_PER_PRIME_PACK", "NTTAMD_ROW_LOG", "NTTAMD_PASSES")
+FORM_ENV = ("NTTAMD_ALT_TILES", "NTTAMD_INV16", "NTTAMD_PER_PRIME_PACK", "NTTAMD_ROWS8", "NTTAMD_ROW_LOG",
+            "NTTAMD_PASSES")
 
 SHAPES = [(4000, 4000, 3), (30000, 30000, 0), (800000, 800000, 0), (2500000, 2500000, 0),
           (8000000, 8000000, 0)]
@@ -90,7 +92,7 @@ def test_forms_match_the_default(oracle, monkeypatch, la, lb, force_primes):
     primes, width, K, _ = ntt.mul_geometry(la, lb)
     say(f"{la}x{lb}: {primes} primes, 2^{K} points")
     a, b = operand(la, 5), operand(lb, 6)
-    force(NTTAMD_ALT_TILES=0, NTTAMD_INV16=0, NTTAMD_PER_PRIME_PACK=0)
+    force(NTTAMD_ALT_TILES=0, NTTAMD_INV16=0, NTTAMD_PER_PRIME_PACK=0, NTTAMD_ROWS8=0)
     ref = product(a, b)
     ref_sq = square(a)
     if K <= 21:
@@ -106,27 +108,37 @@ def test_forms_match_the_default(oracle, monkeypatch, la, lb, force_primes):
             assert form["passes"][-1] == r and "alt_tiles" in form["forms"], form
             say(f"  {passes} passes, rows 2^{r}: alternative tiles")
             assert (product(a, b) == ref).all(), f"alternative tiles, {passes} passes, rows 2^{r}"
-        force(NTTAMD_ALT_TILES=0, NTTAMD_INV16=1, NTTAMD_PER_PRIME_PACK=1)
+        force(NTTAMD_ALT_TILES=0, NTTAMD_INV16=1, NTTAMD_PER_PRIME_PACK=1, NTTAMD_ROWS8=0)
         say(f"  {passes} passes, rows 2^{r}: radix-16 inverse rows, per-prime packing")
         assert (product(a, b) == ref).all(), f"inv16 + per-prime packing, {passes} passes, rows 2^{r}"
+        force(NTTAMD_INV16=0, NTTAMD_PER_PRIME_PACK=0, NTTAMD_ROWS8=1)
+        say(f"  {passes} passes, rows 2^{r}: radix-8 rows")
+        assert (product(a, b) == ref).all(), f"radix-8 rows, {passes} passes, rows 2^{r}"
+        assert (square(a) == ref_sq).all(), f"square, radix-8 rows, {passes} passes, rows 2^{r}"
     os.environ.pop("NTTAMD_PASSES")
     os.environ.pop("NTTAMD_ROW_LOG")
-    force(NTTAMD_ALT_TILES=1, NTTAMD_INV16=1, NTTAMD_PER_PRIME_PACK=1)
+    force(NTTAMD_ALT_TILES=1, NTTAMD_INV16=1, NTTAMD_PER_PRIME_PACK=1, NTTAMD_ROWS8=0)
     say("  every form at the default split; square")
     assert (product(a, b) == ref).all(), "all forms at the default split"
     assert (square(a) == ref_sq).all(), "square with alternative tiles and per-prime packing"
+    force(NTTAMD_ROWS8=1)
+    assert (product(a, b) == ref).all(), "all forms and radix-8 rows at the default split"
+    assert (square(a) == ref_sq).all(), "square with every form and radix-8 rows"
 
 
 @pytest.mark.parametrize("la,lb", [(1024, 1024), (100, 3000), (700, 701), (2000, 2000)])
-def test_single_pass_inv16(oracle, monkeypatch, la, lb):
-    """Products of one fused pass (up to 2^12 points) in either inverse radix."""
+def test_single_pass_forms(oracle, monkeypatch, la, lb):
+    """Products of one fused pass (up to 2^12 points) in each row form."""
     monkeypatch.setenv("NTTAMD_SCHOOL", "0")
     a, b = operand(la, 7), operand(lb, 8)
     want = oracle.mul(a, b)
-    for inv in (0, 1):
-        force(NTTAMD_INV16=inv)
-        assert ntt.transform_form(ntt.mul_geometry(la, lb)[2])["forms"] == (["inv16"] if inv else [])
-        assert (product(a, b) == want).all(), f"{la}x{lb} inverse radix {16 if inv else 8}"
+    for inv, r8 in ((0, 0), (1, 0), (0, 1)):
+        force(NTTAMD_INV16=inv, NTTAMD_ROWS8=r8)
+        forms = ntt.transform_form(ntt.mul_geometry(la, lb)[2])["forms"]
+        assert forms == (["inv16"] if inv else []) + (["rows8"] if r8 else []), forms
+        assert (product(a, b) == want).all(), f"{la}x{lb} inv16={inv} rows8={r8}"
+        if la == lb:
+            assert (square(a) == oracle.sqr(a)).all(), f"{la} square inv16={inv} rows8={r8}"
 
 
 @pytest.mark.parametrize("split", [False, True])
@@ -145,9 +157,10 @@ def test_batch_forms(oracle, monkeypatch, split):
         la, lb, sq = mixed_shapes()
     a, b = batch_operands(la, lb, sq, seed=31)
     want = [expected(oracle, a[i], b[i], sq[i]) for i in range(len(la))]
-    for alt, inv, ppp in ((0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0), (0, 0, 1), (1, 1, 1)):
-        force(NTTAMD_ALT_TILES=alt, NTTAMD_INV16=inv, NTTAMD_PER_PRIME_PACK=ppp)
-        say(f"batch alt_tiles={alt} inv16={inv} per_prime_pack={ppp}")
+    for alt, inv, ppp, r8 in ((0, 0, 0, 0), (1, 0, 0, 0), (0, 1, 0, 0), (1, 1, 0, 0), (0, 0, 1, 0), (1, 1, 1, 0),
+                              (0, 0, 0, 1), (1, 0, 1, 1)):
+        force(NTTAMD_ALT_TILES=alt, NTTAMD_INV16=inv, NTTAMD_PER_PRIME_PACK=ppp, NTTAMD_ROWS8=r8)
+        say(f"batch alt_tiles={alt} inv16={inv} per_prime_pack={ppp} rows8={r8}")
         batch = plan_batch(la, lb, sq)
         out = [np.zeros(batch.output_len(i), dtype=np.uint64) for i in range(len(la))]
         assert batch.mul_host(a, b, out), ntt.last_error()
@@ -171,7 +184,7 @@ def test_tuned_forms_are_reported(oracle, n):
     form = ntt.transform_form(K, "product")
     say(f"2^{K} product: {form}")
     assert form["tuned"]
-    assert set(form["forms"]) <= {"alt_tiles", "inv16", "per_prime_pack"}
+    assert set(form["forms"]) <= {"alt_tiles", "inv16", "per_prime_pack", "rows8"}
     assert sum(form["passes"]) == K
     sq = ntt.transform_form(K, "square")
     assert sq["tuned"] and "inv16" not in sq["forms"], sq
