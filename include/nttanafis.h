@@ -383,6 +383,8 @@ bool ntt_profile_replay(const char *kind, int reps, double *avg_ms);
 #define NTT_FORM_INV16 2u            /* product rows run the inverse in radix 16 (else 8) */
 #define NTT_FORM_PER_PRIME_PACK 4u   /* first pass packs per (tile, prime), not shared across primes */
 #define NTT_FORM_ROWS8 8u            /* row kernel runs radix-8 rounds with twice the threads */
+#define NTT_FORM_STRIDED8 16u        /* strided passes run radix-8 rounds with twice the threads */
+#define NTT_FORM_PAIRED 32u          /* product rows: one thread per product group, both operands' groups */
 #define NTT_FORM_TUNED 256u          /* the autotuner has timed this length and use */
 int ntt_transform_split(uint32_t log_len, int use, uint32_t *pass_logs, uint32_t *flags);
 

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnttanafis_amd.so")
-SOURCES = ["kernels.hip", "engine.hip", "server.hip", "field.hip"]
+SOURCES = ["kernels_fused.hip", "kernels_fused_sq.hip", "kernels_strided.hip", "kernels.hip", "engine.hip", "server.hip", "field.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -171,9 +171,12 @@ struct Split {
     // alternative strided tiles (alt_strided_lines), radix-16 inverse rows
     // in the product row kernel, packing per (tile, prime) instead of
     // packing shared across the primes of a tile, and radix-8 row rounds
-    // with twice the threads (taking precedence over inv16).
-    uint8_t alt_tiles, inv16, per_prime_pack, rows8;
-    int fused_form() const { return rows8 ? 2 : (inv16 ? 1 : 0); }
+    // with twice the threads (taking precedence over inv16); strided passes
+    // of 2^5, 2^6 or 2^9 points in radix-8 rounds with twice the threads
+    // (as many rounds as radix 16 there); product rows with one thread per
+    // group of the product, taking both operands' groups (k_fused FORM 3).
+    uint8_t alt_tiles, inv16, per_prime_pack, rows8, strided8, paired;
+    int fused_form() const { return rows8 ? 2 : (paired ? 3 : (inv16 ? 1 : 0)); }
 };
 
 // Which tuned split a transform takes: a single operation's product (two
@@ -181,6 +184,7 @@ struct Split {
 enum SplitUse { kUseProduct = 0, kUseSquare = 1, kUseBatch = 2 };
 constexpr uint32_t kFormAltTiles = NTT_FORM_ALT_TILES, kFormInv16 = NTT_FORM_INV16,
                    kFormPerPrimePack = NTT_FORM_PER_PRIME_PACK, kFormRows8 = NTT_FORM_ROWS8,
+                   kFormStrided8 = NTT_FORM_STRIDED8, kFormPaired = NTT_FORM_PAIRED,
                    kFormTuned = NTT_FORM_TUNED;
 Split split_of(int K, int use);
 

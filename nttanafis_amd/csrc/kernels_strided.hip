@@ -1,0 +1,422 @@
+// Strided passes of the transforms (packing, forward, inverse) and their launchers.
+#include "passes.h"
+
+namespace nttamd {
+
+// ---------------------------------------------------------------------------
+// Strided passes: the tile is LINES consecutive lines lo0..lo0+LINES-1 of
+// first node H; element m of line lo sits at H*(T >> level_start) + m*S + lo.
+
+// R8: radix-8 rounds with one thread per radix-8 group (twice the threads of
+// the radix-16 form, half the values in flight per thread).
+template <int LOGL, int LINES, int NT, bool PACK, bool BATCH, bool R8 = false>
+__global__ __launch_bounds__(NT) void k_fwd_strided(PassArgs a) {
+    constexpr int LS = Lay<LOGL>::STRIDE;
+    __shared__ double lds[LINES * LS];
+    const uint64_t S = a.line_stride;
+    const uint64_t tiles_per_node = S / LINES;
+    Job j;
+    if (!resolve<BATCH, xcd_group(LINES)>(a, tiles_per_node << a.level_start, strided_xcd(PACK, LINES), &j))
+        return;
+    const int y = j.y;
+    const double p = a.field.p[y], pinv = a.field.pinv[y];
+    const uint64_t H = j.tile / tiles_per_node;
+    const uint64_t lo0 = (j.tile % tiles_per_node) * LINES;
+    const uint64_t span = (uint64_t)1 << (a.log_t - a.level_start);
+    const uint64_t base = H * span + lo0;
+    double* row = j.data + (uint64_t)y * a.row_stride;
+    if constexpr (PACK) {
+        pack_strided<LOGL, LINES, NT>(lds, j.src_a, base, S, fold_of(a, y), a.powers[y], p, pinv);
+    } else {
+        load_strided<LOGL, LINES, NT>(lds, row + base, S);
+    }
+    __syncthreads();
+    const Tw tw{a.fwd + (uint64_t)y * a.table_stride, H + (a.node_base << a.level_start), 0, 0, 64, p, pinv};
+    // A packing pass starts the transform (level_start 0); at node 0 its first
+    // root is 1.
+    if constexpr (R8)
+        fwd_rounds_from<LOGL, LINES, NT, 0, 3>(lds, tw, p, pinv, PACK && a.node_base == 0);
+    else
+        fwd_rounds<LOGL, LINES, NT, 0>(lds, tw, p, pinv, PACK && a.node_base == 0);
+    store_strided<LOGL, LINES, NT, false>(lds, row + base, S, 0.0, p, pinv);
+}
+
+// Levels S0..RL-1 of the transform's first round (radix 2^RL) on the points
+// of one group, in registers (fwd_round's butterflies with ROOT1: the group
+// sits at table node 0, whose level-0 root is 1, and holds packed inputs).
+// Levels below S0 were trivial: their multiplied inputs were all zero.
+template <int RL, int S0>
+__device__ __forceinline__ void first_round_regs(double* x, const Tw& tw, double p, double pinv) {
+    constexpr int RADIX = 1 << RL;
+#pragma unroll
+    for (int s = S0; s < RL; ++s) {
+        const int half = RADIX >> (s + 1);
+#pragma unroll
+        for (int blk = 0; blk < (1 << s); ++blk) {
+            const double w = tw.at(0, s, blk);
+#pragma unroll
+            for (int j = 0; j < half; ++j) {
+                const int k = blk * 2 * half + j;
+                const double u = s == RL - 1 ? reduce(x[k], p, pinv) : x[k];
+                const double t = s == 0 ? x[k + half] : mulmod(x[k + half], w, p, pinv);
+                x[k] = u + t;
+                x[k + half] = u - t;
+            }
+        }
+    }
+}
+
+// Tiles of the shared packing pass: 4096 points (its extracted parts stay in
+// registers), eight points per thread.
+template <int LOGL>
+static constexpr int pack_primes_lines() {
+    return 4096 >> LOGL;
+}
+template <int LOGL>
+static constexpr int pack_primes_threads() {
+    return 4096 / 8;
+}
+
+// First forward pass with packing shared across primes: one workgroup packs
+// one tile of LINES lines of 2^LOGL points (4096 points) for every prime of
+// the operation. Each thread extracts the parts of its 8 coefficients once
+// (elements m0 + r*L/8 of line c: exactly one radix-8 group of the first
+// three levels), then per prime combines them into residues, runs those
+// levels in registers, stores the group to LDS, runs the remaining rounds,
+// and writes the tile to that prime's row. The rows start at table node 0
+// and are not folded (single operations on power-of-two rows or cyclic
+// parts).
+//
+// Zero padding past `digits` prunes the first levels: a group whose upper
+// half is zero has a trivial level 0 (both outputs are the lower input),
+// and one whose upper three quarters are zero also a trivial level 1 (the
+// pruned first stages of src/arch/schedule/expression/stages.rs:109-163).
+// Bounds: packed inputs are below p, level 0 (root 1) gives 2p, level 1 adds
+// 15p/16 (multiplied inputs below 4p, products below 2p^2), level 2 reduces
+// the added input and ends below 9p/16 + 15p/16 = 3p/2, and every later round
+// is entered below 31p/16 (fwd_round's schedule).
+template <int LOGL, int LINES, int NT, int KIND>
+__device__ __forceinline__ void pack_primes_tile(const PassArgs& a, const Job& j, int primes, double* lds) {
+    constexpr int L = 1 << LOGL;
+    constexpr int LS = Lay<LOGL>::STRIDE;
+    constexpr int D = L / 8;
+    constexpr int P = PackParts<KIND>::value;
+    static_assert(LINES * L == 8 * NT, "one radix-8 group of the first levels per thread");
+    const uint64_t S = a.line_stride;
+    const int c = threadIdx.x % LINES, m0 = threadIdx.x / LINES;
+    const uint64_t base = j.tile * LINES;
+    const uint64_t n0 = base + c + (uint64_t)m0 * S;  // element m0 + r*D at n0 + r*D*S
+    const uint64_t step = (uint64_t)D * S;
+    const Source& src = j.src_a;
+    const uint64_t digits = src.digits;
+    const uint64_t safe = pack_safe<KIND>(src);
+    double d[8][P];
+    int fill = 0;  // leading elements below digits (the rest are zero)
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const uint64_t n = n0 + (uint64_t)r * step;
+        if (n < safe) {
+            extract_as<KIND, false>(src, n, d[r]);
+        } else {
+            extract_as<KIND, true>(src, n, d[r]);
+        }
+        fill += n < digits ? 1 : 0;
+    }
+#pragma unroll 1
+    for (int y = 0; y < primes; ++y) {
+        const double p = a.field.p[y], pinv = a.field.pinv[y];
+        const double* pw = a.powers[y];
+        double x[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) x[r] = combine_as<KIND>(src, d[r], pw, p, pinv);
+        const Tw tw{a.fwd + (uint64_t)y * a.table_stride, 0, 0, 0, 64, p, pinv};
+        if (fill <= 2) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                x[k + 2] = x[k];
+                x[k + 4] = x[k];
+                x[k + 6] = x[k];
+            }
+            first_round_regs<3, 2>(x, tw, p, pinv);
+        } else if (fill <= 4) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) x[k + 4] = x[k];
+            first_round_regs<3, 1>(x, tw, p, pinv);
+        } else {
+            first_round_regs<3, 0>(x, tw, p, pinv);
+        }
+        double* line = lds + c * LS;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) line[pidx(m0 + k * D)] = x[k];
+        __syncthreads();
+        // Radix-8 rounds: one group per thread, and eight values in flight
+        // beside the extracted parts.
+        fwd_rounds_from<LOGL, LINES, NT, 3, 3>(lds, tw, p, pinv);
+        store_strided<LOGL, LINES, NT, false>(lds, j.data + (uint64_t)y * a.row_stride + base, S, 0.0, p, pinv);
+        __syncthreads();  // the tile is read out before the next prime overwrites it
+    }
+}
+
+template <int LOGL, int LINES, int NT, int KIND>
+__global__ __launch_bounds__(NT) void k_pack_primes(PassArgs a) {
+    __shared__ double lds[LINES * Lay<LOGL>::STRIDE];
+    Job j;
+    resolve<false>(a, a.line_stride / LINES, false, &j);
+    pack_primes_tile<LOGL, LINES, NT, KIND>(a, j, a.primes, lds);
+}
+
+// Batched form: workgroup b packs tile b % tiles_per_item of the operand (or
+// block) that item b / tiles_per_item names, for every prime of its product.
+// The items are (product, block, operand) with prime 0; the host sends no
+// product whose coefficients need four parts.
+template <int LOGL, int LINES, int NT>
+__global__ __launch_bounds__(NT) void k_pack_primes_batch(PassArgs a) {
+    __shared__ double lds[LINES * Lay<LOGL>::STRIDE];
+    Job j;
+    resolve<true>(a, 0, false, &j);
+    const int primes = (int)a.batch[a.items[blockIdx.x / a.tiles_per_item].product].primes;
+    switch (pack_kind(j.src_a)) {
+        case kPackW64: pack_primes_tile<LOGL, LINES, NT, kPackW64>(a, j, primes, lds); break;
+        case kPackW64Signed: pack_primes_tile<LOGL, LINES, NT, kPackW64Signed>(a, j, primes, lds); break;
+        case kPack1: pack_primes_tile<LOGL, LINES, NT, kPack1>(a, j, primes, lds); break;
+        case kPack2: pack_primes_tile<LOGL, LINES, NT, kPack2>(a, j, primes, lds); break;
+        case kPack3: pack_primes_tile<LOGL, LINES, NT, kPack3>(a, j, primes, lds); break;
+        default: break;
+    }
+}
+
+// Inverse strided pass; CANON (level_start == 0) multiplies by T^-1 and stores
+// canonical residues in [0, p).
+template <int LOGL, int LINES, int NT, bool CANON, bool BATCH, bool R8 = false>
+__global__ __launch_bounds__(NT) void k_inv_strided(PassArgs a) {
+    constexpr int LS = Lay<LOGL>::STRIDE;
+    __shared__ double lds[LINES * LS];
+    const uint64_t S = a.line_stride;
+    const uint64_t tiles_per_node = S / LINES;
+    Job j;
+    if (!resolve<BATCH, xcd_group(LINES)>(a, tiles_per_node << a.level_start, strided_xcd(false, LINES), &j))
+        return;
+    const int y = j.y;
+    const double p = a.field.p[y], pinv = a.field.pinv[y];
+    const uint64_t H = j.tile / tiles_per_node;
+    const uint64_t lo0 = (j.tile % tiles_per_node) * LINES;
+    const uint64_t span = (uint64_t)1 << (a.log_t - a.level_start);
+    const uint64_t base = H * span + lo0;
+    double* row = j.data + (uint64_t)y * a.row_stride;
+    load_strided<LOGL, LINES, NT>(lds, row + base, S);
+    __syncthreads();
+    const Tw tw{a.inv + (uint64_t)y * a.table_stride, H + (a.node_base << a.level_start), 0, 0, 64, p, pinv};
+    // The canonical pass ends the transform (level_start 0); at node 0 its
+    // last root is 1.
+    if constexpr (R8)
+        inv_rounds8<LOGL, LINES, NT, Lay8<LOGL>::ROUNDS - 1>(lds, tw, p, pinv, CANON && a.node_base == 0);
+    else
+        inv_rounds<LOGL, LINES, NT, Lay<LOGL>::ROUNDS - 1>(lds, tw, p, pinv, CANON && a.node_base == 0);
+    store_strided<LOGL, LINES, NT, CANON>(lds, row + base, S, a.tinv[y], p, pinv);
+}
+
+// Lines per strided tile (strided_lines, device.h). Measured with a plain
+// tile copy at 2^24 points x 6 rows: 64 x 128 moves 5.6 TB/s at a first
+// pass's line stride and 6.0 TB/s at a middle pass's, against 5.1 and 5.35
+// for 32 x 128; 32 x 256 moves 5.3-5.5 TB/s against 4.6-4.8 for 16 x 256.
+
+// Threads of a strided tile: one radix-16 group per thread and round.
+template <int LOGL, int LINES>
+static constexpr int strided_threads() {
+    return LINES * (1 << LOGL) / 16 >= 512 ? 512 : 256;
+}
+
+// Threads of the radix-8 form: one radix-8 group per thread and round.
+template <int LOGL, int LINES>
+static constexpr int strided_threads8() {
+    return LINES * (1 << LOGL) / 8 >= 1024 ? 1024 : (LINES * (1 << LOGL) / 8 <= 256 ? 256 : LINES * (1 << LOGL) / 8);
+}
+// Line lengths whose radix-8 rounds are as many as the radix-16 rounds.
+__host__ __device__ constexpr bool r8_fits(int logl) { return logl == 5 || logl == 6 || logl == 9; }
+
+// In a batched launch (a.batch set) `units` counts work items (product,
+// prime); otherwise it counts primes.
+template <int LOGL, int LINES>
+static bool launch_fwd_strided_l(const PassArgs& a, int units, int operands, bool pack, hipStream_t s,
+                                 bool r8 = false) {
+    constexpr int NT = strided_threads<LOGL, LINES>();
+    if (a.line_stride < (uint64_t)LINES) return false;
+    const uint64_t nodes = (uint64_t)1 << a.level_start;
+    const uint64_t tiles = nodes * (a.line_stride / LINES);
+    PassArgs b = a;
+    b.primes = units;
+    b.tiles_per_item = tiles;
+    constexpr uint64_t G8 = 8 * (uint64_t)xcd_group(LINES);
+    dim3 grid((unsigned)tiles, (unsigned)units, (unsigned)operands);
+    if (strided_xcd(pack, LINES)) grid = dim3((unsigned)(((tiles + G8 - 1) / G8) * G8 * units), 1, (unsigned)operands);
+    if (a.batch) grid = dim3((unsigned)(tiles * units), 1);
+    if (a.batch) {
+        if (pack)
+            NTT_LAUNCH(nullptr, (k_fwd_strided<LOGL, LINES, NT, true, true>), grid, dim3(NT), s, b);
+        else
+            NTT_LAUNCH(nullptr, (k_fwd_strided<LOGL, LINES, NT, false, true>), grid, dim3(NT), s, b);
+    } else if (r8 && !pack) {
+        if constexpr (r8_fits(LOGL)) {
+            constexpr int NT8 = strided_threads8<LOGL, LINES>();
+            NTT_LAUNCH(nullptr, (k_fwd_strided<LOGL, LINES, NT8, false, false, true>), grid, dim3(NT8), s, b);
+        }
+    } else {
+        if (pack)
+            NTT_LAUNCH(nullptr, (k_fwd_strided<LOGL, LINES, NT, true, false>), grid, dim3(NT), s, b);
+        else
+            NTT_LAUNCH(nullptr, (k_fwd_strided<LOGL, LINES, NT, false, false>), grid, dim3(NT), s, b);
+    }
+    return NTT_CHECK_LAUNCH();
+}
+
+template <int LOGL>
+static bool launch_fwd_strided_t(const PassArgs& a, int units, int operands, bool pack, bool alt, bool r8,
+                                 hipStream_t s) {
+    r8 = r8 && r8_fits(LOGL);
+    return alt ? launch_fwd_strided_l<LOGL, alt_strided_lines(LOGL)>(a, units, operands, pack, s, r8)
+               : launch_fwd_strided_l<LOGL, strided_lines(LOGL)>(a, units, operands, pack, s, r8);
+}
+
+// First forward pass of a single operation with packing shared across its
+// `primes` primes: one workgroup per tile and operand (grid z as in
+// k_fwd_strided). Whole-limb and one- to three-part coefficients.
+template <int LOGL>
+static bool launch_pack_primes_t(const PassArgs& a, int primes, int operands, hipStream_t s) {
+    constexpr int LINES = pack_primes_lines<LOGL>();
+    constexpr int NT = pack_primes_threads<LOGL>();
+    static_assert(LINES * (1 << LOGL) == 8 * NT, "one radix-8 group per thread");
+    if (a.line_stride < (uint64_t)LINES || a.level_start != 0 || a.fold_terms != 1 || a.node_base != 0 || a.batch)
+        return false;
+    PassArgs b = a;
+    b.primes = primes;
+    const dim3 grid((unsigned)(a.line_stride / LINES), 1, (unsigned)operands);
+    switch (pack_kind(a.src_a)) {
+        case kPackW64: NTT_LAUNCH(nullptr, (k_pack_primes<LOGL, LINES, NT, kPackW64>), grid, dim3(NT), s, b); break;
+        case kPackW64Signed:
+            NTT_LAUNCH(nullptr, (k_pack_primes<LOGL, LINES, NT, kPackW64Signed>), grid, dim3(NT), s, b);
+            break;
+        case kPack1: NTT_LAUNCH(nullptr, (k_pack_primes<LOGL, LINES, NT, kPack1>), grid, dim3(NT), s, b); break;
+        case kPack2: NTT_LAUNCH(nullptr, (k_pack_primes<LOGL, LINES, NT, kPack2>), grid, dim3(NT), s, b); break;
+        case kPack3: NTT_LAUNCH(nullptr, (k_pack_primes<LOGL, LINES, NT, kPack3>), grid, dim3(NT), s, b); break;
+        default: return false;
+    }
+    return NTT_CHECK_LAUNCH();
+}
+
+template <int LOGL, int LINES>
+static bool launch_inv_strided_l(const PassArgs& a, int units, bool canon, hipStream_t s, bool r8 = false) {
+    constexpr int NT = strided_threads<LOGL, LINES>();
+    if (a.line_stride < (uint64_t)LINES) return false;
+    const uint64_t nodes = (uint64_t)1 << a.level_start;
+    const uint64_t tiles = nodes * (a.line_stride / LINES);
+    PassArgs b = a;
+    b.primes = units;
+    b.tiles_per_item = tiles;
+    constexpr uint64_t G8 = 8 * (uint64_t)xcd_group(LINES);
+    const unsigned blocks = a.split_blocks ? a.split_blocks : 1;
+    dim3 grid((unsigned)tiles, (unsigned)units, blocks);
+    if (strided_xcd(false, LINES)) grid = dim3((unsigned)(((tiles + G8 - 1) / G8) * G8 * units), 1, blocks);
+    if (a.batch) grid = dim3((unsigned)(tiles * units), 1);
+    if (a.batch) {
+        if (canon)
+            NTT_LAUNCH(nullptr, (k_inv_strided<LOGL, LINES, NT, true, true>), grid, dim3(NT), s, b);
+        else
+            NTT_LAUNCH(nullptr, (k_inv_strided<LOGL, LINES, NT, false, true>), grid, dim3(NT), s, b);
+    } else if (r8) {
+        if constexpr (r8_fits(LOGL)) {
+            constexpr int NT8 = strided_threads8<LOGL, LINES>();
+            if (canon)
+                NTT_LAUNCH(nullptr, (k_inv_strided<LOGL, LINES, NT8, true, false, true>), grid, dim3(NT8), s, b);
+            else
+                NTT_LAUNCH(nullptr, (k_inv_strided<LOGL, LINES, NT8, false, false, true>), grid, dim3(NT8), s, b);
+        }
+    } else {
+        if (canon)
+            NTT_LAUNCH(nullptr, (k_inv_strided<LOGL, LINES, NT, true, false>), grid, dim3(NT), s, b);
+        else
+            NTT_LAUNCH(nullptr, (k_inv_strided<LOGL, LINES, NT, false, false>), grid, dim3(NT), s, b);
+    }
+    return NTT_CHECK_LAUNCH();
+}
+
+template <int LOGL>
+static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, bool alt, bool r8, hipStream_t s) {
+    r8 = r8 && r8_fits(LOGL);
+    return alt ? launch_inv_strided_l<LOGL, alt_strided_lines(LOGL)>(a, units, canon, s, r8)
+               : launch_inv_strided_l<LOGL, strided_lines(LOGL)>(a, units, canon, s, r8);
+}
+
+bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, bool pack, bool alt, bool r8,
+                        hipStream_t s) {
+    switch (logl) {
+        case 5: return launch_fwd_strided_t<5>(a, primes, operands, pack, alt, r8, s);
+        case 6: return launch_fwd_strided_t<6>(a, primes, operands, pack, alt, r8, s);
+        case 7: return launch_fwd_strided_t<7>(a, primes, operands, pack, alt, r8, s);
+        case 8: return launch_fwd_strided_t<8>(a, primes, operands, pack, alt, r8, s);
+        case 9: return launch_fwd_strided_t<9>(a, primes, operands, pack, alt, r8, s);
+        case 10: return launch_fwd_strided_t<10>(a, primes, operands, pack, alt, r8, s);
+        case 11: return launch_fwd_strided_t<11>(a, primes, operands, pack, alt, r8, s);
+        default: return false;
+    }
+}
+
+// False when the pass cannot take the shared form (the caller then packs
+// per prime): a batch, a folded or tail part, four-part coefficients, a
+// source kind other than the second operand's, or lines of other than 2^6 to
+// 2^9 points (4096-point tiles).
+bool launch_pack_primes(int logl, const PassArgs& a, int primes, int operands, hipStream_t s) {
+    if (pack_kind(a.src_a) != pack_kind(a.src_b) || pack_kind(a.src_a) == kPack4) return false;
+    switch (logl) {
+        case 5: return launch_pack_primes_t<5>(a, primes, operands, s);
+        case 6: return launch_pack_primes_t<6>(a, primes, operands, s);
+        case 7: return launch_pack_primes_t<7>(a, primes, operands, s);
+        case 8: return launch_pack_primes_t<8>(a, primes, operands, s);
+        case 9: return launch_pack_primes_t<9>(a, primes, operands, s);
+        default: return false;
+    }
+}
+
+// Batched shared packing: `items` (product, block, operand) units, every
+// product of kind below kPack4 (pack_primes_batch_fits checks the shape).
+template <int LOGL>
+static bool launch_pack_primes_batch_t(const PassArgs& a, int items, hipStream_t s) {
+    constexpr int LINES = pack_primes_lines<LOGL>();
+    constexpr int NT = pack_primes_threads<LOGL>();
+    if (a.line_stride < (uint64_t)LINES || a.level_start != 0 || !a.batch) return false;
+    PassArgs b = a;
+    b.tiles_per_item = a.line_stride / LINES;
+    const dim3 grid((unsigned)(b.tiles_per_item * (uint64_t)items), 1);
+    NTT_LAUNCH(nullptr, (k_pack_primes_batch<LOGL, LINES, NT>), grid, dim3(NT), s, b);
+    return NTT_CHECK_LAUNCH();
+}
+
+bool pack_primes_batch_fits(int logl, uint64_t line_stride) {
+    return logl >= 5 && logl <= 9 && line_stride >= (uint64_t)(4096 >> logl);
+}
+
+bool launch_pack_primes_batch(int logl, const PassArgs& a, int items, hipStream_t s) {
+    switch (logl) {
+        case 5: return launch_pack_primes_batch_t<5>(a, items, s);
+        case 6: return launch_pack_primes_batch_t<6>(a, items, s);
+        case 7: return launch_pack_primes_batch_t<7>(a, items, s);
+        case 8: return launch_pack_primes_batch_t<8>(a, items, s);
+        case 9: return launch_pack_primes_batch_t<9>(a, items, s);
+        default: return false;
+    }
+}
+
+bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, bool alt, bool r8, hipStream_t s) {
+    switch (logl) {
+        case 5: return launch_inv_strided_t<5>(a, primes, canon, alt, r8, s);
+        case 6: return launch_inv_strided_t<6>(a, primes, canon, alt, r8, s);
+        case 7: return launch_inv_strided_t<7>(a, primes, canon, alt, r8, s);
+        case 8: return launch_inv_strided_t<8>(a, primes, canon, alt, r8, s);
+        case 9: return launch_inv_strided_t<9>(a, primes, canon, alt, r8, s);
+        case 10: return launch_inv_strided_t<10>(a, primes, canon, alt, r8, s);
+        case 11: return launch_inv_strided_t<11>(a, primes, canon, alt, r8, s);
+        default: return false;
+    }
+}
+
+
+}  // namespace nttamd

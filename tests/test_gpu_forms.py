@@ -1,12 +1,15 @@
 """Every kernel form the autotuner may choose gives the same limbs.
 
-Beside the pass split, the autotuner picks four kernel forms per transform
+Beside the pass split, the autotuner picks six kernel forms per transform
 length and use: the alternative strided tiles (other line counts per tile),
 radix-16 instead of radix-8 inverse rows in the product row kernel, radix-8
-row rounds with twice the threads, and first-pass packing per (tile, prime)
-instead of one packing workgroup shared by the primes of a tile.
-NTTAMD_ALT_TILES, NTTAMD_INV16, NTTAMD_ROWS8 and NTTAMD_PER_PRIME_PACK force
-them. Each forced form must reproduce the
+row rounds with twice the threads, first-pass packing per (tile, prime)
+instead of one packing workgroup shared by the primes of a tile, radix-8
+strided passes with twice the threads, and paired product rows (one thread
+per product group, both operands' groups, the pointwise product between the
+last forward and first inverse round in registers). NTTAMD_ALT_TILES,
+NTTAMD_INV16, NTTAMD_ROWS8, NTTAMD_PER_PRIME_PACK, NTTAMD_STRIDED8 and
+NTTAMD_PAIRED force them. Each forced form must reproduce the
 default form's product bit for bit under every admissible split, for single
 products, squares and batches; the default itself is checked against the
 oracle (MulPlan::mul, src/plan.rs:112-150) or, for the largest shapes,
@@ -27,7 +30,7 @@ from nttanafis_amd.workloads import batch_operands, operand
 pytestmark = pytest.mark.gpu
 
 FORM_ENV = ("NTTAMD_ALT_TILES", "NTTAMD_INV16", "NTTAMD_PER_PRIME_PACK", "NTTAMD_ROWS8", "NTTAMD_ROW_LOG",
-            "NTTAMD_PASSES")
+            "NTTAMD_PASSES", "NTTAMD_STRIDED8", "NTTAMD_PAIRED")
 
 SHAPES = [(4000, 4000, 3), (30000, 30000, 0), (800000, 800000, 0), (2500000, 2500000, 0),
           (8000000, 8000000, 0)]
@@ -92,7 +95,8 @@ def test_forms_match_the_default(oracle, monkeypatch, la, lb, force_primes):
     primes, width, K, _ = ntt.mul_geometry(la, lb)
     say(f"{la}x{lb}: {primes} primes, 2^{K} points")
     a, b = operand(la, 5), operand(lb, 6)
-    force(NTTAMD_ALT_TILES=0, NTTAMD_INV16=0, NTTAMD_PER_PRIME_PACK=0, NTTAMD_ROWS8=0)
+    force(NTTAMD_ALT_TILES=0, NTTAMD_INV16=0, NTTAMD_PER_PRIME_PACK=0, NTTAMD_ROWS8=0, NTTAMD_STRIDED8=0,
+          NTTAMD_PAIRED=0)
     ref = product(a, b)
     ref_sq = square(a)
     if K <= 21:
@@ -115,6 +119,11 @@ def test_forms_match_the_default(oracle, monkeypatch, la, lb, force_primes):
         say(f"  {passes} passes, rows 2^{r}: radix-8 rows")
         assert (product(a, b) == ref).all(), f"radix-8 rows, {passes} passes, rows 2^{r}"
         assert (square(a) == ref_sq).all(), f"square, radix-8 rows, {passes} passes, rows 2^{r}"
+        force(NTTAMD_ROWS8=0, NTTAMD_PAIRED=1, NTTAMD_STRIDED8=1)
+        say(f"  {passes} passes, rows 2^{r}: paired rows, radix-8 strided passes")
+        assert (product(a, b) == ref).all(), f"paired rows + radix-8 strided, {passes} passes, rows 2^{r}"
+        assert (square(a) == ref_sq).all(), f"square, radix-8 strided, {passes} passes, rows 2^{r}"
+        force(NTTAMD_PAIRED=0, NTTAMD_STRIDED8=0)
     os.environ.pop("NTTAMD_PASSES")
     os.environ.pop("NTTAMD_ROW_LOG")
     force(NTTAMD_ALT_TILES=1, NTTAMD_INV16=1, NTTAMD_PER_PRIME_PACK=1, NTTAMD_ROWS8=0)
@@ -124,6 +133,9 @@ def test_forms_match_the_default(oracle, monkeypatch, la, lb, force_primes):
     force(NTTAMD_ROWS8=1)
     assert (product(a, b) == ref).all(), "all forms and radix-8 rows at the default split"
     assert (square(a) == ref_sq).all(), "square with every form and radix-8 rows"
+    force(NTTAMD_ROWS8=0, NTTAMD_PAIRED=1, NTTAMD_STRIDED8=1)
+    assert (product(a, b) == ref).all(), "all forms with paired rows and radix-8 strided passes"
+    assert (square(a) == ref_sq).all(), "square with every form and radix-8 strided passes"
 
 
 @pytest.mark.parametrize("la,lb", [(1024, 1024), (100, 3000), (700, 701), (2000, 2000)])
@@ -132,10 +144,10 @@ def test_single_pass_forms(oracle, monkeypatch, la, lb):
     monkeypatch.setenv("NTTAMD_SCHOOL", "0")
     a, b = operand(la, 7), operand(lb, 8)
     want = oracle.mul(a, b)
-    for inv, r8 in ((0, 0), (1, 0), (0, 1)):
-        force(NTTAMD_INV16=inv, NTTAMD_ROWS8=r8)
+    for inv, r8, pr in ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)):
+        force(NTTAMD_INV16=inv, NTTAMD_ROWS8=r8, NTTAMD_PAIRED=pr)
         forms = ntt.transform_form(ntt.mul_geometry(la, lb)[2])["forms"]
-        assert forms == (["inv16"] if inv else []) + (["rows8"] if r8 else []), forms
+        assert forms == (["inv16"] if inv else []) + (["rows8"] if r8 else []) + (["paired"] if pr else []), forms
         assert (product(a, b) == want).all(), f"{la}x{lb} inv16={inv} rows8={r8}"
         if la == lb:
             assert (square(a) == oracle.sqr(a)).all(), f"{la} square inv16={inv} rows8={r8}"
@@ -157,9 +169,9 @@ def test_batch_forms(oracle, monkeypatch, split):
         la, lb, sq = mixed_shapes()
     a, b = batch_operands(la, lb, sq, seed=31)
     want = [expected(oracle, a[i], b[i], sq[i]) for i in range(len(la))]
-    for alt, inv, ppp, r8 in ((0, 0, 0, 0), (1, 0, 0, 0), (0, 1, 0, 0), (1, 1, 0, 0), (0, 0, 1, 0), (1, 1, 1, 0),
-                              (0, 0, 0, 1), (1, 0, 1, 1)):
-        force(NTTAMD_ALT_TILES=alt, NTTAMD_INV16=inv, NTTAMD_PER_PRIME_PACK=ppp, NTTAMD_ROWS8=r8)
+    for alt, inv, ppp, r8, pr in ((0, 0, 0, 0, 0), (1, 0, 0, 0, 0), (0, 1, 0, 0, 0), (1, 1, 0, 0, 0), (0, 0, 1, 0, 0),
+                                  (1, 1, 1, 0, 0), (0, 0, 0, 1, 0), (1, 0, 1, 1, 0), (0, 0, 0, 0, 1), (1, 0, 1, 0, 1)):
+        force(NTTAMD_ALT_TILES=alt, NTTAMD_INV16=inv, NTTAMD_PER_PRIME_PACK=ppp, NTTAMD_ROWS8=r8, NTTAMD_PAIRED=pr)
         say(f"batch alt_tiles={alt} inv16={inv} per_prime_pack={ppp} rows8={r8}")
         batch = plan_batch(la, lb, sq)
         out = [np.zeros(batch.output_len(i), dtype=np.uint64) for i in range(len(la))]
@@ -184,7 +196,7 @@ def test_tuned_forms_are_reported(oracle, n):
     form = ntt.transform_form(K, "product")
     say(f"2^{K} product: {form}")
     assert form["tuned"]
-    assert set(form["forms"]) <= {"alt_tiles", "inv16", "per_prime_pack", "rows8"}
+    assert set(form["forms"]) <= {"alt_tiles", "inv16", "per_prime_pack", "rows8", "strided8", "paired"}
     assert sum(form["passes"]) == K
     sq = ntt.transform_form(K, "square")
     assert sq["tuned"] and "inv16" not in sq["forms"], sq
