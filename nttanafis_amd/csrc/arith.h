@@ -40,6 +40,10 @@ __host__ __device__ __forceinline__ double part_to_double(uint64_t v) {
     return __builtin_bit_cast(double, v | 0x4330000000000000ull) - 4503599627370496.0;
 }
 
+// Bits of the top part at most three: its product with a centered power is
+// exact in binary64 (the SHORT_HIGH form, combine_parts below).
+__host__ __device__ __forceinline__ bool short_high_part(int high_bits) { return high_bits <= 3; }
+
 // One limb per coefficient (width 64): at most two parts, both in the limb;
 // a signed coefficient is two's complement in its value_bits.
 // With CLAMP false the caller guarantees that coefficient n and every limb
@@ -59,7 +63,7 @@ __host__ __device__ __forceinline__ double pack_w64(const Source& s, uint64_t n,
         const double d1 = SIGNED ? (double)((int64_t)(masked ^ (1ull << (high_bits - 1))) -
                                             (int64_t)(1ull << (high_bits - 1)))
                                  : (double)(uint32_t)masked;
-        r = reduce(d0 + mulmod(d1, pw[1], p, pinv), p, pinv);
+        r = reduce(d0 + (short_high_part(high_bits) ? d1 * pw[1] : mulmod(d1, pw[1], p, pinv)), p, pinv);
     } else {
         const uint64_t masked = limb & ((1ull << vb) - 1);
         r = SIGNED ? (double)((int64_t)(masked ^ (1ull << (vb - 1))) - (int64_t)(1ull << (vb - 1)))
@@ -98,13 +102,18 @@ __host__ __device__ __forceinline__ void parts_from_window(const uint64_t* w, ui
 }
 
 // Residue of a coefficient from its parts: part_0 + sum_k part_k (2^(49k) mod p),
-// reduced once.
+// reduced once. short_high (the SHORT_HIGH form of
+// src/arch/schedule/expression/packing.rs:12, 99-101): the top part has at
+// most three bits, so its product with the centered power (below 8 * p/2 =
+// 4p) is exact in binary64 and needs no residue product; the sum then stays
+// below p + 2 * 15p/16 + 4p < 8p for four parts, inside reduce's range.
 template <int PARTS>
-__host__ __device__ __forceinline__ double combine_parts(const double* d, const double* pw, double p,
-                                                         double pinv) {
+__host__ __device__ __forceinline__ double combine_parts(const double* d, const double* pw, double p, double pinv,
+                                                         bool short_high = false) {
     double sum = d[0];
 #pragma unroll
-    for (int k = 1; k < PARTS; ++k) sum = sum + mulmod(d[k], pw[k], p, pinv);
+    for (int k = 1; k < PARTS; ++k)
+        sum = sum + ((k == PARTS - 1 && short_high) ? d[k] * pw[k] : mulmod(d[k], pw[k], p, pinv));
     return PARTS > 1 ? reduce(sum, p, pinv) : sum;
 }
 
@@ -113,7 +122,7 @@ __host__ __device__ __forceinline__ double residue_from_window(const uint64_t* w
                                                                const double* pw, double p, double pinv) {
     double d[PARTS];
     parts_from_window<PARTS>(w, off, high_bits, d);
-    return combine_parts<PARTS>(d, pw, p, pinv);
+    return combine_parts<PARTS>(d, pw, p, pinv, short_high_part(high_bits));
 }
 
 // Parts of coefficient n of a general-width source (zero at or beyond
@@ -226,9 +235,13 @@ template <int KIND>
 __host__ __device__ __forceinline__ double combine_as(const Source& s, const double* d, const double* pw, double p,
                                                       double pinv) {
     if constexpr (KIND == kPackW64 || KIND == kPackW64Signed) {
-        return s.value_bits > (uint32_t)kPartBits ? reduce(d[0] + mulmod(d[1], pw[1], p, pinv), p, pinv) : d[0];
+        if (s.value_bits <= (uint32_t)kPartBits) return d[0];
+        const double high = short_high_part((int)s.value_bits - kPartBits) ? d[1] * pw[1] : mulmod(d[1], pw[1], p, pinv);
+        return reduce(d[0] + high, p, pinv);
     } else {
-        return combine_parts<KIND - kPack1 + 1>(d, pw, p, pinv);
+        constexpr int PARTS = KIND - kPack1 + 1;
+        return combine_parts<PARTS>(d, pw, p, pinv,
+                                    PARTS > 1 && short_high_part((int)s.value_bits - kPartBits * (PARTS - 1)));
     }
 }
 

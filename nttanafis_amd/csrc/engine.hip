@@ -2840,7 +2840,8 @@ int ntt_selftest_host(uint64_t seed) {
 #define NTT_PARTS_CASE(P)                                                  \
     case kPack1 + P - 1:                                                   \
         extract_parts<P>(s, n, d);                                         \
-        split = combine_parts<P>(d, c.powers[y], pf, pinv);                \
+        split = combine_parts<P>(d, c.powers[y], pf, pinv,                 \
+                                 P > 1 && short_high_part((int)s.value_bits - kPartBits * (P - 1))); \
         break;
                             NTT_PARTS_CASE(1)
                             NTT_PARTS_CASE(2)

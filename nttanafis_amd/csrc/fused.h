@@ -101,10 +101,31 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     // The lines of both tiles go through one set of rounds; line c of either
     // tile is node H0 + (c mod LINES).
     if constexpr ((SQUARE && FORM != 2) || FORM == 3) {
-        fwd_rounds<LOGL, TILES * LINES, NT, 0, ROUNDS - 1>(lds, twF, p, pinv, root1);
-        fused_middle<LOGL, LINES, NT, SQUARE>(lds, twF, twI, p, pinv);
-        __syncthreads();
-        inv_rounds<LOGL, LINES, NT, ROUNDS - 2>(lds, twI, p, pinv, root1);
+        constexpr int L1 = Lay<LOGL>::FIRST;  // first level of round 1
+        constexpr int GPL = L / 16;
+        if constexpr (ROUNDS == 3 && (NT >= TILES * LINES * GPL || NT % (GPL * LINES) == 0)) {
+            // Round 1 reads its roots from the table at nodes spread over
+            // the table (rows of long transforms miss the L2): each thread
+            // loads them for its group ahead, the forward ones before round
+            // 0 and the inverse ones before the middle rounds.
+            const int g = threadIdx.x, c1 = g / GPL, b1 = (g - c1 * GPL) / (L >> (L1 + 4));
+            double w1[15], v1[15];
+            group_roots16(w1, twF, c1, L1, b1);
+            fwd_rounds<LOGL, TILES * LINES, NT, 0, 1>(lds, twF, p, pinv, root1);
+            round16_w<LOGL, TILES * LINES, NT, L1, LINES, false>(lds, w1, p, pinv);
+            __syncthreads();
+            group_roots16(v1, twI, c1, L1, b1);
+            fused_middle<LOGL, LINES, NT, SQUARE>(lds, twF, twI, p, pinv);
+            __syncthreads();
+            round16_w<LOGL, LINES, NT, L1, LINES, true>(lds, v1, p, pinv);
+            __syncthreads();
+            inv_rounds<LOGL, LINES, NT, 0>(lds, twI, p, pinv, root1);
+        } else {
+            fwd_rounds<LOGL, TILES * LINES, NT, 0, ROUNDS - 1>(lds, twF, p, pinv, root1);
+            fused_middle<LOGL, LINES, NT, SQUARE>(lds, twF, twI, p, pinv);
+            __syncthreads();
+            inv_rounds<LOGL, LINES, NT, ROUNDS - 2>(lds, twI, p, pinv, root1);
+        }
     } else {
         if constexpr (FORM == 2)
             fwd_rounds_from<LOGL, TILES * LINES, NT, 0, 3>(lds, twF, p, pinv);

@@ -202,6 +202,85 @@ __device__ __forceinline__ void inv_group(double* x, const Tw& tw, int c, int l0
     }
 }
 
+// Roots of one radix-16 group loaded ahead of its round (so that their
+// latency overlaps the work before it): w[(1 << s) - 1 + blk] is the root of
+// node (b << s) + blk at local level l0 + s of line c.
+__device__ __forceinline__ void group_roots16(double* w, const Tw& tw, int c, int l0, int b) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int blk = 0; blk < (1 << s); ++blk) w[(1 << s) - 1 + blk] = tw.at(c, l0 + s, (b << s) + blk);
+}
+
+// fwd_group / inv_group of a radix-16 group whose roots were loaded ahead.
+__device__ __forceinline__ void fwd_group16_w(double* x, const double* w, double p, double pinv) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int half = 16 >> (s + 1);
+#pragma unroll
+        for (int blk = 0; blk < (1 << s); ++blk) {
+#pragma unroll
+            for (int j = 0; j < half; ++j) {
+                const int k = blk * 2 * half + j;
+                const double u = s == 3 ? reduce(x[k], p, pinv) : x[k];
+                const double t = mulmod(x[k + half], w[(1 << s) - 1 + blk], p, pinv);
+                x[k] = u + t;
+                x[k + half] = u - t;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void inv_group16_w(double* x, const double* w, double p, double pinv) {
+#pragma unroll
+    for (int s = 3; s >= 0; --s) {
+        const int half = 16 >> (s + 1);
+#pragma unroll
+        for (int blk = 0; blk < (1 << s); ++blk) {
+#pragma unroll
+            for (int j = 0; j < half; ++j) {
+                const int k = blk * 2 * half + j;
+                const double u = x[k], v = x[k + half];
+                const double sum = u + v;
+                x[k] = (s & 1) ? sum : reduce(sum, p, pinv);
+                x[k + half] = mulmod(u - v, w[(1 << s) - 1 + blk], p, pinv);
+            }
+        }
+    }
+}
+
+// A radix-16 round (local levels L0..L0+3, not the transform's first or
+// last level) whose roots the thread loaded ahead for its first group; every
+// further group the thread takes (g + NT, g + 2 NT, ...) must sit at the same
+// nodes, which holds when NT is a multiple of GPL * TILE_LINES (lines of one
+// tile of rows share nodes with the same lines of the other tile).
+template <int LOGL, int LINES, int NT, int L0, int TILE_LINES, bool INV>
+__device__ __forceinline__ void round16_w(double* lds, const double* w, double p, double pinv) {
+    constexpr int L = 1 << LOGL;
+    constexpr int D = L >> (L0 + 4);
+    constexpr int GPL = L / 16;
+    constexpr int GROUPS = LINES * GPL;
+    constexpr int LS = Lay<LOGL>::STRIDE;
+    static_assert(NT >= GROUPS || NT % (GPL * TILE_LINES) == 0, "later groups of a thread sit at other nodes");
+    for (int g = threadIdx.x; g < GROUPS; g += NT) {
+        const int c = g / GPL;
+        const int gl = g - c * GPL;
+        const int b = gl / D;
+        const int i = gl - b * D;
+        double* line = lds + c * LS;
+        const int base = b * (L >> L0) + i;
+        double x[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) x[k] = line[pidx(base + k * D)];
+        if constexpr (INV)
+            inv_group16_w(x, w, p, pinv);
+        else
+            fwd_group16_w(x, w, p, pinv);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) line[pidx(base + k * D)] = x[k];
+    }
+}
+
 template <int LOGL, int LINES, int NT, int L0, int RLOG, bool ROOT1 = false>
 __device__ __forceinline__ void inv_round(double* lds, Tw tw, double p, double pinv) {
     constexpr int L = 1 << LOGL;

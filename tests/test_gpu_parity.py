@@ -204,8 +204,13 @@ def test_conv32s_2p22_golden(digests):
 
 
 @pytest.mark.parametrize("bits,signed", [(1, False), (16, False), (32, True), (49, False), (50, True),
-                                         (63, True), (64, False)])
+                                         (51, False), (52, False), (52, True), (53, False), (63, True),
+                                         (64, False)])
 def test_convolutions_match_integers(bits, signed):
+    """Convolutions of 1- to 64-bit coefficients (ConvPlan::convolve,
+    src/convolve.rs:75-203) against exact integers or the oracle; 50 to 52
+    bits put one to three bits in the top 49-bit part, the SHORT_HIGH
+    packing form (src/arch/schedule/expression/packing.rs:12, 99-101)."""
     rng = np.random.default_rng(bits)
     for la, lb in [(3, 2), (24, 30), (25, 25), (200, 77), (3000, 2900)]:
         say(f"conv bits {bits} signed {signed} {la}x{lb}")

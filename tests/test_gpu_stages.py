@@ -128,3 +128,30 @@ def test_reconstruction_of_exact_rows(oracle, la, lb):
                            out.ctypes.data_as(_ffi.P64))
     assert ok, ntt.last_error()
     assert (out == oracle.mul(a, b)).all()
+
+
+def test_device_root_tables_are_the_bit_reversed_powers(oracle):
+    """ntt_debug_tables: the device root tables (k_tables builds W[j] as a
+    product of level roots, the product form of generate_tables,
+    src/transform/twiddle.rs:116-156) equal the oracle's bit-reversed powers
+    W[j] = w^rev(j) for every catalog prime, centered, and W^-1[j] W[j] = 1;
+    the first 64 entries of each are also checked against Python's pow."""
+    n = 4096
+    lib = ntt.load()
+    fwd = np.zeros(8 * n, dtype=np.float64)
+    inv = np.zeros(8 * n, dtype=np.float64)
+    pd = ctypes.POINTER(ctypes.c_double)
+    assert lib.ntt_debug_tables(n, fwd.ctypes.data_as(pd), inv.ctypes.data_as(pd)), ntt.last_error()
+    fwd, inv = fwd.reshape(8, n), inv.reshape(8, n)
+    want_f = oracle.root_table(8, n, False)
+    want_i = oracle.root_table(8, n, True)
+    for y in range(8):
+        p = oracle.prime(y)
+        assert (np.abs(fwd[y]) <= p / 2).all() and (np.abs(inv[y]) <= p / 2).all()
+        assert all(int(a) % p == int(b) % p for a, b in zip(fwd[y], want_f[y])), f"prime {y}: forward table"
+        assert all(int(a) % p == int(b) % p for a, b in zip(inv[y], want_i[y])), f"prime {y}: inverse table"
+        omega = pow(oracle.generator(y), (p - 1) // (2 * n), p)
+        for j in range(64):
+            w = pow(omega, int(format(j, "012b")[::-1], 2), p)
+            assert int(fwd[y][j]) % p == w
+            assert int(inv[y][j]) * w % p == 1
