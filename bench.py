@@ -135,14 +135,16 @@ def step_bytes(name: str, primes: int, log_len: int, tail: int, la: int, lb: int
     return total
 
 
-def committed_counters(workload: str) -> dict:
-    """The committed rocprofv3 counter summary of a workload
-    (profiles/<workload>_counters.txt): `VALU <kind> <instructions>` and
-    `TRAFFIC <kind> <bytes>` lines (per launch, FETCH_SIZE and WRITE_SIZE each
-    scaled by what the counter reads for the plain copy kernel), and the
-    `SPLIT <log_len> <pass levels>` the profiled runs used."""
-    path = os.path.join(ROOT, "profiles", f"{workload}_counters.txt")
-    out = {"valu": {}, "traffic": {}, "split": None}
+def committed_profile(workload: str) -> dict:
+    """The committed rocprofv3 summary of a workload
+    (profiles/<workload>_kernels.txt, written by --profile-summary): the
+    `SPLIT <log_len> <pass levels> <forms>` the profiled runs used, and per
+    kernel kind `KERNEL <kind> <launches per step> <average us> <algorithmic
+    bytes per launch>` (kernel trace), `TRAFFIC <kind> <bytes>` (FETCH_SIZE
+    and WRITE_SIZE per dispatch, each scaled by what it reads for the plain
+    copy kernel of the same run) and `VALU <kind> <wave instructions>`."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_kernels.txt")
+    out = {"valu": {}, "traffic": {}, "kernels": {}, "split": None}
     if os.path.exists(path):
         for line in open(path):
             parts = line.split()
@@ -150,9 +152,134 @@ def committed_counters(workload: str) -> dict:
                 out["valu"][parts[1]] = int(float(parts[2]))
             elif len(parts) == 3 and parts[0] == "TRAFFIC":
                 out["traffic"][parts[1]] = int(float(parts[2]))
-            elif len(parts) == 3 and parts[0] == "SPLIT":
-                out["split"] = [int(parts[1]), [int(x) for x in parts[2].split(",")]]
+            elif len(parts) == 5 and parts[0] == "KERNEL":
+                out["kernels"][parts[1]] = {"launches_per_step": int(parts[2]), "us": float(parts[3]),
+                                            "bytes": int(float(parts[4]))}
+            elif len(parts) == 4 and parts[0] == "SPLIT":
+                forms = [] if parts[3] == "-" else parts[3].split(",")
+                out["split"] = [int(parts[1]), [int(x) for x in parts[2].split(",")], forms]
     return out
+
+
+# Kernel kinds of the engine's launches, from the kernel names rocprofv3
+# records (template arguments: k_fwd_strided<LOGL, LINES, NT, PACK, ...>,
+# k_inv_strided<LOGL, LINES, NT, CANON, ...>, k_fused<LOGL, LINES, NT, PACK, ...>).
+def kernel_kind(name: str):
+    import re
+    m = re.search(r"nttamd::(k_[a-z_]+)(?:<([^>]*)>)?", name)
+    if not m:
+        return None
+    k, args = m.group(1), [x.strip() for x in (m.group(2) or "").split(",")]
+    if k == "k_pack_primes":
+        return "fwd_pack_strided"
+    if k == "k_fwd_strided":
+        return "fwd_pack_strided" if args[3] == "true" else "fwd_strided"
+    if k == "k_inv_strided":
+        return "inv_strided_canon" if args[3] == "true" else "inv_strided"
+    if k == "k_fused":
+        return "fused_single" if args[3] == "true" else "fused_rows"
+    return {"k_crt_product": "crt_product", "k_carry_apply": "carry_apply", "k_carry_scan": "carry_scan",
+            "k_crt_disjoint": "crt_disjoint", "k_recombine": "recombine", "k_school": "school",
+            "k_copy": "copy"}.get(k)
+
+
+def profile_summary(d: str, workload_name: str, steps: int) -> str:
+    """Text summary of a profiled run directory: <d>/trace.json (the bench
+    line of the kernel-trace run), <d>/trace/run_kernel_trace.csv, and
+    <d>/pmc_<counter>/run_counter_collection.csv of the counter runs (each
+    its own process with the same split forced). Per kernel kind it averages
+    the timed steps' dispatches: the last (5 + 1) * steps * launches-per-step
+    of each kernel (five timed runs and the events run of gpu_workload; the
+    profiled runs replay nothing), and scales FETCH_SIZE and WRITE_SIZE by
+    what they read for the plain copy kernel the counter runs end with."""
+    import csv
+    import glob
+    bench_line = json.loads([x for x in open(os.path.join(d, "trace.json")) if x.startswith("{")][-1])
+    lps = {k: r["launches_per_step"] for k, r in bench_line["kernels"].items()}
+    cfg = bench_line["config"]
+
+    def timed(rows):
+        # The timed steps are the last dispatches of the run (after the
+        # prepare-time autotuner's trials; the copy kernel of a counter run
+        # excluded): (5 + 1) * steps * the launches of one step.
+        rows = [r for r in rows if kernel_kind(r["Kernel_Name"]) in lps]
+        tail = rows[-6 * steps * sum(lps.values()):]
+        out = {}
+        for r in tail:
+            out.setdefault(kernel_kind(r["Kernel_Name"]), []).append(r)
+        return out
+
+    trace = list(csv.DictReader(open(os.path.join(d, "trace", "run_kernel_trace.csv"))))
+    tk = timed(trace)
+    avg = {k: sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in v) / len(v) / 1000.0
+           for k, v in tk.items()}
+    counters = {}
+    copy_scale = {}
+    for path in sorted(glob.glob(os.path.join(d, "pmc_*", "run_counter_collection.csv"))):
+        rows = list(csv.DictReader(open(path)))
+        per = {}
+        for r in rows:
+            per.setdefault((r["Kernel_Name"], int(r["Dispatch_Id"])), {}).setdefault(r["Counter_Name"], 0.0)
+            per[(r["Kernel_Name"], int(r["Dispatch_Id"]))][r["Counter_Name"]] += float(r["Counter_Value"])
+        flat = [dict(Kernel_Name=name, Dispatch_Id=did, **vals) for (name, did), vals in sorted(per.items(),
+                key=lambda kv: kv[0][1])]
+        for kind, lst in timed(flat).items():
+            for cname in lst[0]:
+                if cname in ("Kernel_Name", "Dispatch_Id"):
+                    continue
+                counters.setdefault(kind, {})[cname] = sum(x[cname] for x in lst) / len(lst)
+        for x in flat:
+            if kernel_kind(x["Kernel_Name"]) == "copy":
+                for cname in ("FETCH_SIZE", "WRITE_SIZE"):
+                    if cname in x:
+                        copy_scale[cname] = x[cname]
+    bytes_of = {k: r.get("bytes_per_launch") for k, r in bench_line["kernels"].items()}
+    copy_bytes = 268435456
+    fscale = copy_bytes / (copy_scale["FETCH_SIZE"] * 1024) if copy_scale.get("FETCH_SIZE") else None
+    wscale = copy_bytes / (copy_scale["WRITE_SIZE"] * 1024) if copy_scale.get("WRITE_SIZE") else None
+    forms = ",".join(bench_line.get("config", {}).get("transform_forms") or []) or "-"
+    split = cfg.get("transform_split")
+    lines = [f"# {workload_name}: {cfg}",
+             f"# ms per step (this run, median of five timed runs): {bench_line['ms_per_step']}",
+             "# kind, dispatches averaged, launches per step, trace us, algorithmic bytes per launch, GB/s, "
+             "fraction of 8 TB/s; traffic (scaled counters) against the algorithmic bytes; VALU wave "
+             "instructions per dispatch, issue fraction at 2.4 GHz (1024 SIMDs, 4 cycles each), clock",
+             f"#   counter scaling from the plain copy of {copy_bytes} bytes: FETCH_SIZE x{fscale:.3f}, "
+             f"WRITE_SIZE x{wscale:.3f}" if fscale and wscale else "#   no copy-kernel calibration in this run"]
+    if split:
+        lines.append(f"SPLIT {bench_line['config']['cyclic_len'].bit_length() - 1} {','.join(map(str, split))} "
+                     f"{forms}")
+    pass_bytes = pass_us = 0.0
+    for kind in sorted(avg, key=lambda k: -avg[k] * lps[k]):
+        us, b = avg[kind], bytes_of.get(kind)
+        c = counters.get(kind, {})
+        row = f"{kind:18s} n {len(tk[kind]):5d} lps {lps[kind]} us {us:9.2f}"
+        if b:
+            row += f" bytes {b:11d} GBps {b / (us * 1e-6) / 1e9:7.1f} frac {b / (us * 1e-6) / 8e12:.3f}"
+            if kind in NTT_KINDS:
+                pass_bytes += b * lps[kind]
+                pass_us += us * lps[kind]
+        traffic = None
+        if fscale and wscale and "FETCH_SIZE" in c and "WRITE_SIZE" in c:
+            traffic = c["FETCH_SIZE"] * 1024 * fscale + c["WRITE_SIZE"] * 1024 * wscale
+            row += f" traffic_MB {traffic / 1e6:8.1f}" + (f" x{traffic / b:.2f}" if b else "")
+        if "SQ_INSTS_VALU" in c:
+            row += f" valu {c['SQ_INSTS_VALU']:.0f} issue {c['SQ_INSTS_VALU'] * 4 / (1024 * us * 1e-6 * 2.4e9):.3f}"
+            if "SQ_INSTS_VALU_FMA_F64" in c:
+                row += f" fma_share {c['SQ_INSTS_VALU_FMA_F64'] / max(c['SQ_INSTS_VALU'], 1):.2f}"
+        if "GRBM_GUI_ACTIVE" in c:
+            row += f" clock_GHz {c['GRBM_GUI_ACTIVE'] / 8 / (us * 1e3):.2f}"
+        lines.append(row)
+        if b:
+            lines.append(f"KERNEL {kind} {lps[kind]} {us:.2f} {b}")
+        if traffic:
+            lines.append(f"TRAFFIC {kind} {traffic:.0f}")
+        if "SQ_INSTS_VALU" in c:
+            lines.append(f"VALU {kind} {c['SQ_INSTS_VALU']:.0f}")
+    if pass_us:
+        lines.append(f"NTT_PASSES bytes {pass_bytes:.0f} us {pass_us:.1f} TBps {pass_bytes / (pass_us * 1e-6) / 1e12:.3f} "
+                     f"frac {pass_bytes / (pass_us * 1e-6) / 8e12:.4f}")
+    return "\n".join(lines) + "\n"
 
 
 def _median_time(fn, runs: int, budget_s: float) -> tuple:
@@ -735,10 +862,18 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=240)
     ap.add_argument("--serve-load", type=float, default=0.7)
+    ap.add_argument("--replay-reps", type=int, default=20,
+                    help="replays of each kernel kind's last launch after the timed steps (0 in profiled runs)")
+    ap.add_argument("--profile-summary", default=None, metavar="DIR",
+                    help="print the text summary of a profiled run directory (kernel trace and counter runs "
+                         "of this workload with these --steps) instead of running anything")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="after the timed region, run the plain copy kernel over this many bytes "
                          "(the calibration reference for hardware byte counters)")
     args = ap.parse_args()
+    if args.profile_summary:
+        sys.stdout.write(profile_summary(args.profile_summary, args.workload, args.steps))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
@@ -776,7 +911,7 @@ def main() -> int:
         return 0
 
     # The timed workload (default: the headline, 2^20 x 2^20 limbs).
-    head = gpu_workload(args.workload, args.steps, args.warmup)
+    head = gpu_workload(args.workload, args.steps, args.warmup, replay_reps=args.replay_reps)
     if args.copy_bytes:
         n = (args.copy_bytes // 16) * 16
         src, dst = ntt.DeviceBuffer(n), ntt.DeviceBuffer(n)
@@ -794,12 +929,13 @@ def main() -> int:
         if row.get("bytes_per_launch") and us > dom_us:
             dominant, dom_us = k, us
     roofline, valu = None, None
-    committed = committed_counters(args.workload)
-    split_matches = committed["split"] == [head["log_len"], head["transform_split"]]
+    committed = committed_profile(args.workload)
+    split_matches = committed["split"] == [head["log_len"], head["transform_split"], head["transform_forms"] or []]
     if dominant:
         row = kernels[dominant]
         achieved = row["bytes_per_launch"] / (row["event_us"] * 1e-6) / 1e9
         replay = row.get("replay_us")
+        prof = committed["kernels"].get(dominant) if split_matches else None
         roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": committed["traffic"].get(dominant) if split_matches else None,
@@ -807,7 +943,11 @@ def main() -> int:
                     "duration_source": "HIP events around each launch on its stream, timed steps",
                     "launch_us_replayed": replay,
                     "achieved_replayed": round(row["bytes_per_launch"] / (replay * 1e-6) / 1e9, 1) if replay else None,
-                    "counters_split": committed["split"], "counters_match_split": split_matches}
+                    "profile": {"file": f"profiles/{args.workload}_kernels.txt", "split": committed["split"],
+                                "split_matches": split_matches,
+                                "launch_us": prof["us"] if prof else None,
+                                "frac": round(prof["bytes"] / (prof["us"] * 1e-6) / 8e12, 4) if prof else None,
+                                "event_vs_profile": round(row["event_us"] / prof["us"], 3) if prof else None}}
         insts = committed["valu"].get(dominant) if split_matches else None
         if insts:
             rate = insts / (row["event_us"] * 1e-6) / 1e9
@@ -840,7 +980,8 @@ def main() -> int:
         "config": {"workload": args.workload, "len_a": head["len_a"], "len_b": head["len_b"],
                    "primes": head["primes"], "chunk_bits": head["chunk_bits"],
                    "transform_len": ntt.row_len(head["log_len"], head["row_tail"]),
-                   "cyclic_len": 1 << head["log_len"], "transform_split": head["transform_split"]},
+                   "cyclic_len": 1 << head["log_len"], "transform_split": head["transform_split"],
+                   "transform_forms": head["transform_forms"]},
         "kernels": kernels,
         "ntt_passes": head["ntt_passes"],
         "check": head["check"],

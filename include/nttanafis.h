@@ -385,6 +385,7 @@ bool ntt_profile_replay(const char *kind, int reps, double *avg_ms);
 #define NTT_FORM_ROWS8 8u            /* row kernel runs radix-8 rounds with twice the threads */
 #define NTT_FORM_STRIDED8 16u        /* strided passes run radix-8 rounds with twice the threads */
 #define NTT_FORM_PAIRED 32u          /* product rows: one thread per product group, both operands' groups */
+#define NTT_FORM_WAVES 64u           /* 2^5- and 2^6-point strided passes in wave-owned lines, no barrier */
 #define NTT_FORM_TUNED 256u          /* the autotuner has timed this length and use */
 int ntt_transform_split(uint32_t log_len, int use, uint32_t *pass_logs, uint32_t *flags);
 

@@ -585,7 +585,7 @@ class Profile:
 SPLIT_USES = {"product": 0, "square": 1, "batch": 2}
 
 
-FORM_FLAGS = {"alt_tiles": 1, "inv16": 2, "per_prime_pack": 4, "rows8": 8, "strided8": 16, "paired": 32}
+FORM_FLAGS = {"alt_tiles": 1, "inv16": 2, "per_prime_pack": 4, "rows8": 8, "strided8": 16, "paired": 32, "waves": 64}
 FORM_TUNED = 256
 
 

@@ -175,7 +175,9 @@ struct Split {
     // of 2^5, 2^6 or 2^9 points in radix-8 rounds with twice the threads
     // (as many rounds as radix 16 there); product rows with one thread per
     // group of the product, taking both operands' groups (k_fused FORM 3).
-    uint8_t alt_tiles, inv16, per_prime_pack, rows8, strided8, paired;
+    // Strided passes of 2^5 or 2^6 points in wave-owned lines (no workgroup
+    // barrier, k_strided_waves).
+    uint8_t alt_tiles, inv16, per_prime_pack, rows8, strided8, paired, waves;
     int fused_form() const { return rows8 ? 2 : (paired ? 3 : (inv16 ? 1 : 0)); }
 };
 
@@ -185,6 +187,7 @@ enum SplitUse { kUseProduct = 0, kUseSquare = 1, kUseBatch = 2 };
 constexpr uint32_t kFormAltTiles = NTT_FORM_ALT_TILES, kFormInv16 = NTT_FORM_INV16,
                    kFormPerPrimePack = NTT_FORM_PER_PRIME_PACK, kFormRows8 = NTT_FORM_ROWS8,
                    kFormStrided8 = NTT_FORM_STRIDED8, kFormPaired = NTT_FORM_PAIRED,
+                   kFormWaves = NTT_FORM_WAVES,
                    kFormTuned = NTT_FORM_TUNED;
 Split split_of(int K, int use);
 

@@ -21,9 +21,10 @@ namespace nttamd {
 typedef unsigned __int128 u128;
 
 bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, bool pack, bool alt, bool r8,
-                        hipStream_t s);
+                        hipStream_t s, bool waves = false);
 bool launch_pack_primes(int logl, const PassArgs& a, int primes, int operands, hipStream_t s);
-bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, bool alt, bool r8, hipStream_t s);
+bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, bool alt, bool r8, hipStream_t s,
+                        bool waves = false);
 bool launch_fused(int logl, const PassArgs& a, int primes, bool pack, bool square, int form, hipStream_t s);
 bool pack_primes_batch_fits(int logl, uint64_t line_stride);
 bool launch_pack_primes_batch(int logl, const PassArgs& a, int items, hipStream_t s);
@@ -579,7 +580,7 @@ static bool split_ok(int K, const Split& s) {
 
 static bool same_split(const Split& a, const Split& b) {
     if (a.n != b.n || a.alt_tiles != b.alt_tiles || a.inv16 != b.inv16 || a.per_prime_pack != b.per_prime_pack ||
-        a.rows8 != b.rows8 || a.strided8 != b.strided8 || a.paired != b.paired)
+        a.rows8 != b.rows8 || a.strided8 != b.strided8 || a.paired != b.paired || a.waves != b.waves)
         return false;
     for (int i = 0; i < a.n; ++i)
         if (a.logs[i] != b.logs[i]) return false;
@@ -588,7 +589,7 @@ static bool same_split(const Split& a, const Split& b) {
 
 // Kernel-form flags forced from the environment (tests, A/B runs):
 // NTTAMD_ALT_TILES, NTTAMD_INV16, NTTAMD_PER_PRIME_PACK, NTTAMD_ROWS8,
-// NTTAMD_STRIDED8 and NTTAMD_PAIRED set to 0 or 1. -1: not forced.
+// NTTAMD_STRIDED8, NTTAMD_PAIRED and NTTAMD_WAVES set to 0 or 1. -1: not forced.
 static int env_flag(const char* name) {
     const char* e = getenv(name);
     return e && *e ? (atoi(e) != 0) : -1;
@@ -596,12 +597,12 @@ static int env_flag(const char* name) {
 static bool flags_forced() {
     return env_flag("NTTAMD_ALT_TILES") >= 0 || env_flag("NTTAMD_INV16") >= 0 ||
            env_flag("NTTAMD_PER_PRIME_PACK") >= 0 || env_flag("NTTAMD_ROWS8") >= 0 ||
-           env_flag("NTTAMD_STRIDED8") >= 0 || env_flag("NTTAMD_PAIRED") >= 0;
+           env_flag("NTTAMD_STRIDED8") >= 0 || env_flag("NTTAMD_PAIRED") >= 0 || env_flag("NTTAMD_WAVES") >= 0;
 }
 static Split with_env_flags(int K, Split s) {
     const int alt = env_flag("NTTAMD_ALT_TILES"), inv = env_flag("NTTAMD_INV16"),
               ppp = env_flag("NTTAMD_PER_PRIME_PACK"), r8 = env_flag("NTTAMD_ROWS8"),
-              s8 = env_flag("NTTAMD_STRIDED8"), pr = env_flag("NTTAMD_PAIRED");
+              s8 = env_flag("NTTAMD_STRIDED8"), pr = env_flag("NTTAMD_PAIRED"), wv = env_flag("NTTAMD_WAVES");
     if (alt >= 0 && s.n > 1) {
         Split t = s;
         t.alt_tiles = (uint8_t)alt;
@@ -612,6 +613,7 @@ static Split with_env_flags(int K, Split s) {
     if (r8 >= 0) s.rows8 = (uint8_t)r8;
     if (s8 >= 0 && s.n > 1) s.strided8 = (uint8_t)s8;
     if (pr >= 0) s.paired = (uint8_t)pr;
+    if (wv >= 0 && s.n > 1) s.waves = (uint8_t)wv;
     return s;
 }
 
@@ -621,8 +623,10 @@ static Split with_env_flags(int K, Split s) {
 // square's rows already run radix 16), radix-8 rows with twice the threads,
 // packing per prime instead of the shared first packing pass, radix-8
 // strided passes (single operations whose strided passes have 2^5, 2^6 or
-// 2^9 points), paired product rows (products).
-constexpr int kMaxVariants = 6;
+// 2^9 points), paired product rows (products), strided passes in
+// wave-owned lines (single operations whose strided passes have 2^5 or 2^6
+// points).
+constexpr int kMaxVariants = 7;
 static int flag_variants(int K, int use, const Split& best, Split out[kMaxVariants]) {
     int n = 0;
     Split t = best;
@@ -645,6 +649,11 @@ static int flag_variants(int K, int use, const Split& best, Split out[kMaxVarian
     t = best;
     t.paired = 1;
     if (use != kUseSquare && !best.paired) out[n++] = t;
+    t = best;
+    t.waves = 1;
+    bool w_pass = false;
+    for (int i = 0; i + 1 < best.n; ++i) w_pass |= best.logs[i] == 5 || best.logs[i] == 6;
+    if (use != kUseBatch && w_pass && !best.waves) out[n++] = t;
     return n;
 }
 
@@ -656,6 +665,7 @@ static Split with_variant(Split best, const Split& v) {
     best.rows8 |= v.rows8;
     best.strided8 |= v.strided8;
     best.paired |= v.paired;
+    best.waves |= v.waves;
     return best;
 }
 
@@ -825,7 +835,7 @@ static bool forward_head(const Split& sp, const PassArgs& a0, const Source& sa, 
         Timed t(i == 0 ? "fwd_pack_strided" : "fwd_strided", s);
         const bool shared = i == 0 && pack_shared() && !sp.per_prime_pack &&
                             launch_pack_primes(sp.logs[i], a, primes, operands, s);
-        if (!shared && !launch_fwd_strided(sp.logs[i], a, primes, operands, i == 0, sp.alt_tiles, sp.strided8, s)) return false;
+        if (!shared && !launch_fwd_strided(sp.logs[i], a, primes, operands, i == 0, sp.alt_tiles, sp.strided8, s, sp.waves)) return false;
         level += sp.logs[i];
     }
     return true;
@@ -839,7 +849,7 @@ static bool inverse_tail(const Split& sp, const PassArgs& a0, int primes, hipStr
         a.level_start = level;
         a.line_stride = (uint64_t)1 << (a.log_t - level - sp.logs[i]);
         Timed t(i == 0 ? "inv_strided_canon" : "inv_strided", s);
-        if (!launch_inv_strided(sp.logs[i], a, primes, i == 0, sp.alt_tiles, sp.strided8, s)) return false;
+        if (!launch_inv_strided(sp.logs[i], a, primes, i == 0, sp.alt_tiles, sp.strided8, s, sp.waves)) return false;
     }
     return true;
 }
@@ -2708,6 +2718,7 @@ int ntt_transform_split(uint32_t log_len, int use, uint32_t* pass_logs, uint32_t
         *flags = (sp.alt_tiles ? kFormAltTiles : 0u) | (sp.inv16 ? kFormInv16 : 0u) |
                  (sp.per_prime_pack ? kFormPerPrimePack : 0u) | (sp.rows8 ? kFormRows8 : 0u) |
                  (sp.strided8 ? kFormStrided8 : 0u) | (sp.paired ? kFormPaired : 0u) |
+                 (sp.waves ? kFormWaves : 0u) |
                  (tuned ? kFormTuned : 0u);
     }
     return sp.n;

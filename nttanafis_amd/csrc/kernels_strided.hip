@@ -41,6 +41,104 @@ __global__ __launch_bounds__(NT) void k_fwd_strided(PassArgs a) {
     store_strided<LOGL, LINES, NT, false>(lds, row + base, S, 0.0, p, pinv);
 }
 
+// Strided pass of 2^5 or 2^6-point lines in wave-owned lines: each wave
+// takes WL = 1024 / L whole lines of the tile (16 values per lane), so the
+// two radix rounds exchange values through LDS within the wave only and the
+// pass runs without a workgroup barrier. Lane (c, q) of a wave (line c of
+// its WL lines, q < L/16) holds in the round of the first FIRST = LOGL - 4
+// levels the groups i = q + QW*jj (points i + 16k) and in the round of the
+// last four levels the group of points 16q..16q+15; both rounds read or
+// write global memory directly, WL consecutive lines (WL * 8 bytes) per
+// row of the tile per instruction, and the LDS only carries the transpose
+// between them. MODE 0: forward (no packing), 2: inverse, 3: inverse ending
+// the transform (times T^-1, canonical residues). Bounds as fwd_round and
+// inv_round (a forward round entered below 31p/16, an inverse round below
+// 11p/8).
+// Orders a wave's LDS stores before its later LDS loads for every lane: the
+// hardware runs one wave's LDS instructions in order, and the fences keep the
+// compiler from moving the loads above the stores.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int LOGL, int WAVES, int MODE>
+__global__ __launch_bounds__(64 * WAVES) void k_strided_waves(PassArgs a) {
+    constexpr int L = 1 << LOGL;
+    constexpr int LS = Lay<LOGL>::STRIDE;
+    constexpr int FIRST = LOGL - 4;
+    constexpr int RA = 1 << FIRST;       // points of a first-round group
+    constexpr int WL = 1024 / L;         // lines per wave
+    constexpr int QW = L / 16;           // lanes per line
+    constexpr int GA = 16 / QW;          // first-round groups per lane
+    constexpr int LINES = WL * WAVES;
+    static_assert(LOGL == 5 || LOGL == 6, "two rounds with 16 first-round groups per line");
+    constexpr bool INV = MODE >= 2, CANON = MODE == 3;
+    __shared__ double lds[LINES * LS];
+    const uint64_t S = a.line_stride;
+    const uint64_t tiles_per_node = S / LINES;
+    Job j;
+    if (!resolve<false, xcd_group(LINES)>(a, tiles_per_node << a.level_start, strided_xcd(false, LINES), &j))
+        return;
+    const int y = j.y;
+    const double p = a.field.p[y], pinv = a.field.pinv[y];
+    const uint64_t H = j.tile / tiles_per_node;
+    const uint64_t lo0 = (j.tile % tiles_per_node) * LINES;
+    const uint64_t span = (uint64_t)1 << (a.log_t - a.level_start);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int cl = lane % WL, q = lane / WL;
+    const int c = wave * WL + cl;  // line of the tile
+    double* g = j.data + (uint64_t)y * a.row_stride + H * span + lo0 + c;  // point m at g[m * S]
+    double* line = lds + c * LS;
+    const Tw tw{(INV ? a.inv : a.fwd) + (uint64_t)y * a.table_stride, H + (a.node_base << a.level_start), 0, 0, 64, p,
+                pinv};
+    double x[16];
+    if constexpr (!INV) {
+#pragma unroll
+        for (int jj = 0; jj < GA; ++jj)
+#pragma unroll
+            for (int k = 0; k < RA; ++k) x[jj * RA + k] = g[(uint64_t)(q + QW * jj + 16 * k) * S];
+#pragma unroll
+        for (int jj = 0; jj < GA; ++jj) {
+            fwd_group<FIRST, false>(x + jj * RA, nullptr, tw, c, 0, 0, p, pinv);
+#pragma unroll
+            for (int k = 0; k < RA; ++k) line[pidx(q + QW * jj + 16 * k)] = x[jj * RA + k];
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int k = 0; k < 16; ++k) x[k] = line[pidx(16 * q + k)];
+        fwd_group<4, false>(x, nullptr, tw, c, FIRST, q, p, pinv);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) g[(uint64_t)(16 * q + k) * S] = x[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) x[k] = g[(uint64_t)(16 * q + k) * S];
+        inv_group<4, false>(x, tw, c, FIRST, q, p, pinv);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) line[pidx(16 * q + k)] = x[k];
+        wave_lds_sync();
+        const bool root1 = CANON && a.node_base == 0;
+        const double tinv = a.tinv[y];
+#pragma unroll
+        for (int jj = 0; jj < GA; ++jj) {
+            double* z = x + jj * RA;
+#pragma unroll
+            for (int k = 0; k < RA; ++k) z[k] = line[pidx(q + QW * jj + 16 * k)];
+            if (root1)
+                inv_group<FIRST, true>(z, tw, c, 0, 0, p, pinv);
+            else
+                inv_group<FIRST, false>(z, tw, c, 0, 0, p, pinv);
+#pragma unroll
+            for (int k = 0; k < RA; ++k) {
+                double v = z[k];
+                if constexpr (CANON) v = canonical(mulmod(v, tinv, p, pinv), p);
+                g[(uint64_t)(q + QW * jj + 16 * k) * S] = v;
+            }
+        }
+    }
+}
+
 // Levels S0..RL-1 of the transform's first round (radix 2^RL) on the points
 // of one group, in registers (fwd_round's butterflies with ROOT1: the group
 // sits at table node 0, whose level-0 root is 1, and holds packed inputs).
@@ -269,9 +367,42 @@ static bool launch_fwd_strided_l(const PassArgs& a, int units, int operands, boo
     return NTT_CHECK_LAUNCH();
 }
 
+// Wave-owned lines (k_strided_waves, four waves per workgroup) for lines of
+// 2^5 and 2^6 points; mode as the kernel's.
+__host__ __device__ constexpr bool waves_fit(int logl) { return logl == 5 || logl == 6; }
+template <int LOGL, int WAVES>
+static bool launch_waves_w(const PassArgs& a, int units, int z, int mode, hipStream_t s) {
+    if constexpr (waves_fit(LOGL)) {
+        constexpr int LINES = 1024 / (1 << LOGL) * WAVES;
+        if (a.line_stride < (uint64_t)LINES || a.batch) return false;
+        const uint64_t tiles = ((uint64_t)1 << a.level_start) * (a.line_stride / LINES);
+        PassArgs b = a;
+        b.primes = units;
+        b.tiles_per_item = tiles;
+        const dim3 grid((unsigned)tiles, (unsigned)units, (unsigned)z);
+        if (mode == 0)
+            NTT_LAUNCH(nullptr, (k_strided_waves<LOGL, WAVES, 0>), grid, dim3(64 * WAVES), s, b);
+        else if (mode == 2)
+            NTT_LAUNCH(nullptr, (k_strided_waves<LOGL, WAVES, 2>), grid, dim3(64 * WAVES), s, b);
+        else
+            NTT_LAUNCH(nullptr, (k_strided_waves<LOGL, WAVES, 3>), grid, dim3(64 * WAVES), s, b);
+        return NTT_CHECK_LAUNCH();
+    }
+    return false;
+}
+
+// Four waves per workgroup (64 lines of 2^6 points, 512-byte runs per row of
+// the tile); measured at 2^24 points x 2 primes: 1, 2, 8 or 16 waves per
+// workgroup move the same or less.
+template <int LOGL>
+static bool launch_waves(const PassArgs& a, int units, int z, int mode, hipStream_t s) {
+    return launch_waves_w<LOGL, 4>(a, units, z, mode, s);
+}
+
 template <int LOGL>
 static bool launch_fwd_strided_t(const PassArgs& a, int units, int operands, bool pack, bool alt, bool r8,
-                                 hipStream_t s) {
+                                 hipStream_t s, bool waves = false) {
+    if (waves && !pack && !a.batch && waves_fit(LOGL) && launch_waves<LOGL>(a, units, operands, 0, s)) return true;
     r8 = r8 && r8_fits(LOGL);
     return alt ? launch_fwd_strided_l<LOGL, alt_strided_lines(LOGL)>(a, units, operands, pack, s, r8)
                : launch_fwd_strided_l<LOGL, strided_lines(LOGL)>(a, units, operands, pack, s, r8);
@@ -340,22 +471,26 @@ static bool launch_inv_strided_l(const PassArgs& a, int units, bool canon, hipSt
 }
 
 template <int LOGL>
-static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, bool alt, bool r8, hipStream_t s) {
+static bool launch_inv_strided_t(const PassArgs& a, int units, bool canon, bool alt, bool r8, hipStream_t s,
+                                 bool waves = false) {
+    if (waves && !a.batch && waves_fit(LOGL) &&
+        launch_waves<LOGL>(a, units, a.split_blocks ? a.split_blocks : 1, canon ? 3 : 2, s))
+        return true;
     r8 = r8 && r8_fits(LOGL);
     return alt ? launch_inv_strided_l<LOGL, alt_strided_lines(LOGL)>(a, units, canon, s, r8)
                : launch_inv_strided_l<LOGL, strided_lines(LOGL)>(a, units, canon, s, r8);
 }
 
 bool launch_fwd_strided(int logl, const PassArgs& a, int primes, int operands, bool pack, bool alt, bool r8,
-                        hipStream_t s) {
+                        hipStream_t s, bool waves) {
     switch (logl) {
-        case 5: return launch_fwd_strided_t<5>(a, primes, operands, pack, alt, r8, s);
-        case 6: return launch_fwd_strided_t<6>(a, primes, operands, pack, alt, r8, s);
-        case 7: return launch_fwd_strided_t<7>(a, primes, operands, pack, alt, r8, s);
-        case 8: return launch_fwd_strided_t<8>(a, primes, operands, pack, alt, r8, s);
-        case 9: return launch_fwd_strided_t<9>(a, primes, operands, pack, alt, r8, s);
-        case 10: return launch_fwd_strided_t<10>(a, primes, operands, pack, alt, r8, s);
-        case 11: return launch_fwd_strided_t<11>(a, primes, operands, pack, alt, r8, s);
+        case 5: return launch_fwd_strided_t<5>(a, primes, operands, pack, alt, r8, s, waves);
+        case 6: return launch_fwd_strided_t<6>(a, primes, operands, pack, alt, r8, s, waves);
+        case 7: return launch_fwd_strided_t<7>(a, primes, operands, pack, alt, r8, s, waves);
+        case 8: return launch_fwd_strided_t<8>(a, primes, operands, pack, alt, r8, s, waves);
+        case 9: return launch_fwd_strided_t<9>(a, primes, operands, pack, alt, r8, s, waves);
+        case 10: return launch_fwd_strided_t<10>(a, primes, operands, pack, alt, r8, s, waves);
+        case 11: return launch_fwd_strided_t<11>(a, primes, operands, pack, alt, r8, s, waves);
         default: return false;
     }
 }
@@ -405,15 +540,16 @@ bool launch_pack_primes_batch(int logl, const PassArgs& a, int items, hipStream_
     }
 }
 
-bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, bool alt, bool r8, hipStream_t s) {
+bool launch_inv_strided(int logl, const PassArgs& a, int primes, bool canon, bool alt, bool r8, hipStream_t s,
+                        bool waves) {
     switch (logl) {
-        case 5: return launch_inv_strided_t<5>(a, primes, canon, alt, r8, s);
-        case 6: return launch_inv_strided_t<6>(a, primes, canon, alt, r8, s);
-        case 7: return launch_inv_strided_t<7>(a, primes, canon, alt, r8, s);
-        case 8: return launch_inv_strided_t<8>(a, primes, canon, alt, r8, s);
-        case 9: return launch_inv_strided_t<9>(a, primes, canon, alt, r8, s);
-        case 10: return launch_inv_strided_t<10>(a, primes, canon, alt, r8, s);
-        case 11: return launch_inv_strided_t<11>(a, primes, canon, alt, r8, s);
+        case 5: return launch_inv_strided_t<5>(a, primes, canon, alt, r8, s, waves);
+        case 6: return launch_inv_strided_t<6>(a, primes, canon, alt, r8, s, waves);
+        case 7: return launch_inv_strided_t<7>(a, primes, canon, alt, r8, s, waves);
+        case 8: return launch_inv_strided_t<8>(a, primes, canon, alt, r8, s, waves);
+        case 9: return launch_inv_strided_t<9>(a, primes, canon, alt, r8, s, waves);
+        case 10: return launch_inv_strided_t<10>(a, primes, canon, alt, r8, s, waves);
+        case 11: return launch_inv_strided_t<11>(a, primes, canon, alt, r8, s, waves);
         default: return false;
     }
 }

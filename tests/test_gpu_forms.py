@@ -5,11 +5,12 @@ length and use: the alternative strided tiles (other line counts per tile),
 radix-16 instead of radix-8 inverse rows in the product row kernel, radix-8
 row rounds with twice the threads, first-pass packing per (tile, prime)
 instead of one packing workgroup shared by the primes of a tile, radix-8
-strided passes with twice the threads, and paired product rows (one thread
+strided passes with twice the threads, paired product rows (one thread
 per product group, both operands' groups, the pointwise product between the
-last forward and first inverse round in registers). NTTAMD_ALT_TILES,
-NTTAMD_INV16, NTTAMD_ROWS8, NTTAMD_PER_PRIME_PACK, NTTAMD_STRIDED8 and
-NTTAMD_PAIRED force them. Each forced form must reproduce the
+last forward and first inverse round in registers), and 2^5- and 2^6-point
+strided passes in wave-owned lines. NTTAMD_ALT_TILES, NTTAMD_INV16,
+NTTAMD_ROWS8, NTTAMD_PER_PRIME_PACK, NTTAMD_STRIDED8, NTTAMD_PAIRED and
+NTTAMD_WAVES force them. Each forced form must reproduce the
 default form's product bit for bit under every admissible split, for single
 products, squares and batches; the default itself is checked against the
 oracle (MulPlan::mul, src/plan.rs:112-150) or, for the largest shapes,
@@ -30,7 +31,7 @@ from nttanafis_amd.workloads import batch_operands, operand
 pytestmark = pytest.mark.gpu
 
 FORM_ENV = ("NTTAMD_ALT_TILES", "NTTAMD_INV16", "NTTAMD_PER_PRIME_PACK", "NTTAMD_ROWS8", "NTTAMD_ROW_LOG",
-            "NTTAMD_PASSES", "NTTAMD_STRIDED8", "NTTAMD_PAIRED")
+            "NTTAMD_PASSES", "NTTAMD_STRIDED8", "NTTAMD_PAIRED", "NTTAMD_WAVES")
 
 SHAPES = [(4000, 4000, 3), (30000, 30000, 0), (800000, 800000, 0), (2500000, 2500000, 0),
           (8000000, 8000000, 0)]
@@ -96,7 +97,7 @@ def test_forms_match_the_default(oracle, monkeypatch, la, lb, force_primes):
     say(f"{la}x{lb}: {primes} primes, 2^{K} points")
     a, b = operand(la, 5), operand(lb, 6)
     force(NTTAMD_ALT_TILES=0, NTTAMD_INV16=0, NTTAMD_PER_PRIME_PACK=0, NTTAMD_ROWS8=0, NTTAMD_STRIDED8=0,
-          NTTAMD_PAIRED=0)
+          NTTAMD_PAIRED=0, NTTAMD_WAVES=0)
     ref = product(a, b)
     ref_sq = square(a)
     if K <= 21:
@@ -123,7 +124,11 @@ def test_forms_match_the_default(oracle, monkeypatch, la, lb, force_primes):
         say(f"  {passes} passes, rows 2^{r}: paired rows, radix-8 strided passes")
         assert (product(a, b) == ref).all(), f"paired rows + radix-8 strided, {passes} passes, rows 2^{r}"
         assert (square(a) == ref_sq).all(), f"square, radix-8 strided, {passes} passes, rows 2^{r}"
-        force(NTTAMD_PAIRED=0, NTTAMD_STRIDED8=0)
+        force(NTTAMD_PAIRED=0, NTTAMD_STRIDED8=0, NTTAMD_WAVES=1)
+        say(f"  {passes} passes, rows 2^{r}: strided passes in wave-owned lines")
+        assert (product(a, b) == ref).all(), f"wave-owned strided lines, {passes} passes, rows 2^{r}"
+        assert (square(a) == ref_sq).all(), f"square, wave-owned strided lines, {passes} passes, rows 2^{r}"
+        force(NTTAMD_WAVES=0)
     os.environ.pop("NTTAMD_PASSES")
     os.environ.pop("NTTAMD_ROW_LOG")
     force(NTTAMD_ALT_TILES=1, NTTAMD_INV16=1, NTTAMD_PER_PRIME_PACK=1, NTTAMD_ROWS8=0)
@@ -196,7 +201,7 @@ def test_tuned_forms_are_reported(oracle, n):
     form = ntt.transform_form(K, "product")
     say(f"2^{K} product: {form}")
     assert form["tuned"]
-    assert set(form["forms"]) <= {"alt_tiles", "inv16", "per_prime_pack", "rows8", "strided8", "paired"}
+    assert set(form["forms"]) <= {"alt_tiles", "inv16", "per_prime_pack", "rows8", "strided8", "paired", "waves"}
     assert sum(form["passes"]) == K
     sq = ntt.transform_form(K, "square")
     assert sq["tuned"] and "inv16" not in sq["forms"], sq
