@@ -193,6 +193,46 @@ static constexpr int pack_primes_threads() {
 // 15p/16 (multiplied inputs below 4p, products below 2p^2), level 2 reduces
 // the added input and ends below 9p/16 + 15p/16 = 3p/2, and every later round
 // is entered below 31p/16 (fwd_round's schedule).
+// Last forward round of a strided tile (local levels L0..LOGL-1) read from
+// LDS and written straight to global memory (point m of line c at
+// g[m * S + c]): thread t takes line t % LINES, so each store instruction
+// writes LINES consecutive points of one row of the tile.
+template <int LOGL, int LINES, int NT, int L0>
+__device__ __forceinline__ void last_round_direct(const double* lds, const Tw& tw, double* g, uint64_t S, double p,
+                                                  double pinv) {
+    constexpr int L = 1 << LOGL;
+    constexpr int RL = LOGL - L0;
+    constexpr int RADIX = 1 << RL;
+    constexpr int GPL = L / RADIX;  // groups per line; D = 1
+    constexpr int LS = Lay<LOGL>::STRIDE;
+    constexpr int TPL = NT / LINES;  // threads per line
+    static_assert(NT % LINES == 0, "whole lines per thread set");
+    const int c = threadIdx.x % LINES;
+    const double* line = lds + c * LS;
+#pragma unroll
+    for (int b = threadIdx.x / LINES; b < GPL; b += TPL) {
+        double x[RADIX];
+#pragma unroll
+        for (int k = 0; k < RADIX; ++k) x[k] = line[pidx(b * RADIX + k)];
+        fwd_group<RL, false>(x, nullptr, tw, c, L0, b, p, pinv);
+#pragma unroll
+        for (int k = 0; k < RADIX; ++k) g[(uint64_t)(b * RADIX + k) * S + c] = x[k];
+    }
+}
+
+// Levels L0..LOGL-1 of a packing tile after its first round: radix-8 rounds
+// through LDS, the last one straight to global memory.
+template <int LOGL, int LINES, int NT, int L0>
+__device__ __forceinline__ void pack_rest(double* lds, const Tw& tw, double* g, uint64_t S, double p, double pinv) {
+    if constexpr (LOGL - L0 > 3) {
+        fwd_round<LOGL, LINES, NT, L0, 3, false>(lds, tw, p, pinv);
+        __syncthreads();
+        pack_rest<LOGL, LINES, NT, L0 + 3>(lds, tw, g, S, p, pinv);
+    } else {
+        last_round_direct<LOGL, LINES, NT, L0>(lds, tw, g, S, p, pinv);
+    }
+}
+
 template <int LOGL, int LINES, int NT, int KIND>
 __device__ __forceinline__ void pack_primes_tile(const PassArgs& a, const Job& j, int primes, double* lds) {
     constexpr int L = 1 << LOGL;
@@ -243,21 +283,23 @@ __device__ __forceinline__ void pack_primes_tile(const PassArgs& a, const Job& j
         } else {
             first_round_regs<3, 0>(x, tw, p, pinv);
         }
-        double* line = lds + c * LS;
+        // Primes alternate between two LDS tiles: a prime's first barrier
+        // (after its stores below) also tells that every thread has read the
+        // tile of the prime before last, so no barrier closes a prime.
+        double* buf = lds + (y & 1) * LINES * LS;
+        double* line = buf + c * LS;
 #pragma unroll
         for (int k = 0; k < 8; ++k) line[pidx(m0 + k * D)] = x[k];
         __syncthreads();
-        // Radix-8 rounds: one group per thread, and eight values in flight
-        // beside the extracted parts.
-        fwd_rounds_from<LOGL, LINES, NT, 3, 3>(lds, tw, p, pinv);
-        store_strided<LOGL, LINES, NT, false>(lds, j.data + (uint64_t)y * a.row_stride + base, S, 0.0, p, pinv);
-        __syncthreads();  // the tile is read out before the next prime overwrites it
+        // Radix-8 rounds, one group per thread, eight values in flight beside
+        // the extracted parts; the last one stores to global memory directly.
+        pack_rest<LOGL, LINES, NT, 3>(buf, tw, j.data + (uint64_t)y * a.row_stride + base, S, p, pinv);
     }
 }
 
 template <int LOGL, int LINES, int NT, int KIND>
 __global__ __launch_bounds__(NT) void k_pack_primes(PassArgs a) {
-    __shared__ double lds[LINES * Lay<LOGL>::STRIDE];
+    __shared__ double lds[2 * LINES * Lay<LOGL>::STRIDE];  // two tiles (pack_primes_tile)
     Job j;
     resolve<false>(a, a.line_stride / LINES, false, &j);
     pack_primes_tile<LOGL, LINES, NT, KIND>(a, j, a.primes, lds);
@@ -269,7 +311,7 @@ __global__ __launch_bounds__(NT) void k_pack_primes(PassArgs a) {
 // product whose coefficients need four parts.
 template <int LOGL, int LINES, int NT>
 __global__ __launch_bounds__(NT) void k_pack_primes_batch(PassArgs a) {
-    __shared__ double lds[LINES * Lay<LOGL>::STRIDE];
+    __shared__ double lds[2 * LINES * Lay<LOGL>::STRIDE];  // two tiles (pack_primes_tile)
     Job j;
     resolve<true>(a, 0, false, &j);
     const int primes = (int)a.batch[a.items[blockIdx.x / a.tiles_per_item].product].primes;
