@@ -96,11 +96,6 @@ def row_parts(log_len: int, tail: int):
     return [n] + {0: [], 1: [n // 4], 2: [n // 2], 3: [n // 2, n // 4]}[tail]
 
 
-def passes_of(n: int) -> int:
-    k = n.bit_length() - 1
-    return 1 if k <= 12 else (2 if k <= 23 else 3)
-
-
 def step_bytes(name: str, primes: int, log_len: int, tail: int, la: int, lb: int, out_len: int,
                square: bool) -> float:
     """Algorithmic HBM bytes that the launches of one kernel kind move in one
@@ -113,7 +108,8 @@ def step_bytes(name: str, primes: int, log_len: int, tail: int, la: int, lb: int
     T = n + tail * (n // 4)
     total = 0.0
     for m in row_parts(log_len, tail):
-        passes = passes_of(m)
+        # The passes the engine runs for this length (its tuned split).
+        passes = len(ntt.transform_form(m.bit_length() - 1, "square" if square else "product")["passes"])
         if name == "fused_single" and passes == 1:
             total += limbs + 8.0 * primes * m
         elif name == "fwd_pack_strided" and passes > 1:
@@ -133,6 +129,49 @@ def step_bytes(name: str, primes: int, log_len: int, tail: int, la: int, lb: int
     if name == "crt_disjoint":
         total = 8.0 * primes * (la + lb - 1) + 8.0 * out_len
     return total
+
+
+FORM_ENV = {"alt_tiles": "NTTAMD_ALT_TILES", "inv16": "NTTAMD_INV16", "per_prime_pack": "NTTAMD_PER_PRIME_PACK",
+            "rows8": "NTTAMD_ROWS8", "strided8": "NTTAMD_STRIDED8", "paired": "NTTAMD_PAIRED", "waves": "NTTAMD_WAVES"}
+
+
+def pin_env(split) -> dict:
+    """Environment that forces a split [log_len, pass levels, forms]: the
+    pass count and row size, and every kernel-form flag on or off."""
+    _, logs, forms = split
+    env = {}
+    if len(logs) > 1:
+        env["NTTAMD_PASSES"] = str(len(logs))
+        env["NTTAMD_ROW_LOG"] = str(logs[-1])
+    for f, e in FORM_ENV.items():
+        env[e] = "1" if f in forms else "0"
+    return env
+
+
+# Workloads that run the transforms of another workload's profile: the same
+# lengths, prime counts and kinds of operation.
+PROFILE_OF = {"ones_mul_2p20": "mul_2p20", "ones_sqr_2p24": "sqr_2p24", "conv32s_2p22": "conv16_2p22"}
+
+
+def pinned_workload(name: str, steps: int, warmup: int, pin: bool, **kw) -> dict:
+    """gpu_workload with the split and kernel forms of the workload's
+    committed profile forced for its duration (when pin and a profile
+    exists), so that the line and the profile describe the same kernels;
+    `split_source` says which."""
+    committed = committed_profile(PROFILE_OF.get(name, name))
+    env = pin_env(committed["split"]) if pin and committed["split"] else {}
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        r = gpu_workload(name, steps, warmup, **kw)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    r["split_source"] = "committed profile" if env else "autotuned"
+    return r
 
 
 def committed_profile(workload: str) -> dict:
@@ -183,6 +222,21 @@ def kernel_kind(name: str):
             "k_copy": "copy"}.get(k)
 
 
+def stats_summary(d: str) -> str:
+    """Per-kernel statistics of a profiled run without a single timed
+    workload (batch, served stream, small products): rocprofv3's
+    <d>/trace/run_kernel_stats.csv with the kernel kind of each name."""
+    import csv
+    rows = list(csv.DictReader(open(os.path.join(d, "trace", "run_kernel_stats.csv"))))
+    lines = [f"{'kernel':64s} {'kind':18s} {'calls':>7s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'pct':>6s}"]
+    for r in rows:
+        name = r["Name"].replace("void ", "").replace("nttamd::", "").replace("(PassArgs)", "").replace("(CrtArgs)", "")
+        kind = kernel_kind(r["Name"]) or "-"
+        lines.append(f"{name[:64]:64s} {kind:18s} {int(r['Calls']):7d} {float(r['AverageNs']) / 1000:9.2f} "
+                     f"{float(r['MinNs']) / 1000:9.2f} {float(r['MaxNs']) / 1000:9.2f} {float(r['Percentage']):6.2f}")
+    return "\n".join(lines) + "\n"
+
+
 def profile_summary(d: str, workload_name: str, steps: int) -> str:
     """Text summary of a profiled run directory: <d>/trace.json (the bench
     line of the kernel-trace run), <d>/trace/run_kernel_trace.csv, and
@@ -194,6 +248,8 @@ def profile_summary(d: str, workload_name: str, steps: int) -> str:
     what they read for the plain copy kernel the counter runs end with."""
     import csv
     import glob
+    if not os.path.exists(os.path.join(d, "trace.json")):
+        return stats_summary(d)
     bench_line = json.loads([x for x in open(os.path.join(d, "trace.json")) if x.startswith("{")][-1])
     lps = {k: r["launches_per_step"] for k, r in bench_line["kernels"].items()}
     cfg = bench_line["config"]
@@ -867,6 +923,9 @@ def main() -> int:
     ap.add_argument("--profile-summary", default=None, metavar="DIR",
                     help="print the text summary of a profiled run directory (kernel trace and counter runs "
                          "of this workload with these --steps) instead of running anything")
+    ap.add_argument("--no-pin", action="store_true",
+                    help="autotune the timed workload's split instead of running the split and kernel forms of "
+                         "its committed profile (profiles/<workload>_kernels.txt)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="after the timed region, run the plain copy kernel over this many bytes "
                          "(the calibration reference for hardware byte counters)")
@@ -911,7 +970,13 @@ def main() -> int:
         return 0
 
     # The timed workload (default: the headline, 2^20 x 2^20 limbs).
-    head = gpu_workload(args.workload, args.steps, args.warmup, replay_reps=args.replay_reps)
+    # The timed workload runs the split and kernel forms of its committed
+    # profile (when there is one), so that the roofline below and the
+    # profile describe the same kernels: the autotuner's choice between
+    # near-equal candidates differs from box to box.
+    committed = committed_profile(args.workload)
+    head = pinned_workload(args.workload, args.steps, args.warmup, not args.no_pin, replay_reps=args.replay_reps)
+    pinned = head["split_source"] == "committed profile"
     if args.copy_bytes:
         n = (args.copy_bytes // 16) * 16
         src, dst = ntt.DeviceBuffer(n), ntt.DeviceBuffer(n)
@@ -929,7 +994,6 @@ def main() -> int:
         if row.get("bytes_per_launch") and us > dom_us:
             dominant, dom_us = k, us
     roofline, valu = None, None
-    committed = committed_profile(args.workload)
     split_matches = committed["split"] == [head["log_len"], head["transform_split"], head["transform_forms"] or []]
     if dominant:
         row = kernels[dominant]
@@ -981,7 +1045,8 @@ def main() -> int:
                    "primes": head["primes"], "chunk_bits": head["chunk_bits"],
                    "transform_len": ntt.row_len(head["log_len"], head["row_tail"]),
                    "cyclic_len": 1 << head["log_len"], "transform_split": head["transform_split"],
-                   "transform_forms": head["transform_forms"]},
+                   "transform_forms": head["transform_forms"],
+                   "split_source": head["split_source"]},
         "kernels": kernels,
         "ntt_passes": head["ntt_passes"],
         "check": head["check"],
@@ -995,7 +1060,7 @@ def main() -> int:
         configs = {"config1_1024": bench_config1()}
         for name, steps in (("sqr_2p24", 10), ("ones_mul_2p20", 20), ("ones_sqr_2p24", 10), ("conv16_2p22", 10),
                             ("conv32s_2p22", 10)):
-            r = gpu_workload(name, steps, 2)
+            r = pinned_workload(name, steps, 2, not args.no_pin)
             pub = PUBLISHED.get(name)
             if pub:
                 r["vs_reference_published"] = round(r["rate"] / pub["value"], 1)

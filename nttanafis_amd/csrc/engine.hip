@@ -1212,20 +1212,28 @@ static void tune_split(Executor& ex, int K, int use, const std::function<bool()>
         if (e0) hipEventDestroy(e0);
         return;
     }
+    // Time per run of a candidate: one warm run (timed), then enough runs
+    // for about 2 ms of work (3 to 20), so that short products are not
+    // decided by launch jitter.
     auto time_of = [&](const Split& sp) {
         t_force = sp;
-        bool ok = run();  // warm
         hipEventRecord(e0, ex.stream);
-        for (int r = 0; r < 3 && ok; ++r) ok = run();
+        bool ok = run();
+        hipEventRecord(e1, ex.stream);
+        float warm = 1e30f;
+        if (ok && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&warm, e0, e1);
+        const int reps = warm <= 0.1f ? 20 : std::max(3, std::min(20, (int)(2.0f / warm)));
+        hipEventRecord(e0, ex.stream);
+        for (int r = 0; r < reps && ok; ++r) ok = run();
         hipEventRecord(e1, ex.stream);
         float ms = 1e30f;
         if (ok && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
-        return ok ? ms : 1e30f;
+        return ok ? ms / reps : 1e30f;
     };
     // A later candidate replaces the best only when it is faster by the
-    // margin, so near-equal candidates resolve to the first one (the largest
-    // row, no flags) instead of to timing noise.
-    constexpr float kMargin = 0.98f;
+    // margin (3 %), so near-equal candidates resolve to the first one (the
+    // largest row, no flags) instead of to timing noise.
+    constexpr float kMargin = 0.97f;
     Split best = cand[0];
     float best_ms = 1e30f;
     if (cand.size() > 1)
