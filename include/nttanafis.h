@@ -376,9 +376,10 @@ bool ntt_profile_replay(const char *kind, int reps, double *avg_ms);
  * `flags` is not NULL, the kernel forms chosen with it (NTT_FORM_*); the split
  * is the one the autotuner chose for this length and use, the one
  * NTTAMD_ROW_LOG / NTTAMD_PASSES (and NTTAMD_ALT_TILES, NTTAMD_INV16,
- * NTTAMD_PER_PRIME_PACK, NTTAMD_ROWS8 set to 0 or 1) force, or the default. 0 for lengths
- * out of range. No counterpart in the reference (the reference's split lives
- * in its Geometry). */
+ * NTTAMD_PER_PRIME_PACK, NTTAMD_ROWS8, NTTAMD_STRIDED8, NTTAMD_PAIRED,
+ * NTTAMD_WAVES set to 0 or 1) force, or the default. 0 for lengths out of
+ * range. No counterpart in the reference (the reference's split lives in its
+ * Geometry). */
 #define NTT_FORM_ALT_TILES 1u        /* strided passes use the alternative tiles */
 #define NTT_FORM_INV16 2u            /* product rows run the inverse in radix 16 (else 8) */
 #define NTT_FORM_PER_PRIME_PACK 4u   /* first pass packs per (tile, prime), not shared across primes */
