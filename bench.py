@@ -7,7 +7,7 @@ resident in HBM. A step is one MulWorkspace product on the device; the metric
 is operand limbs per second (Mlimbs/s). Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--workload mul_2p20|sqr_2p24|ones_mul_2p20|ones_sqr_2p24|
-                                                          conv16_2p22|conv16_5x2p20|conv32s_2p22|
+                                                          conv16_2p22|conv16_5x2p20|conv32s_2p22|split_mul|
                                                           mul_1024|sqr_1024|batch4096|served|small]
 
 Besides `value` (K steps between device synchronizations), the line carries:
@@ -60,6 +60,10 @@ def workload(name: str):
         return dict(kind="mul", a=splitmix(1 << 20, 1), b=splitmix(1 << 20, 2))
     if name == "ones_mul_2p20":
         return dict(kind="mul", a=ones(1 << 20), b=ones(1 << 20))
+    if name == "split_mul":
+        # 65,536 x 4,194,304 limbs: an unbalanced product the engine runs as
+        # 9 blocks of the longer operand against one spectrum of the shorter.
+        return dict(kind="mul", a=splitmix(1 << 22, 1), b=splitmix(1 << 16, 2))
     if name == "mul_1024":
         return dict(kind="mul", a=splitmix(1024, 1), b=splitmix(1024, 2))
     if name == "sqr_1024":
@@ -97,12 +101,17 @@ def row_parts(log_len: int, tail: int):
 
 
 def step_bytes(name: str, primes: int, log_len: int, tail: int, la: int, lb: int, out_len: int,
-               square: bool) -> float:
+               square: bool, blocks: int = 0) -> float:
     """Algorithmic HBM bytes that the launches of one kernel kind move in one
     step: every residue of a pass read and written once, packing reads the
     limbs once per transform it feeds, and the recombination of a composite
-    row reads the values it combines and writes the ones it changes."""
-    ops = 1 if square else 2
+    row reads the values it combines and writes the ones it changes. A split
+    product (blocks > 0) transforms each block of the longer operand and the
+    shorter operand once (blocks + 1 rows forward), multiplies every block by
+    the shorter operand's spectrum and inverts every block; its
+    reconstruction reads every block's rows."""
+    ops = 1 if square else (blocks + 1 if blocks else 2)
+    rows = blocks if blocks else 1  # product rows of the row and inverse passes
     limbs = 8.0 * (la if square else la + lb)
     n = 1 << log_len
     T = n + tail * (n // 4)
@@ -117,15 +126,15 @@ def step_bytes(name: str, primes: int, log_len: int, tail: int, la: int, lb: int
         elif name == "fwd_strided" and passes == 3:
             total += 16.0 * ops * primes * m
         elif name == "inv_strided" and passes == 3:
-            total += 16.0 * primes * m
+            total += 16.0 * rows * primes * m
         elif name == "fused_rows" and passes > 1:
-            total += (16.0 if square else 24.0) * primes * m
+            total += (16.0 if square else 24.0) * rows * primes * m
         elif name == "inv_strided_canon" and passes > 1:
-            total += 16.0 * primes * m
+            total += 16.0 * rows * primes * m
     if name == "recombine" and tail:
         total = 8.0 * primes * {1: 7 * n / 4, 2: 5 * n / 2, 3: 13 * n / 4}[tail]
     if name == "crt_product":
-        total = 8.0 * primes * min(T, out_len * 64) + 8.0 * out_len
+        total = 8.0 * primes * (rows * T if blocks else min(T, out_len * 64)) + 8.0 * out_len
     if name == "crt_disjoint":
         total = 8.0 * primes * (la + lb - 1) + 8.0 * out_len
     return total
@@ -753,6 +762,7 @@ def gpu_workload(name: str, steps: int, warmup: int, replay_reps: int = 20, runs
         ws = ntt.ConvWorkspace()
         ws.prepare(plan)
         primes, log_len, tail, chunk = plan.primes, plan.log_len, plan.row_tail, 64
+        blocks = plan.split_blocks
         dout = ntt.DeviceBuffer(out_len * 8)
 
         def step():
@@ -763,6 +773,7 @@ def gpu_workload(name: str, steps: int, warmup: int, replay_reps: int = 20, runs
         ws = ntt.MulWorkspace()
         ws.prepare(plan)
         primes, log_len, tail, chunk = plan.primes, plan.log_len, plan.row_tail, plan.chunk_bits
+        blocks = 0 if kind == "sqr" else (ntt.mul_split(la, lb) or (0, 0))[0]
         dout = ntt.DeviceBuffer(out_len * 8)
         if kind == "sqr":
             def step():
@@ -798,7 +809,7 @@ def gpu_workload(name: str, steps: int, warmup: int, replay_reps: int = 20, runs
     for k, (ms, count) in prof.items():
         per_step = max(1, int(round(count / steps)))
         rep = ntt.Profile.replay(k, replay_reps)
-        nbytes = step_bytes(k, primes, log_len, tail, la, lb, out_len, kind == "sqr") / per_step
+        nbytes = step_bytes(k, primes, log_len, tail, la, lb, out_len, kind == "sqr", blocks) / per_step
         ev_us = 1000.0 * ms / max(count, 1)
         row = {"event_us": round(ev_us, 2), "launches_per_step": per_step}
         if rep is not None:
