@@ -134,11 +134,12 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
         // Pointwise product: every value loaded before the first store (the
         // stores could alias later loads, which would serialise them).
         double v[R], w[R];
+        const int t0 = row_off<LOGL>(threadIdx.x);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int e = threadIdx.x + r * NT;
             if (!FULL && e >= PTS) continue;
-            const double* x = lds + (e >> LOGL) * LS + pidx(e & (L - 1));
+            const double* x = lds + t0 + row_off<LOGL>(r * NT);
             v[r] = x[0];
             w[r] = SQUARE ? x[0] : x[LINES * LS];
         }
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
         for (int r = 0; r < R; ++r) {
             const int e = threadIdx.x + r * NT;
             if (!FULL && e >= PTS) continue;
-            lds[(e >> LOGL) * LS + pidx(e & (L - 1))] = v[r];
+            lds[t0 + row_off<LOGL>(r * NT)] = v[r];
         }
         __syncthreads();
         // The product is one tile: radix-16 groups would occupy half of the
@@ -161,11 +162,12 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
     }
     if constexpr (PACK) {
         const double tinv = a.tinv[y];
+        const int t0 = row_off<LOGL>(threadIdx.x);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int m = threadIdx.x + r * NT;
             if (!FULL && m >= L) continue;
-            row[m] = canonical(mulmod(lds[pidx(m)], tinv, p, pinv), p);
+            row[m] = canonical(mulmod(lds[t0 + row_off<LOGL>(r * NT)], tinv, p, pinv), p);
         }
     } else {
         store_rows<LOGL, LINES, NT>(lds, row);

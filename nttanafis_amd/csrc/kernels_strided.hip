@@ -103,11 +103,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_strided_waves(PassArgs a) {
         for (int jj = 0; jj < GA; ++jj) {
             fwd_group<FIRST, false>(x + jj * RA, nullptr, tw, c, 0, 0, p, pinv);
 #pragma unroll
-            for (int k = 0; k < RA; ++k) line[pidx(q + QW * jj + 16 * k)] = x[jj * RA + k];
+            for (int k = 0; k < RA; ++k) line[q + QW * jj + 17 * k] = x[jj * RA + k];
         }
         wave_lds_sync();
 #pragma unroll
-        for (int k = 0; k < 16; ++k) x[k] = line[pidx(16 * q + k)];
+        for (int k = 0; k < 16; ++k) x[k] = line[17 * q + k];
         fwd_group<4, false>(x, nullptr, tw, c, FIRST, q, p, pinv);
 #pragma unroll
         for (int k = 0; k < 16; ++k) g[(uint64_t)(16 * q + k) * S] = x[k];
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_strided_waves(PassArgs a) {
         for (int k = 0; k < 16; ++k) x[k] = g[(uint64_t)(16 * q + k) * S];
         inv_group<4, false>(x, tw, c, FIRST, q, p, pinv);
 #pragma unroll
-        for (int k = 0; k < 16; ++k) line[pidx(16 * q + k)] = x[k];
+        for (int k = 0; k < 16; ++k) line[17 * q + k] = x[k];
         wave_lds_sync();
         const bool root1 = CANON && a.node_base == 0;
         const double tinv = a.tinv[y];
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_strided_waves(PassArgs a) {
         for (int jj = 0; jj < GA; ++jj) {
             double* z = x + jj * RA;
 #pragma unroll
-            for (int k = 0; k < RA; ++k) z[k] = line[pidx(q + QW * jj + 16 * k)];
+            for (int k = 0; k < RA; ++k) z[k] = line[q + QW * jj + 17 * k];
             if (root1)
                 inv_group<FIRST, true>(z, tw, c, 0, 0, p, pinv);
             else
@@ -213,7 +213,7 @@ __device__ __forceinline__ void last_round_direct(const double* lds, const Tw& t
     for (int b = threadIdx.x / LINES; b < GPL; b += TPL) {
         double x[RADIX];
 #pragma unroll
-        for (int k = 0; k < RADIX; ++k) x[k] = line[pidx(b * RADIX + k)];
+        for (int k = 0; k < RADIX; ++k) x[k] = line[pidx(b * RADIX) + k];
         fwd_group<RL, false>(x, nullptr, tw, c, L0, b, p, pinv);
 #pragma unroll
         for (int k = 0; k < RADIX; ++k) g[(uint64_t)(b * RADIX + k) * S + c] = x[k];
@@ -289,7 +289,7 @@ __device__ __forceinline__ void pack_primes_tile(const PassArgs& a, const Job& j
         double* buf = lds + (y & 1) * LINES * LS;
         double* line = buf + c * LS;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) line[pidx(m0 + k * D)] = x[k];
+        for (int k = 0; k < 8; ++k) line[pidx(m0) + pidx(k * D)] = x[k];
         __syncthreads();
         // Radix-8 rounds, one group per thread, eight values in flight beside
         // the extracted parts; the last one stores to global memory directly.

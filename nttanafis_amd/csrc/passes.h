@@ -60,7 +60,7 @@ inline void prof_remember(const char* kind, const void* fn, dim3 grid, dim3 bloc
 // 16 values they fall on 16 distinct bank pairs (a pad per 32 values left
 // pairs of lanes on one bank pair). Consecutive points and the 2-value
 // stripes of row loads stay conflict-free.
-__device__ __forceinline__ int pidx(int e) { return e + (e >> 4); }
+__host__ __device__ constexpr int pidx(int e) { return e + (e >> 4); }
 
 template <int LOGL>
 struct Lay {
@@ -69,6 +69,28 @@ struct Lay {
     static constexpr int ROUNDS = (LOGL + 3) / 4;
     static constexpr int FIRST = LOGL - 4 * (ROUNDS - 1);  // levels of round 0
 };
+
+// pidx(m0 + m) = pidx(m0) + pidx(m) when m is a multiple of a power of two
+// M and m0 < M, or when m0 is a multiple of 16: the pads before m0 + m are
+// then those before m0 plus m >> 4. The LDS walks split every index that
+// way, so a thread computes the variable part once and the constant part of
+// each access folds into the instruction's offset (the compiler does not
+// see through the shift of a sum on its own).
+//
+// Offset of point f of a tile of rows: row f >> LOGL, point f mod 2^LOGL.
+// row_off(f0 + s) = row_off(f0) + row_off(s) for s a multiple of a power of
+// two S > f0.
+template <int LOGL>
+__host__ __device__ constexpr int row_off(int f) {
+    return (f >> LOGL) * Lay<LOGL>::STRIDE + pidx(f & ((1 << LOGL) - 1));
+}
+// Offset of element f of a strided tile: line f mod LINES, point f / LINES.
+// col_off(f0 + s) = col_off(f0) + col_off(s) for s a multiple of a power of
+// two S > f0.
+template <int LOGL, int LINES>
+__host__ __device__ constexpr int col_off(int f) {
+    return (f % LINES) * Lay<LOGL>::STRIDE + pidx(f / LINES);
+}
 
 // Roots of the tree come from the root table of one prime in global memory.
 // Line c of a tile starts at node h0 + c*hstep of the pass's first level, so
@@ -161,14 +183,15 @@ __device__ __forceinline__ void fwd_round(double* lds, Tw tw, double p, double p
         const int gl = g - c * GPL;
         const int b = gl / D;
         const int i = gl - b * D;
-        double* line = lds + c * LS;
-        const int base = b * (L >> L0) + i;
+        // base = b * (L >> L0) + i with i < D: point base + k*D sits at
+        // pidx(base) + pidx(k*D).
+        double* line = lds + c * LS + pidx(b * (L >> L0) + i);
         double x[RADIX];
 #pragma unroll
-        for (int k = 0; k < RADIX; ++k) x[k] = line[pidx(base + k * D)];
+        for (int k = 0; k < RADIX; ++k) x[k] = line[pidx(k * D)];
         fwd_group<RLOG, ROOT1 && L0 == 0>(x, nullptr, tw, c, L0, b, p, pinv);
 #pragma unroll
-        for (int k = 0; k < RADIX; ++k) line[pidx(base + k * D)] = x[k];
+        for (int k = 0; k < RADIX; ++k) line[pidx(k * D)] = x[k];
     }
 }
 
@@ -267,17 +290,18 @@ __device__ __forceinline__ void round16_w(double* lds, const double* w, double p
         const int gl = g - c * GPL;
         const int b = gl / D;
         const int i = gl - b * D;
-        double* line = lds + c * LS;
-        const int base = b * (L >> L0) + i;
+        // base = b * (L >> L0) + i with i < D: point base + k*D sits at
+        // pidx(base) + pidx(k*D).
+        double* line = lds + c * LS + pidx(b * (L >> L0) + i);
         double x[16];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) x[k] = line[pidx(base + k * D)];
+        for (int k = 0; k < 16; ++k) x[k] = line[pidx(k * D)];
         if constexpr (INV)
             inv_group16_w(x, w, p, pinv);
         else
             fwd_group16_w(x, w, p, pinv);
 #pragma unroll
-        for (int k = 0; k < 16; ++k) line[pidx(base + k * D)] = x[k];
+        for (int k = 0; k < 16; ++k) line[pidx(k * D)] = x[k];
     }
 }
 
@@ -294,14 +318,15 @@ __device__ __forceinline__ void inv_round(double* lds, Tw tw, double p, double p
         const int gl = g - c * GPL;
         const int b = gl / D;
         const int i = gl - b * D;
-        double* line = lds + c * LS;
-        const int base = b * (L >> L0) + i;
+        // base = b * (L >> L0) + i with i < D: point base + k*D sits at
+        // pidx(base) + pidx(k*D).
+        double* line = lds + c * LS + pidx(b * (L >> L0) + i);
         double x[RADIX];
 #pragma unroll
-        for (int k = 0; k < RADIX; ++k) x[k] = line[pidx(base + k * D)];
+        for (int k = 0; k < RADIX; ++k) x[k] = line[pidx(k * D)];
         inv_group<RLOG, ROOT1 && L0 == 0>(x, tw, c, L0, b, p, pinv);
 #pragma unroll
-        for (int k = 0; k < RADIX; ++k) line[pidx(base + k * D)] = x[k];
+        for (int k = 0; k < RADIX; ++k) line[pidx(k * D)] = x[k];
     }
 }
 
@@ -376,6 +401,7 @@ __device__ __forceinline__ void load_strided(double* lds, const double* g, uint6
     constexpr int LS = Lay<LOGL>::STRIDE;
     constexpr int TOTAL = LINES * L;
     constexpr int R = (TOTAL + NT - 1) / NT;
+    const int t0 = col_off<LOGL, LINES>(threadIdx.x);
     double v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -385,7 +411,7 @@ __device__ __forceinline__ void load_strided(double* lds, const double* g, uint6
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int f = threadIdx.x + r * NT;
-        if (TOTAL % NT == 0 || f < TOTAL) lds[(f % LINES) * LS + pidx(f / LINES)] = v[r];
+        if (TOTAL % NT == 0 || f < TOTAL) lds[t0 + col_off<LOGL, LINES>(r * NT)] = v[r];
     }
 }
 
@@ -396,11 +422,12 @@ __device__ __forceinline__ void store_strided(const double* lds, double* g, uint
     constexpr int LS = Lay<LOGL>::STRIDE;
     constexpr int TOTAL = LINES * L;
     constexpr int R = (TOTAL + NT - 1) / NT;
+    const int t0 = col_off<LOGL, LINES>(threadIdx.x);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int f = threadIdx.x + r * NT;
         if (TOTAL % NT != 0 && f >= TOTAL) continue;
-        double v = lds[(f % LINES) * LS + pidx(f / LINES)];
+        double v = lds[t0 + col_off<LOGL, LINES>(r * NT)];
         if constexpr (CANON) v = canonical(mulmod(v, tinv, p, pinv), p);
         g[(uint64_t)(f / LINES) * S + (f % LINES)] = v;
     }
@@ -423,6 +450,7 @@ __device__ __forceinline__ void pack_tile_as(double* lds, const Source& src, uin
     constexpr int BATCH = (R >= 4 && R % 4 == 0) ? 4 : 1;
     const uint64_t safe = pack_safe<KIND>(src);
     const uint64_t digits = src.digits;
+    const int t0 = col_off<LOGL, LINES>(threadIdx.x);
 #pragma unroll
     for (int r0 = 0; r0 < R; r0 += BATCH) {
         double v[BATCH];
@@ -446,7 +474,7 @@ __device__ __forceinline__ void pack_tile_as(double* lds, const Source& src, uin
 #pragma unroll
         for (int r = 0; r < BATCH; ++r) {
             const int f = threadIdx.x + (r0 + r) * NT;
-            if (TOTAL % NT == 0 || f < TOTAL) lds[(f % LINES) * LS + pidx(f / LINES)] = v[r];
+            if (TOTAL % NT == 0 || f < TOTAL) lds[t0 + col_off<LOGL, LINES>((r0 + r) * NT)] = v[r];
         }
     }
 }
@@ -529,13 +557,14 @@ __device__ __forceinline__ void load_rows(double* lds, const double* g) {
     for (int r = 0; r < R; ++r)
         if (PAIRS % NT == 0 || threadIdx.x + r * NT < PAIRS)
             v[r] = reinterpret_cast<const double2*>(g)[threadIdx.x + r * NT];
+    // Point 2t + 2rNT at row_off(2t) + row_off(2rNT); 2t is even, so 2t + 1
+    // sits right after it.
+    double* t0 = lds + row_off<LOGL>(2 * threadIdx.x);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (PAIRS % NT != 0 && threadIdx.x + r * NT >= PAIRS) continue;
-        const int f = 2 * (threadIdx.x + r * NT);
-        const int c = f / L, m = f % L;
-        lds[c * LS + pidx(m)] = v[r].x;
-        lds[c * LS + pidx(m + 1)] = v[r].y;
+        t0[row_off<LOGL>(2 * r * NT)] = v[r].x;
+        t0[row_off<LOGL>(2 * r * NT) + 1] = v[r].y;
     }
 }
 
@@ -545,14 +574,13 @@ __device__ __forceinline__ void store_rows(const double* lds, double* g) {
     constexpr int LS = Lay<LOGL>::STRIDE;
     constexpr int PAIRS = LINES * L / 2;
     constexpr int R = (PAIRS + NT - 1) / NT;
+    const double* t0 = lds + row_off<LOGL>(2 * threadIdx.x);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (PAIRS % NT != 0 && threadIdx.x + r * NT >= PAIRS) continue;
-        const int f = 2 * (threadIdx.x + r * NT);
-        const int c = f / L, m = f % L;
         double2 v;
-        v.x = lds[c * LS + pidx(m)];
-        v.y = lds[c * LS + pidx(m + 1)];
+        v.x = t0[row_off<LOGL>(2 * r * NT)];
+        v.y = t0[row_off<LOGL>(2 * r * NT) + 1];
         reinterpret_cast<double2*>(g)[threadIdx.x + r * NT] = v;
     }
 }
