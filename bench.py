@@ -335,6 +335,25 @@ def profile_summary(d: str, workload_name: str, steps: int) -> str:
         if "GRBM_GUI_ACTIVE" in c:
             row += f" clock_GHz {c['GRBM_GUI_ACTIVE'] / 8 / (us * 1e3):.2f}"
         lines.append(row)
+        if "SQ_INSTS_VALU_MUL_F64" in c and c.get("SQ_INSTS_VALU"):
+            # Instruction mix: FP64 FMA, MUL, ADD and transcendental-unit
+            # (rint) instructions, 32- and 64-bit integer ones, conversions,
+            # and the rest (moves, selects, bit operations).
+            v = c["SQ_INSTS_VALU"]
+            mix = {n: c.get("SQ_INSTS_VALU_" + k, 0.0) for n, k in
+                   (("fma64", "FMA_F64"), ("mul64", "MUL_F64"), ("add64", "ADD_F64"), ("trans64", "TRANS_F64"),
+                    ("int32", "INT32"), ("int64", "INT64"), ("cvt", "CVT"))}
+            mix["other"] = max(v - sum(mix.values()), 0.0)
+            lines.append(f"MIX {kind} valu {v:.0f} " + " ".join(f"{n} {x / v:.3f}" for n, x in mix.items()))
+        if "SQ_WAVE_CYCLES" in c and "SQ_WAIT_ANY" in c and c["SQ_WAVE_CYCLES"]:
+            # Where the waves' time goes (shares of SQ_WAVE_CYCLES): waiting
+            # on a counter (s_waitcnt), waiting to issue, issuing any
+            # instruction, issuing VALU, issuing LDS.
+            w = c["SQ_WAVE_CYCLES"]
+            lines.append(f"STALL {kind} wait_any {c['SQ_WAIT_ANY'] / w:.3f} wait_inst_any "
+                         f"{c.get('SQ_WAIT_INST_ANY', 0.0) / w:.3f} active_any {c.get('SQ_ACTIVE_INST_ANY', 0.0) / w:.3f} "
+                         f"active_valu {c.get('SQ_ACTIVE_INST_VALU', 0.0) / w:.3f} active_lds "
+                         f"{c.get('SQ_ACTIVE_INST_LDS', 0.0) / w:.3f} wait_inst_lds {c.get('SQ_WAIT_INST_LDS', 0.0) / w:.3f}")
         if b:
             lines.append(f"KERNEL {kind} {lps[kind]} {us:.2f} {b}")
         if traffic:
@@ -840,7 +859,7 @@ def gpu_workload(name: str, steps: int, warmup: int, replay_reps: int = 20, runs
         "transform_split": split,
         "transform_forms": form["forms"] if form else None,
         "steps": steps,
-        "ms_per_step": round(region_ms[0], 4),
+        "ms_per_step": round(float(np.median(region_ms)), 4),
         "ms_per_step_runs": [round(x, 4) for x in region_ms],
         "ms_per_step_median": round(float(np.median(region_ms)), 4),
         "ms_per_step_with_events": round(events_ms, 4),

@@ -115,65 +115,75 @@ __device__ __forceinline__ CarryFn shfl_up_fn(CarryFn f, int d) {
 
 // Reconstruction parameters of one workgroup: the single operation's, or
 // those of the batched product its item names.
+// Its pointers are typed as global memory: pointers that reach a kernel
+// inside structures are generic to the compiler, and generic (flat)
+// accesses also count against the LDS counter, so each LDS wait of the
+// reconstruction (its atomic accumulation) would wait for them as well.
+#define NTT_GLOBAL __attribute__((address_space(1)))
 struct CrtJob {
-    const double* residues;
+    const NTT_GLOBAL double* residues;
     uint64_t row_stride, coefficients, out_len;
-    uint64_t* out;
+    NTT_GLOBAL uint64_t* out;
     uint32_t width, magic, value_bits, block, blocks;
-    uint32_t *agg_base, *agg_thresh, *first_thresh, *ripple_end;
+    NTT_GLOBAL uint32_t *agg_base, *agg_thresh, *first_thresh, *ripple_end;
     uint32_t split_blocks;  // 0: no split
     uint64_t split_digits, block_stride;
-    const uint64_t *op_a, *op_b;  // schoolbook operands
+    const NTT_GLOBAL uint64_t *op_a, *op_b;  // schoolbook operands
     uint64_t len_a, len_b;
 };
+
+template <class T>
+__device__ __forceinline__ NTT_GLOBAL T* global_ptr(T* p) {
+    return (NTT_GLOBAL T*)p;
+}
 
 __device__ __forceinline__ CrtJob crt_job(const CrtArgs& a) {
     CrtJob j;
     if (a.batch == nullptr) {
-        j.residues = a.residues;
+        j.residues = global_ptr(a.residues);
         j.row_stride = a.row_stride;
         j.coefficients = a.coefficients;
         j.out_len = a.out_len;
-        j.out = a.out;
+        j.out = global_ptr(a.out);
         j.width = a.width;
         j.magic = a.width_magic;
         j.value_bits = a.value_bits;
         j.block = blockIdx.x;
         j.blocks = a.blocks;
-        j.agg_base = a.agg_base;
-        j.agg_thresh = a.agg_thresh;
-        j.first_thresh = a.first_thresh;
-        j.ripple_end = a.ripple_end;
+        j.agg_base = global_ptr(a.agg_base);
+        j.agg_thresh = global_ptr(a.agg_thresh);
+        j.first_thresh = global_ptr(a.first_thresh);
+        j.ripple_end = global_ptr(a.ripple_end);
         j.split_blocks = a.split_blocks;
         j.split_digits = a.split_digits;
         j.block_stride = a.block_stride;
-        j.op_a = a.op_a;
-        j.op_b = a.op_b;
+        j.op_a = global_ptr(a.op_a);
+        j.op_b = global_ptr(a.op_b);
         j.len_a = a.len_a;
         j.len_b = a.len_b;
         return j;
     }
     const BatchItem it = a.items[blockIdx.x];
     const BatchProduct& P = a.batch[it.product];
-    j.residues = (P.square || P.split_blocks) ? P.spec_a : P.spec_b;
+    j.residues = global_ptr((P.square || P.split_blocks) ? P.spec_a : P.spec_b);
     j.split_blocks = P.split_blocks;
     j.split_digits = P.block_digits;
     j.block_stride = P.block_stride;
     j.row_stride = P.row_stride;
     j.coefficients = P.coefficients;
     j.out_len = P.out_len;
-    j.out = P.out;
+    j.out = global_ptr(P.out);
     j.width = P.width;
     j.magic = P.width_magic;
     j.value_bits = P.value_bits;
     j.block = it.sub;
     j.blocks = P.blocks;
-    j.agg_base = P.carry;
-    j.agg_thresh = P.carry + P.blocks;
-    j.first_thresh = P.carry + 2 * (uint64_t)P.blocks;
-    j.ripple_end = P.carry + 3 * (uint64_t)P.blocks;
-    j.op_a = P.src_a.limbs;
-    j.op_b = P.src_b.limbs;
+    j.agg_base = global_ptr(P.carry);
+    j.agg_thresh = global_ptr(P.carry + P.blocks);
+    j.first_thresh = global_ptr(P.carry + 2 * (uint64_t)P.blocks);
+    j.ripple_end = global_ptr(P.carry + 3 * (uint64_t)P.blocks);
+    j.op_a = global_ptr(P.src_a.limbs);
+    j.op_b = global_ptr(P.src_b.limbs);
     j.len_a = P.src_a.limb_len;
     j.len_b = P.src_b.limb_len;
     return j;
@@ -293,8 +303,8 @@ __global__ __launch_bounds__(NT) void k_crt_product(CrtArgs a) {
             if (z1 >= J.split_blocks) z1 = J.split_blocks - 1;
             const uint64_t k1 = i - z1 * D;
             const bool two = z1 > 0 && k1 + D < J.row_stride;
-            const double* r1 = J.residues + z1 * J.block_stride + k1;
-            const double* r0 = two ? r1 - J.block_stride + D : r1;
+            const NTT_GLOBAL double* r1 = J.residues + z1 * J.block_stride + k1;
+            const NTT_GLOBAL double* r0 = two ? r1 - J.block_stride + D : r1;
 #pragma unroll
             for (int k = 0; k < NP; ++k) {
                 double v = 0.0;
@@ -393,8 +403,8 @@ __global__ __launch_bounds__(kSchoolThreads) void k_school(CrtArgs a) {
     __shared__ double sp0[kSchoolMaxShort], sp1[kSchoolMaxShort], sp2[kSchoolMaxShort];
     const CrtJob J = crt_job(a);
     const bool swap = J.len_b > J.len_a;
-    const uint64_t* pl = swap ? J.op_b : J.op_a;
-    const uint64_t* ps = swap ? J.op_a : J.op_b;
+    const NTT_GLOBAL uint64_t* pl = swap ? J.op_b : J.op_a;
+    const NTT_GLOBAL uint64_t* ps = swap ? J.op_a : J.op_b;
     const int64_t ll = (int64_t)(swap ? J.len_b : J.len_a);
     const int ls = (int)(swap ? J.len_a : J.len_b);  // 1 .. kSchoolMaxShort (host-checked)
     const int nch = (ls + kSchoolChunk - 1) / kSchoolChunk;
@@ -621,8 +631,8 @@ __global__ __launch_bounds__(apply_threads<BL>()) void k_carry_apply(CrtArgs a) 
             uint32_t cin_chunk = 0;
             if (j > 0) {
                 const uint32_t chunks = carry_chunks(J.blocks);
-                const uint32_t* cb = J.agg_base + 4ull * J.blocks;
-                const uint32_t* ct = cb + chunks;
+                const NTT_GLOBAL uint32_t* cb = J.agg_base + 4ull * J.blocks;
+                const NTT_GLOBAL uint32_t* ct = cb + chunks;
                 const uint32_t per = (j + 63) / 64;
                 const uint32_t s0 = lane * per;
                 const uint32_t s1 = s0 + per < j ? s0 + per : j;

@@ -210,7 +210,12 @@ __device__ __forceinline__ void last_round_direct(const double* lds, const Tw& t
     const int c = threadIdx.x % LINES;
     const double* line = lds + c * LS;
 #pragma unroll
-    for (int b = threadIdx.x / LINES; b < GPL; b += TPL) {
+    for (int b0 = threadIdx.x / LINES; b0 < GPL; b0 += TPL) {
+        // With LINES a multiple of the wave size every lane of a wave takes
+        // the same group, so its roots are one scalar load: they then wait
+        // on the scalar counter, not behind the vector stores of the tile's
+        // earlier primes (vector memory operations complete in issue order).
+        const int b = LINES % 64 == 0 ? __builtin_amdgcn_readfirstlane(b0) : b0;
         double x[RADIX];
 #pragma unroll
         for (int k = 0; k < RADIX; ++k) x[k] = line[pidx(b * RADIX) + k];

@@ -115,10 +115,17 @@ struct Tw {
     int cmask;  // line c is node h0 + (c & cmask) * hstep (several tiles of rows)
     int split;  // first local level whose roots are formed as products
     double p, pinv;
+    // Tables are read through the constant address space: no pass writes
+    // them, so a root every lane of a wave shares becomes one scalar load,
+    // which waits on its own counter instead of behind the wave's earlier
+    // vector loads and stores (those complete in issue order).
+    __device__ __forceinline__ double ld(uint64_t i) const {
+        return ((const __attribute__((address_space(4))) double*)table)[i];
+    }
     __device__ __forceinline__ double at(int c, int lvl, int node) const {
         const uint64_t h = (h0 + (uint64_t)((c & cmask) * hstep)) << lvl;
-        if (lvl >= split) return mulmod(table[node], table[h], p, pinv);
-        return table[h + node];
+        if (lvl >= split) return mulmod(ld(node), ld(h), p, pinv);
+        return ld(h + node);
     }
 };
 
@@ -451,6 +458,23 @@ __device__ __forceinline__ void pack_tile_as(double* lds, const Source& src, uin
     const uint64_t safe = pack_safe<KIND>(src);
     const uint64_t digits = src.digits;
     const int t0 = col_off<LOGL, LINES>(threadIdx.x);
+    if (n0 + (uint64_t)(L - 1) * S + (LINES - 1) < safe) {
+        // The whole tile lies inside the operand (one test for the tile):
+        // every load is issued before the first result is needed.
+        double v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int f = threadIdx.x + r * NT;
+            if (TOTAL % NT == 0 || f < TOTAL)
+                v[r] = pack_as<KIND, false>(src, n0 + (uint64_t)(f / LINES) * S + (f % LINES), pw, p, pinv);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int f = threadIdx.x + r * NT;
+            if (TOTAL % NT == 0 || f < TOTAL) lds[t0 + col_off<LOGL, LINES>(r * NT)] = v[r];
+        }
+        return;
+    }
 #pragma unroll
     for (int r0 = 0; r0 < R; r0 += BATCH) {
         double v[BATCH];
