@@ -19,14 +19,23 @@ __host__ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
 // reference's Packing::pack (src/limb/packing.rs:113-292,
 // src/arch/schedule/expression/packing.rs:68-109).
 
+// Limb i of an operand. On the device the load goes through the global
+// address space: operands reach batched kernels inside structures, where the
+// compiler would otherwise load them as generic (flat) accesses, which also
+// count against the LDS counter of the packing's LDS waits.
+__host__ inline uint64_t limb_at(const uint64_t* limbs, uint64_t i) { return limbs[i]; }
+__device__ __forceinline__ uint64_t limb_at(const uint64_t* limbs, uint64_t i) {
+    return ((const __attribute__((address_space(1))) uint64_t*)limbs)[i];
+}
+
 // 64 bits of the operand from bit `pos`, zero beyond its end. Both loads are
 // unconditional (clamped indices), so unrolled callers can issue them early.
 __host__ __device__ __forceinline__ uint64_t window64(const uint64_t* limbs, uint64_t len, uint64_t pos) {
     const uint64_t idx = pos >> 6;
     const uint32_t off = (uint32_t)(pos & 63);
     const uint64_t last = len - 1;
-    const uint64_t lo_raw = limbs[idx < last ? idx : last];
-    const uint64_t hi_raw = limbs[idx + 1 < last ? idx + 1 : last];
+    const uint64_t lo_raw = limb_at(limbs, idx < last ? idx : last);
+    const uint64_t hi_raw = limb_at(limbs, idx + 1 < last ? idx + 1 : last);
     const uint64_t lo = idx < len ? lo_raw : 0;
     const uint64_t hi = idx + 1 < len ? hi_raw : 0;
     return (lo >> off) | ((hi << 1) << (63 - off));
@@ -54,7 +63,7 @@ __host__ __device__ __forceinline__ double pack_w64(const Source& s, uint64_t n,
                                                     double pinv) {
     const uint32_t vb = s.value_bits;
     const uint64_t last = s.limb_len - 1;
-    const uint64_t limb = CLAMP ? s.limbs[n < last ? n : last] : s.limbs[n];
+    const uint64_t limb = CLAMP ? limb_at(s.limbs, n < last ? n : last) : limb_at(s.limbs, n);
     double r;
     if (vb > (uint32_t)kPartBits) {
         const int high_bits = (int)vb - kPartBits;
@@ -139,10 +148,10 @@ __host__ __device__ __forceinline__ void extract_parts(const Source& s, uint64_t
     for (int q = 0; q < W; ++q) {
         const uint64_t i = idx + q;
         if (CLAMP) {
-            const uint64_t raw = s.limbs[i < last ? i : last];
+            const uint64_t raw = limb_at(s.limbs, i < last ? i : last);
             w[q] = i < s.limb_len && n < s.digits ? raw : 0;
         } else {
-            w[q] = s.limbs[i];
+            w[q] = limb_at(s.limbs, i);
         }
     }
     w[W] = 0;
@@ -162,10 +171,10 @@ __host__ __device__ __forceinline__ double pack_parts(const Source& s, uint64_t 
     for (int q = 0; q < W; ++q) {
         const uint64_t i = idx + q;
         if (CLAMP) {
-            const uint64_t raw = s.limbs[i < last ? i : last];
+            const uint64_t raw = limb_at(s.limbs, i < last ? i : last);
             w[q] = i < s.limb_len ? raw : 0;
         } else {
-            w[q] = s.limbs[i];
+            w[q] = limb_at(s.limbs, i);
         }
     }
     w[W] = 0;
@@ -210,7 +219,7 @@ __host__ __device__ __forceinline__ void extract_as(const Source& s, uint64_t n,
         constexpr bool SIGNED = KIND == kPackW64Signed;
         const uint32_t vb = s.value_bits;
         const uint64_t last = s.limb_len - 1;
-        const uint64_t limb = CLAMP ? (n < s.digits ? s.limbs[n < last ? n : last] : 0) : s.limbs[n];
+        const uint64_t limb = CLAMP ? (n < s.digits ? limb_at(s.limbs, n < last ? n : last) : 0) : limb_at(s.limbs, n);
         if (vb > (uint32_t)kPartBits) {
             const int high_bits = (int)vb - kPartBits;
             d[0] = part_to_double(limb & ((1ull << kPartBits) - 1));

@@ -167,7 +167,7 @@ __global__ __launch_bounds__(NT) void k_fused(PassArgs a) {
         for (int r = 0; r < R; ++r) {
             const int m = threadIdx.x + r * NT;
             if (!FULL && m >= L) continue;
-            row[m] = canonical(mulmod(lds[t0 + row_off<LOGL>(r * NT)], tinv, p, pinv), p);
+            global_ptr(row)[m] = canonical(mulmod(lds[t0 + row_off<LOGL>(r * NT)], tinv, p, pinv), p);
         }
     } else {
         store_rows<LOGL, LINES, NT>(lds, row);

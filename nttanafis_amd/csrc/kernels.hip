@@ -115,11 +115,9 @@ __device__ __forceinline__ CarryFn shfl_up_fn(CarryFn f, int d) {
 
 // Reconstruction parameters of one workgroup: the single operation's, or
 // those of the batched product its item names.
-// Its pointers are typed as global memory: pointers that reach a kernel
-// inside structures are generic to the compiler, and generic (flat)
-// accesses also count against the LDS counter, so each LDS wait of the
-// reconstruction (its atomic accumulation) would wait for them as well.
-#define NTT_GLOBAL __attribute__((address_space(1)))
+// Its pointers are typed as global memory (NTT_GLOBAL): the reconstruction's
+// atomic accumulation waits on the LDS counter, which generic accesses
+// would hold up as well.
 struct CrtJob {
     const NTT_GLOBAL double* residues;
     uint64_t row_stride, coefficients, out_len;
@@ -132,10 +130,6 @@ struct CrtJob {
     uint64_t len_a, len_b;
 };
 
-template <class T>
-__device__ __forceinline__ NTT_GLOBAL T* global_ptr(T* p) {
-    return (NTT_GLOBAL T*)p;
-}
 
 __device__ __forceinline__ CrtJob crt_job(const CrtArgs& a) {
     CrtJob j;

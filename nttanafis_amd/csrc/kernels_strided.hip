@@ -89,7 +89,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_strided_waves(PassArgs a) {
     const int wave = threadIdx.x >> 6;
     const int cl = lane % WL, q = lane / WL;
     const int c = wave * WL + cl;  // line of the tile
-    double* g = j.data + (uint64_t)y * a.row_stride + H * span + lo0 + c;  // point m at g[m * S]
+    NTT_GLOBAL double* g = global_ptr(j.data + (uint64_t)y * a.row_stride + H * span + lo0 + c);  // point m at g[m * S]
     double* line = lds + c * LS;
     const Tw tw{(INV ? a.inv : a.fwd) + (uint64_t)y * a.table_stride, H + (a.node_base << a.level_start), 0, 0, 64, p,
                 pinv};
@@ -198,8 +198,9 @@ static constexpr int pack_primes_threads() {
 // g[m * S + c]): thread t takes line t % LINES, so each store instruction
 // writes LINES consecutive points of one row of the tile.
 template <int LOGL, int LINES, int NT, int L0>
-__device__ __forceinline__ void last_round_direct(const double* lds, const Tw& tw, double* g, uint64_t S, double p,
+__device__ __forceinline__ void last_round_direct(const double* lds, const Tw& tw, double* g0, uint64_t S, double p,
                                                   double pinv) {
+    NTT_GLOBAL double* g = global_ptr(g0);
     constexpr int L = 1 << LOGL;
     constexpr int RL = LOGL - L0;
     constexpr int RADIX = 1 << RL;

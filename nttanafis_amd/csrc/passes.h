@@ -54,6 +54,18 @@ inline void prof_remember(const char* kind, const void* fn, dim3 grid, dim3 bloc
         }                                                                                  \
     } while (0)
 
+// Global memory. Pointers that reach a kernel inside structures (a batch's
+// products, a reconstruction's job) are generic to the compiler, and generic
+// (flat) accesses count against the LDS counter as well as the vector memory
+// counter, so each LDS wait after them would also wait for them (a store
+// included). Every pointer the kernels read or write rows through addresses
+// global memory, so the tile movements below cast to it.
+#define NTT_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ NTT_GLOBAL T* global_ptr(T* p) {
+    return (NTT_GLOBAL T*)p;
+}
+
 // Padded LDS index: one pad word after every 16 values. The last round of a
 // pass gives thread t the points 16t..16t+15 (or 8t..8t+7 in radix 8), so
 // the 16 lanes of one LDS cycle read values 16 (or 8) apart; with a pad per
@@ -403,7 +415,8 @@ __device__ __forceinline__ void inv_rounds8(double* lds, Tw tw, double p, double
 // at g[m*S + c]. Consecutive threads take consecutive lines, so each group of
 // LINES threads reads one LINES*8-byte run.
 template <int LOGL, int LINES, int NT>
-__device__ __forceinline__ void load_strided(double* lds, const double* g, uint64_t S) {
+__device__ __forceinline__ void load_strided(double* lds, const double* g0, uint64_t S) {
+    const NTT_GLOBAL double* g = global_ptr(g0);
     constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
     constexpr int TOTAL = LINES * L;
@@ -423,8 +436,9 @@ __device__ __forceinline__ void load_strided(double* lds, const double* g, uint6
 }
 
 template <int LOGL, int LINES, int NT, bool CANON>
-__device__ __forceinline__ void store_strided(const double* lds, double* g, uint64_t S, double tinv, double p,
+__device__ __forceinline__ void store_strided(const double* lds, double* g0, uint64_t S, double tinv, double p,
                                               double pinv) {
+    NTT_GLOBAL double* g = global_ptr(g0);
     constexpr int L = 1 << LOGL;
     constexpr int LS = Lay<LOGL>::STRIDE;
     constexpr int TOTAL = LINES * L;
@@ -568,6 +582,10 @@ __device__ __forceinline__ void pack_strided(double* lds, const Source& src, uin
     }
 }
 
+// Two doubles moved with one 16-byte access (a native vector type, which
+// loads and stores through a global pointer as one value).
+typedef double Pair __attribute__((ext_vector_type(2)));
+
 // Row tile: LINES rows of L contiguous points; thread t moves points 2t, 2t+1
 // of each 2*NT-point stripe with 16-byte accesses.
 template <int LOGL, int LINES, int NT>
@@ -576,11 +594,11 @@ __device__ __forceinline__ void load_rows(double* lds, const double* g) {
     constexpr int LS = Lay<LOGL>::STRIDE;
     constexpr int PAIRS = LINES * L / 2;
     constexpr int R = (PAIRS + NT - 1) / NT;
-    double2 v[R];
+    const NTT_GLOBAL Pair* gp = global_ptr(reinterpret_cast<const Pair*>(g));
+    Pair v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r)
-        if (PAIRS % NT == 0 || threadIdx.x + r * NT < PAIRS)
-            v[r] = reinterpret_cast<const double2*>(g)[threadIdx.x + r * NT];
+        if (PAIRS % NT == 0 || threadIdx.x + r * NT < PAIRS) v[r] = gp[threadIdx.x + r * NT];
     // Point 2t + 2rNT at row_off(2t) + row_off(2rNT); 2t is even, so 2t + 1
     // sits right after it.
     double* t0 = lds + row_off<LOGL>(2 * threadIdx.x);
@@ -602,10 +620,10 @@ __device__ __forceinline__ void store_rows(const double* lds, double* g) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (PAIRS % NT != 0 && threadIdx.x + r * NT >= PAIRS) continue;
-        double2 v;
+        Pair v;
         v.x = t0[row_off<LOGL>(2 * r * NT)];
         v.y = t0[row_off<LOGL>(2 * r * NT) + 1];
-        reinterpret_cast<double2*>(g)[threadIdx.x + r * NT] = v;
+        global_ptr(reinterpret_cast<Pair*>(g))[threadIdx.x + r * NT] = v;
     }
 }
 
