@@ -218,7 +218,7 @@ def kernel_kind(name: str):
     if not m:
         return None
     k, args = m.group(1), [x.strip() for x in (m.group(2) or "").split(",")]
-    if k == "k_pack_primes":
+    if k in ("k_pack_primes", "k_pack_primes_batch"):
         return "fwd_pack_strided"
     if k == "k_fwd_strided":
         return "fwd_pack_strided" if args[3] == "true" else "fwd_strided"
