@@ -112,10 +112,69 @@ struct DevBuf {
     }
 };
 
+// Pinned (page-locked) host memory, grow-only.
+struct HostBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    bool ensure(size_t n) {
+        if (n <= bytes) return true;
+        release();
+        size_t want = std::max<size_t>(n, 4096);
+        if (hipHostMalloc(&ptr, want, hipHostMallocDefault) != hipSuccess) {
+            ptr = nullptr;
+            return false;
+        }
+        bytes = want;
+        return true;
+    }
+    void release() {
+        if (ptr) hipHostFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+// True when p lies in page-locked host memory (hipHostMalloc), which the
+// DMA engines read and write directly.
+bool host_pinned(const void* p);
+
+// Copies between pageable host memory and the device through a ring of
+// pinned chunk buffers: the host copies chunk k into one buffer while the
+// DMA engine moves chunk k-1 out of the other, so the two overlap. The
+// runtime's own pageable copies stage through buffers too, but with a
+// device-to-host rate of ~15 GB/s against ~24 GB/s here (16 MiB, MI355X
+// host), and a first call that maps its staging memory.
+struct HostStage {
+    static constexpr int kBuffers = 2;
+    static constexpr size_t kChunk = (size_t)4 << 20;
+    HostBuf buf[kBuffers];
+    hipEvent_t ev[kBuffers] = {};
+    bool init();
+    // Host to device, queued on s; returns once every chunk has been copied
+    // out of `host` (the last DMAs may still run: later calls wait for them).
+    bool to_device(hipStream_t s, void* dev, const void* host, size_t bytes);
+    // Device to host after the work queued on s; returns when `host` holds
+    // every byte.
+    bool to_host(hipStream_t s, void* host, const void* dev, size_t bytes);
+    void release();
+    ~HostStage() { release(); }
+};
+
+// Host copies of an executor's calls: straight DMA for pinned host memory,
+// the staging ring otherwise (and straight copies when the ring cannot be
+// allocated).
+bool copy_to_device(HostStage& st, hipStream_t s, void* dev, const void* host, size_t bytes);
+bool copy_to_host(HostStage& st, hipStream_t s, void* host, const void* dev, size_t bytes);
+
 struct Executor {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     DevBuf spec_a, spec_b, in_a, in_b, out, carry;
+    HostStage stage;
     std::mutex lock;
 
     bool init() {
@@ -246,6 +305,10 @@ struct Batch {
     ~Batch() {
         drop_graph();
         for (hipStream_t x : side) hipStreamDestroy(x);
+        products.release();
+        items.release();
+        arena.release();
+        carry.release();
     }
     void drop_graph() {
         if (exec) hipGraphExecDestroy(exec);

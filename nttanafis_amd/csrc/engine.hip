@@ -1268,6 +1268,95 @@ static void tune_split(Executor& ex, int K, int use, const std::function<bool()>
 // ---------------------------------------------------------------------------
 // Host-buffer calls: operands to device, run, result back, dst written last.
 
+bool host_pinned(const void* p) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory is not an error here
+        return false;
+    }
+    return at.type == hipMemoryTypeHost;
+}
+
+bool HostStage::init() {
+    if (ev[kBuffers - 1]) return true;
+    for (int i = 0; i < kBuffers; ++i) {
+        if (!buf[i].ensure(kChunk)) {
+            release();
+            return false;
+        }
+    }
+    for (int i = 0; i < kBuffers; ++i) {
+        if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) {
+            ev[i] = nullptr;
+            release();
+            return false;
+        }
+    }
+    return true;
+}
+
+void HostStage::release() {
+    for (int i = 0; i < kBuffers; ++i) {
+        if (ev[i]) hipEventDestroy(ev[i]);
+        ev[i] = nullptr;
+        buf[i].release();
+    }
+}
+
+bool HostStage::to_device(hipStream_t s, void* dev, const void* host, size_t bytes) {
+    const char* src = static_cast<const char*>(host);
+    char* dst = static_cast<char*>(dev);
+    int k = 0;
+    for (size_t o = 0; o < bytes; o += kChunk, k = (k + 1) % kBuffers) {
+        const size_t m = bytes - o < kChunk ? bytes - o : kChunk;
+        // The buffer's previous DMA (this call's or an earlier one's) must
+        // have finished before the buffer is refilled.
+        if (hipEventSynchronize(ev[k]) != hipSuccess) return false;
+        memcpy(buf[k].ptr, src + o, m);
+        if (hipMemcpyAsync(dst + o, buf[k].ptr, m, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipEventRecord(ev[k], s) != hipSuccess)
+            return false;
+    }
+    return true;
+}
+
+bool HostStage::to_host(hipStream_t s, void* host, const void* dev, size_t bytes) {
+    char* dst = static_cast<char*>(host);
+    const char* src = static_cast<const char*>(dev);
+    const size_t chunks = (bytes + kChunk - 1) / kChunk;
+    auto queue = [&](size_t c) {
+        const size_t o = c * kChunk, m = bytes - o < kChunk ? bytes - o : kChunk;
+        const int k = (int)(c % kBuffers);
+        return hipMemcpyAsync(buf[k].ptr, src + o, m, hipMemcpyDeviceToHost, s) == hipSuccess &&
+               hipEventRecord(ev[k], s) == hipSuccess;
+    };
+    for (size_t c = 0; c < chunks && c < (size_t)kBuffers; ++c)
+        if (!queue(c)) return false;
+    for (size_t c = 0; c < chunks; ++c) {
+        const int k = (int)(c % kBuffers);
+        const size_t o = c * kChunk, m = bytes - o < kChunk ? bytes - o : kChunk;
+        if (hipEventSynchronize(ev[k]) != hipSuccess) return false;
+        memcpy(dst + o, buf[k].ptr, m);
+        if (c + kBuffers < chunks && !queue(c + kBuffers)) return false;
+    }
+    return true;
+}
+
+bool copy_to_device(HostStage& st, hipStream_t s, void* dev, const void* host, size_t bytes) {
+    if (bytes == 0) return true;
+    if (bytes <= HostStage::kChunk / 8 || host_pinned(host) || !st.init())
+        return hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+    return st.to_device(s, dev, host, bytes);
+}
+
+bool copy_to_host(HostStage& st, hipStream_t s, void* host, const void* dev, size_t bytes) {
+    if (bytes == 0) return hipStreamSynchronize(s) == hipSuccess;
+    if (bytes <= HostStage::kChunk / 8 || host_pinned(host) || !st.init())
+        return hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s) == hipSuccess &&
+               hipStreamSynchronize(s) == hipSuccess;
+    return st.to_host(s, host, dev, bytes);
+}
+
 static bool stage_and_run(Executor& ex, uint64_t out_len, const uint64_t* a, uint64_t la,
                           const uint64_t* b, uint64_t lb, bool same,
                           bool (*run)(Executor&, void*, const uint64_t*, const uint64_t*, uint64_t*),
@@ -1284,8 +1373,10 @@ static bool stage_and_run(Executor& ex, uint64_t out_len, const uint64_t* a, uin
     const hipStream_t s = ex.stream;
     uint64_t* da = ex.in_a.as<uint64_t>();
     uint64_t* db = same ? da : ex.in_b.as<uint64_t>();
-    if (hipMemcpyAsync(da, a, la * 8, hipMemcpyHostToDevice, s) != hipSuccess) return false;
-    if (!same && hipMemcpyAsync(db, b, lb * 8, hipMemcpyHostToDevice, s) != hipSuccess) return false;
+    if (!copy_to_device(ex.stage, s, da, a, la * 8) || (!same && !copy_to_device(ex.stage, s, db, b, lb * 8))) {
+        set_error("operand copy failed");
+        return false;
+    }
     uint64_t* dout = ex.out.as<uint64_t>();
     if (!run(ex, ctx, da, db, dout)) {
         hipStreamSynchronize(s);
@@ -1298,8 +1389,7 @@ static bool stage_and_run(Executor& ex, uint64_t out_len, const uint64_t* a, uin
         set_error("device execution failed");
         return false;
     }
-    if (hipMemcpyAsync(dst, dout, out_len * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
+    if (!copy_to_host(ex.stage, s, dst, dout, out_len * 8)) {
         set_error("result copy failed");
         return false;
     }
@@ -2193,6 +2283,7 @@ struct ntt_batch {
     std::vector<const uint64_t*> d_a, d_b;
     std::vector<uint64_t*> d_out;
     DevBuf staging_in, staging_out;  // host-operand convenience call
+    HostBuf host_in, host_out;       // its operands and results, gathered in pinned memory
     ntt_mul_workspace* plain = nullptr;
 };
 
@@ -2515,6 +2606,10 @@ ntt_batch* ntt_batch_new(void) { return new (std::nothrow) ntt_batch(); }
 void ntt_batch_free(ntt_batch* b) {
     if (!b) return;
     if (b->plain) ntt_mul_workspace_free(b->plain);
+    b->staging_in.release();
+    b->staging_out.release();
+    b->host_in.release();
+    b->host_out.release();
     delete b;
 }
 
@@ -2599,9 +2694,13 @@ bool ntt_batch_mul_host(ntt_batch* b, const uint64_t* const* a, const uint64_t* 
         set_error("batch: staging allocation failed");
         return false;
     }
+    if (!b->host_in.ensure(in_limbs * 8) || !b->host_out.ensure(out_limbs * 8)) {
+        set_error("batch: pinned staging allocation failed");
+        return false;
+    }
     std::vector<const uint64_t*> da(n), db(n);
     std::vector<uint64_t*> dout(n);
-    std::vector<uint64_t> host_in(in_limbs);
+    uint64_t* host_in = b->host_in.as<uint64_t>();
     uint64_t* din = b->staging_in.as<uint64_t>();
     uint64_t* dst = b->staging_out.as<uint64_t>();
     uint64_t ofs = 0, oofs = 0;
@@ -2620,15 +2719,15 @@ bool ntt_batch_mul_host(ntt_batch* b, const uint64_t* const* a, const uint64_t* 
         dout[i] = dst + oofs;
         oofs += la + lb;
     }
-    if (hipMemcpy(din, host_in.data(), in_limbs * 8, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(din, host_in, in_limbs * 8, hipMemcpyHostToDevice) != hipSuccess) {
         set_error("batch: operand upload failed");
         return false;
     }
     if (!ntt_batch_bind(b, da.data(), db.data(), dout.data())) return false;
     if (!ntt_batch_run(b, nullptr)) return false;
-    std::vector<uint64_t> host_out(out_limbs);
+    uint64_t* host_out = b->host_out.as<uint64_t>();
     if (hipStreamSynchronize(b->batch.ex.stream) != hipSuccess ||
-        hipMemcpy(host_out.data(), dst, out_limbs * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+        hipMemcpy(host_out, dst, out_limbs * 8, hipMemcpyDeviceToHost) != hipSuccess) {
         set_error("batch: device execution failed");
         return false;
     }

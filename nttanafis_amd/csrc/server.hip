@@ -113,35 +113,17 @@ struct Request {
     bool waited = false;  // has waited for device memory (counted once)
 };
 
-// Pinned host staging of one slot (grow-only).
-struct HostStage {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-    bool ensure(size_t n) {
-        if (n <= bytes) return true;
-        if (ptr) hipHostFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-        if (hipHostMalloc(&ptr, n, hipHostMallocDefault) != hipSuccess) {
-            ptr = nullptr;
-            return false;
-        }
-        bytes = n;
-        return true;
-    }
-    ~HostStage() {
-        if (ptr) hipHostFree(ptr);
-    }
-    template <class T>
-    T* as() const {
-        return static_cast<T*>(ptr);
-    }
-};
-
 struct Slot {
     Batch batch;  // its stream is the slot's stream
     DevBuf dev_in, dev_out;
-    HostStage host_in, host_out;
+    HostBuf host_in, host_out;  // a group's operands and results, gathered
+    HostStage stage;            // a large request's pageable host buffers
+    ~Slot() {
+        dev_in.release();
+        dev_out.release();
+        host_in.release();
+        host_out.release();
+    }
 };
 
 }  // namespace
@@ -289,13 +271,13 @@ bool Server::run_large(Slot& slot, Request* q) {
     };
     if (!q->device) {
         uint64_t* ia = reinterpret_cast<uint64_t*>(carve(la * 8));
-        if (hipMemcpyAsync(ia, r.a, la * 8, hipMemcpyHostToDevice, s) != hipSuccess) return false;
+        if (!copy_to_device(slot.stage, s, ia, r.a, la * 8)) return false;
         da = ia;
         if (q->square) {
             db = ia;
         } else {
             uint64_t* ib = reinterpret_cast<uint64_t*>(carve(lb * 8));
-            if (hipMemcpyAsync(ib, r.b, lb * 8, hipMemcpyHostToDevice, s) != hipSuccess) return false;
+            if (!copy_to_device(slot.stage, s, ib, r.b, lb * 8)) return false;
             db = ib;
         }
         dout = reinterpret_cast<uint64_t*>(carve(q->out_len * 8));
@@ -317,9 +299,8 @@ bool Server::run_large(Slot& slot, Request* q) {
                          : product_on(s, q->g, da, la, db, lb, q->square, buf, dout);
     // dst is written only once every kernel of the request has completed.
     if (hipStreamSynchronize(s) != hipSuccess || !ok) return false;
-    if (!q->device && hipMemcpyAsync(r.dst, dout, q->out_len * 8, hipMemcpyDeviceToHost, s) != hipSuccess)
-        return false;
-    return hipStreamSynchronize(s) == hipSuccess;
+    if (!q->device) return copy_to_host(slot.stage, s, r.dst, dout, q->out_len * 8);
+    return true;
 }
 
 // A group of small products as one batch: operands gathered into pinned

@@ -1,0 +1,94 @@
+"""Host-buffer calls through the pinned staging ring.
+
+Pageable host operands and results larger than an eighth of a staging chunk
+(4 MiB) move through two pinned chunk buffers; pinned arrays and short
+buffers are copied directly. Both routes must give the oracle's product,
+including sizes that end inside a chunk, a destination longer than the
+product (its suffix untouched), a convolution and a square.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import nttanafis_amd as ntt
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    ntt.load()
+    assert ntt.device_available(), "no gfx950 device visible"
+    yield
+
+
+def pinned_copy(x):
+    y = ntt.pinned_array(len(x))
+    y[:] = x
+    return y
+
+
+# 524,288 limbs fill one 4 MiB chunk exactly; the others end inside one.
+@pytest.mark.parametrize("la,lb", [(524288, 524288), (600001, 333333), (1048577, 65536), (70000, 70001)])
+def test_pageable_and_pinned_host_buffers_match_oracle(oracle, la, lb):
+    say(f"host buffers {la} x {lb}")
+    a, b = oracle.operand(la, 11), oracle.operand(lb, 12)
+    want = oracle.mul(a, b, threads=4)
+    plan = ntt.MulPlan.new(la, lb)
+    dst = np.full(la + lb + 5, 7, dtype=np.uint64)
+    assert plan.mul(dst, a, b), ntt.last_error()
+    assert (dst[:la + lb] == want).all()
+    assert (dst[la + lb:] == 7).all()
+    pa, pb = pinned_copy(a), pinned_copy(b)
+    pdst = ntt.pinned_array(la + lb)
+    assert plan.mul(pdst, pa, pb), ntt.last_error()
+    assert (pdst == want).all()
+    # Mixed: pinned operands, pageable result and the reverse.
+    dst[:] = 0
+    assert plan.mul(dst[:la + lb], pa, pb), ntt.last_error()
+    assert (dst[:la + lb] == want).all()
+    pdst[:] = 0
+    assert plan.mul(pdst, a, b), ntt.last_error()
+    assert (pdst == want).all()
+
+
+def test_pageable_square_and_convolution(oracle):
+    say("host buffers: square 700,000 limbs, 16-bit convolution of 2^20 terms")
+    a = oracle.operand(700000, 13)
+    plan = ntt.MulPlan.new(len(a), len(a))
+    dst = np.zeros(2 * len(a), dtype=np.uint64)
+    assert plan.sqr(dst, a), ntt.last_error()
+    assert (dst == oracle.mul(a, a, threads=4)).all()
+    x = oracle.operand(1 << 20, 14) & np.uint64(0xFFFF)
+    y = oracle.operand(1 << 20, 15) & np.uint64(0xFFFF)
+    cplan = ntt.ConvPlan.new(len(x), len(y), 65535, 65535, False)
+    out = np.zeros(cplan.output_len(), dtype=np.uint64)
+    assert cplan.convolve(out, x, y), ntt.last_error()
+    assert (out == oracle.convolve(x, y, 65535, 65535, False)).all()
+    # The same convolution from pinned buffers (the direct route).
+    px, py, pout = pinned_copy(x), pinned_copy(y), ntt.pinned_array(cplan.output_len())
+    assert cplan.convolve(pout, px, py), ntt.last_error()
+    assert (out == pout).all()
+
+
+def test_repeated_calls_reuse_the_ring(oracle):
+    """Calls back to back (the ring's buffers are refilled only after their
+    previous DMA finished) and a short call between long ones."""
+    say("host buffers: repeated calls")
+    plan = ntt.MulPlan.new(800000, 800000)
+    small = ntt.MulPlan.new(1000, 999)
+    for seed in range(3):
+        a, b = oracle.operand(800000, 20 + seed), oracle.operand(800000, 30 + seed)
+        dst = np.zeros(1600000, dtype=np.uint64)
+        assert plan.mul(dst, a, b), ntt.last_error()
+        assert (dst == oracle.mul(a, b, threads=4)).all(), seed
+        c, d = oracle.operand(1000, seed), oracle.operand(999, seed + 1)
+        sdst = np.zeros(1999, dtype=np.uint64)
+        assert small.mul(sdst, c, d), ntt.last_error()
+        assert (sdst == oracle.mul(c, d)).all(), seed
