@@ -178,7 +178,7 @@ struct CrtArgs {
     // Batched reconstruction: items[blockIdx.x] names product and block;
     // scan_items (scan_count of them) name the (product, chunk) pairs of the
     // products whose carries go through k_carry_scan (those of more than
-    // kCarryLookBackBlocks blocks).
+    // kCarryDirectBlocks blocks).
     const BatchProduct* batch;
     const BatchItem* items;
     const BatchItem* scan_items;
@@ -237,13 +237,14 @@ __host__ __device__ constexpr int alt_strided_lines(int logl) {
 __host__ __device__ constexpr int tile_lines(int logl, bool alt) { return alt ? alt_strided_lines(logl) : strided_lines(logl); }
 
 constexpr uint32_t kInf = 0xFFFFFFFFu;
-// Carries across the blocks of an output: up to kCarryLookBackBlocks blocks
-// each block looks back over the published block functions itself (at most
-// that many steps); longer outputs first scan their block functions in chunks
-// of kCarryChunk blocks, one workgroup per chunk.
-constexpr uint32_t kCarryLookBackBlocks = 64;
+// Carries across the blocks of an output: up to kCarryDirectBlocks blocks,
+// each block composes the published functions of the blocks before it
+// itself, spread over its threads (at most kCarryDirectBlocks / 256 per
+// thread); longer outputs first scan their block functions in chunks of
+// kCarryChunk blocks, one workgroup per chunk.
+constexpr uint32_t kCarryDirectBlocks = 2048;
 constexpr int kCarryChunk = 1024;
-__host__ __device__ __forceinline__ bool carry_scanned(uint32_t blocks) { return blocks > kCarryLookBackBlocks; }
+__host__ __device__ __forceinline__ bool carry_scanned(uint32_t blocks) { return blocks > kCarryDirectBlocks; }
 __host__ __device__ __forceinline__ uint32_t carry_chunks(uint32_t blocks) {
     return (blocks + kCarryChunk - 1) / kCarryChunk;
 }

@@ -285,7 +285,7 @@ struct Batch {
     // schoolbook products.
     uint32_t crt_begin[kMaxPrimes + 1] = {0}, crt_count[kMaxPrimes + 1] = {0};
     // Products of each prime count whose carries take the scan (more than
-    // kCarryLookBackBlocks output blocks): one item per chunk of carry blocks.
+    // kCarryDirectBlocks output blocks): one item per chunk of carry blocks.
     uint32_t scan_begin[kMaxPrimes + 1] = {0}, scan_count[kMaxPrimes + 1] = {0};
     DevBuf products, items, arena, carry;
     std::vector<BatchLaunch> program;

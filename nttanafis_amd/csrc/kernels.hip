@@ -524,7 +524,7 @@ __global__ __launch_bounds__(kSchoolThreads) void k_school(CrtArgs a) {
 // F_{b-1}(F_{b-2}(... F_0(0))) for the published carry functions F (the block
 // merge of src/limb/reconstruction.rs:211-247).
 //
-// An output of more than kCarryLookBackBlocks blocks first runs k_carry_scan,
+// An output of more than kCarryDirectBlocks blocks first runs k_carry_scan,
 // one workgroup per chunk of kCarryChunk blocks (one block per thread): it
 // replaces each block's function by the composition of the functions before
 // it in its chunk (the first block of a chunk gets the identity, marked by a
@@ -532,10 +532,10 @@ __global__ __launch_bounds__(kSchoolThreads) void k_school(CrtArgs a) {
 // composed function. The apply kernel then composes the functions of the
 // chunks before its own (at most a few per lane of its wave) and applies its
 // block's prefix: O(blocks) work for any carry pattern, all-ones products
-// included. A shorter output skips the scan: each block looks back to the
-// nearest block whose function has no threshold (its output is fixed whatever
-// comes in) and applies the functions after it, at most kCarryLookBackBlocks
-// steps.
+// included. A shorter output skips the scan (and its launch): block b
+// composes the functions of blocks 0..b-1 itself, each thread a contiguous
+// slice of at most kCarryDirectBlocks / NT of them, the slices in order by a
+// scan over each wave and the wave results in order.
 //
 // Then the first limb of the block takes the carry, and the limbs up to the
 // recorded ripple end take one more when the carry reaches the first limb's
@@ -600,17 +600,18 @@ __global__ __launch_bounds__(kCarryChunk) void k_carry_scan(CrtArgs a) {
     }
 }
 
-// Threads of an apply workgroup: one wave finds the carry into the block,
-// then all of them add the ripple (up to BL - 1 limbs).
+// Threads of an apply workgroup: they find the carry into the block
+// together (one wave of them after a scan), then add the ripple (up to
+// BL - 1 limbs).
 template <int BL>
 __host__ __device__ constexpr int apply_threads() {
-    return BL >= 1024 ? 256 : 64;
+    return 256;
 }
 
 template <int BL>
 __global__ __launch_bounds__(apply_threads<BL>()) void k_carry_apply(CrtArgs a) {
     constexpr int NT = apply_threads<BL>();
-    __shared__ uint32_t found;
+    __shared__ CarryFn wave_fn[NT / 64];
     __shared__ uint32_t cin_s;
     const CrtJob J = crt_job(a);
     const int b = (int)J.block;
@@ -648,18 +649,41 @@ __global__ __launch_bounds__(apply_threads<BL>()) void k_carry_apply(CrtArgs a) 
             if (lane == 0) cin_s = pt == 0 ? cin_chunk : pb + (cin_chunk >= pt ? 1u : 0u);
         }
     } else {
-        // b <= kCarryLookBackBlocks = 64: one look-back step by wave 0.
-        if (threadIdx.x == 0) found = 0xFFFFFFFFu;
+        // b <= kCarryDirectBlocks: thread t composes the functions of blocks
+        // [t*per, t*per + per) below b (the threads with a slice are a
+        // prefix of the workgroup), each wave scans its threads' functions
+        // in order, and thread 0 applies the waves' results in order to the
+        // zero carry into block 0.
+        // Every load of the slice is issued before the first composition.
+        constexpr int PER_MAX = ((int)kCarryDirectBlocks + NT - 1) / NT;
+        const uint32_t per = ((uint32_t)b + NT - 1) / NT;
+        const uint32_t s0 = threadIdx.x * per;
+        const uint32_t s1 = s0 + per < (uint32_t)b ? s0 + per : (uint32_t)b;
+        CarryFn fs[PER_MAX];
+#pragma unroll
+        for (int i = 0; i < PER_MAX; ++i)
+            if (s0 + i < s1) fs[i] = CarryFn{J.agg_base[s0 + i], J.agg_thresh[s0 + i]};
+        CarryFn agg = fs[0];
+#pragma unroll
+        for (int i = 1; i < PER_MAX; ++i)
+            if (s0 + i < s1) agg = compose(agg, fs[i]);
+        if (s0 >= s1) agg = CarryFn{0u, kInf};
+        const int wl = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const CarryFn prev = shfl_up_fn(agg, d);
+            if (wl >= d) agg = compose(prev, agg);
+        }
+        // Threads with a slice: nv = ceil(b / per); wave w holds nv - 64w of
+        // them (clamped to 0..64), its result at its last such lane.
+        const int nv = (int)(((uint32_t)b + per - 1) / per);
+        const int in_wave = nv - 64 * wave < 0 ? 0 : (nv - 64 * wave > 64 ? 64 : nv - 64 * wave);
+        if (in_wave > 0 && wl == in_wave - 1) wave_fn[wave] = agg;
         __syncthreads();
-        const int k = b - 1 - lane;
-        if (lane < 64 && k >= 0 && J.agg_thresh[k] == kInf) atomicMin(&found, (uint32_t)lane);
-        __syncthreads();
-        const uint32_t t = found;
-        // Block whose output is fixed, or -1: begin with a zero carry into block 0.
-        const int start = t != 0xFFFFFFFFu ? b - 1 - (int)t : -1;
         if (threadIdx.x == 0) {
-            uint32_t cc = start >= 0 ? J.agg_base[start] : 0u;
-            for (int q = start + 1; q < b; ++q) cc = J.agg_base[q] + (cc >= J.agg_thresh[q] ? 1u : 0u);
+            const int waves = (nv + 63) / 64;
+            uint32_t cc = 0;
+            for (int w = 0; w < waves; ++w) cc = wave_fn[w].base + (cc >= wave_fn[w].thresh ? 1u : 0u);
             cin_s = cc;
         }
     }

@@ -118,8 +118,12 @@ def test_squares_match_oracle(oracle, n):
     assert (engine_mul(a, a) == expect).all()
 
 
+# The last four shapes sit at the edge of the carry pass's direct form
+# (2,048 output blocks of 1,024 limbs, or of 64 limbs for a schoolbook
+# product) and one block past it, where the block functions are scanned first.
 @pytest.mark.parametrize("n,m", [(1, 1), (7, 3), (700, 701), (1024, 1024), (1500, 548), (5000, 17), (65536, 65536),
-                                 (300000, 200000)])
+                                 (300000, 200000), (1 << 20, 1 << 20), ((1 << 20) + 1024, 1 << 20), (8, 131064),
+                                 (8, 131128)])
 def test_all_ones_carry_chains(n, m):
     say(f"all ones {n}x{m}")
     a = np.full(n, MASK, dtype=np.uint64)
