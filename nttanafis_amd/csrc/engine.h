@@ -143,13 +143,15 @@ struct HostBuf {
 bool host_pinned(const void* p);
 
 // Copies between pageable host memory and the device through a ring of
-// pinned chunk buffers: the host copies chunk k into one buffer while the
-// DMA engine moves chunk k-1 out of the other, so the two overlap. The
-// runtime's own pageable copies stage through buffers too, but with a
-// device-to-host rate of ~15 GB/s against ~24 GB/s here (16 MiB, MI355X
-// host), and a first call that maps its staging memory.
+// pinned chunk buffers: the host copies one chunk while the DMA engine moves
+// another, so the two overlap. Device-to-host copies drain the buffers on
+// one thread each (the caller and up to three helpers): the host's copy out
+// of DMA-written memory into pageable pages runs at 15-30 GB/s per thread,
+// below the link's ~55 GB/s. The runtime's own pageable copies stage through
+// buffers too, but at ~15 GB/s device to host (16 MiB, MI355X host) and with
+// a first call that maps its staging memory.
 struct HostStage {
-    static constexpr int kBuffers = 2;
+    static constexpr int kBuffers = 4;
     static constexpr size_t kChunk = (size_t)4 << 20;
     HostBuf buf[kBuffers];
     hipEvent_t ev[kBuffers] = {};

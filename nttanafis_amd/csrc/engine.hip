@@ -12,6 +12,7 @@
 #include <functional>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/nttanafis.h"
@@ -1327,6 +1328,8 @@ bool HostStage::to_host(hipStream_t s, void* host, const void* dev, size_t bytes
     char* dst = static_cast<char*>(host);
     const char* src = static_cast<const char*>(dev);
     const size_t chunks = (bytes + kChunk - 1) / kChunk;
+    // HIP calls are thread-safe; the DMAs of different chunks touch
+    // disjoint memory, so the order in which threads queue them is free.
     auto queue = [&](size_t c) {
         const size_t o = c * kChunk, m = bytes - o < kChunk ? bytes - o : kChunk;
         const int k = (int)(c % kBuffers);
@@ -1335,14 +1338,36 @@ bool HostStage::to_host(hipStream_t s, void* host, const void* dev, size_t bytes
     };
     for (size_t c = 0; c < chunks && c < (size_t)kBuffers; ++c)
         if (!queue(c)) return false;
-    for (size_t c = 0; c < chunks; ++c) {
-        const int k = (int)(c % kBuffers);
-        const size_t o = c * kChunk, m = bytes - o < kChunk ? bytes - o : kChunk;
-        if (hipEventSynchronize(ev[k]) != hipSuccess) return false;
-        memcpy(dst + o, buf[k].ptr, m);
-        if (c + kBuffers < chunks && !queue(c + kBuffers)) return false;
+    // Buffer k belongs to one thread: it waits for the buffer's DMA, copies
+    // the chunk out (the host copy out of DMA-written memory, not the link,
+    // is the slow side) and queues the buffer's next chunk.
+    std::atomic<bool> ok{true};
+    auto drain = [&](int k) {
+        for (size_t c = (size_t)k; c < chunks && ok.load(std::memory_order_relaxed); c += kBuffers) {
+            const size_t o = c * kChunk, m = bytes - o < kChunk ? bytes - o : kChunk;
+            if (hipEventSynchronize(ev[k]) != hipSuccess) {
+                ok = false;
+                return;
+            }
+            memcpy(dst + o, buf[k].ptr, m);
+            if (c + kBuffers < chunks && !queue(c + kBuffers)) {
+                ok = false;
+                return;
+            }
+        }
+    };
+    const int used = (int)std::min<size_t>(chunks, (size_t)kBuffers);
+    std::thread helpers[kBuffers];
+    int started = 1;
+    try {
+        for (; started < used; ++started) helpers[started] = std::thread(drain, started);
+    } catch (...) {
+        // No more threads: the caller drains the rest of the buffers itself.
     }
-    return true;
+    drain(0);
+    for (int k = started; k < used; ++k) drain(k);
+    for (int k = 1; k < started; ++k) helpers[k].join();
+    return ok.load();
 }
 
 bool copy_to_device(HostStage& st, hipStream_t s, void* dev, const void* host, size_t bytes) {
