@@ -138,6 +138,28 @@ struct HostBuf {
     }
 };
 
+// Pinned host memory mapped into the device's address space: kernels read
+// and write it directly over the host link.
+struct MappedBuf {
+    HostBuf host;
+    void* dev = nullptr;
+    bool ensure(size_t n) {
+        if (n <= host.bytes && dev) return true;
+        dev = nullptr;
+        if (!host.ensure(n)) return false;
+        if (hipHostGetDevicePointer(&dev, host.ptr, 0) != hipSuccess) {
+            dev = nullptr;
+            host.release();
+            return false;
+        }
+        return true;
+    }
+    void release() {
+        host.release();
+        dev = nullptr;
+    }
+};
+
 // True when p lies in page-locked host memory (hipHostMalloc), which the
 // DMA engines read and write directly.
 bool host_pinned(const void* p);
@@ -177,6 +199,7 @@ struct Executor {
     bool own_stream = false;
     DevBuf spec_a, spec_b, in_a, in_b, out, carry;
     HostStage stage;
+    MappedBuf mapped;  // operands and result of small host-buffer calls
     std::mutex lock;
 
     bool init() {
@@ -195,6 +218,7 @@ struct Executor {
         in_b.release();
         out.release();
         carry.release();
+        mapped.release();
         if (own_stream && stream) hipStreamDestroy(stream);
     }
 };

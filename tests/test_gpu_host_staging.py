@@ -92,3 +92,41 @@ def test_repeated_calls_reuse_the_ring(oracle):
         sdst = np.zeros(1999, dtype=np.uint64)
         assert small.mul(sdst, c, d), ntt.last_error()
         assert (sdst == oracle.mul(c, d)).all(), seed
+
+
+# Calls moving at most 256 KiB (operands and product) run their kernels on
+# mapped pinned host memory; (8000, 8383) is the largest product at that
+# limit, (8000, 8400) just past it.
+@pytest.mark.parametrize("la,lb", [(1, 1), (64, 64), (1024, 1024), (1024, 192), (5000, 17), (8000, 8383),
+                                   (8000, 8400)])
+def test_small_host_calls_match_oracle(oracle, la, lb):
+    say(f"small host call {la} x {lb}")
+    a, b = oracle.operand(la, 40), oracle.operand(lb, 41)
+    plan = ntt.MulPlan.new(la, lb)
+    dst = np.full(la + lb + 3, 9, dtype=np.uint64)
+    assert plan.mul(dst, a, b), ntt.last_error()
+    assert (dst[:la + lb] == oracle.mul(a, b)).all()
+    assert (dst[la + lb:] == 9).all()
+    sq = ntt.MulPlan.new(la, la)
+    sdst = np.zeros(2 * la, dtype=np.uint64)
+    assert sq.sqr(sdst, a), ntt.last_error()
+    assert (sdst == oracle.mul(a, a)).all()
+    # The same product from pinned arrays and through a workspace.
+    pdst = ntt.pinned_array(la + lb)
+    assert plan.mul(pdst, pinned_copy(a), pinned_copy(b)), ntt.last_error()
+    assert (pdst == dst[:la + lb]).all()
+    ws = ntt.MulWorkspace()
+    wdst = np.zeros(la + lb, dtype=np.uint64)
+    assert ws.mul(wdst, a, b), ntt.last_error()
+    assert (wdst == dst[:la + lb]).all()
+
+
+@pytest.mark.parametrize("n", [24, 25, 1024, 5000])
+def test_small_host_convolutions_match_oracle(oracle, n):
+    say(f"small host convolution {n}")
+    x = oracle.operand(n, 42) & np.uint64(0xFFFF)
+    y = oracle.operand(n, 43) & np.uint64(0xFFFF)
+    plan = ntt.ConvPlan.new(n, n, 65535, 65535, False)
+    out = np.zeros(plan.output_len(), dtype=np.uint64)
+    assert plan.convolve(out, x, y), ntt.last_error()
+    assert (out == oracle.convolve(x, y, 65535, 65535, False)).all()
