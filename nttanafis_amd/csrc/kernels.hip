@@ -608,21 +608,30 @@ __host__ __device__ constexpr int apply_threads() {
     return 256;
 }
 
-template <int BL>
+// The apply workgroup of a single operation whose block functions were
+// scanned takes BPW consecutive blocks (one chunk never splits them: BPW
+// divides kCarryChunk): the 2^24-limb square's 32,768 blocks take 16 us in
+// groups of four against 31 us one by one (groups of eight: 14 us, but an
+// all-ones product's ripples then queue up behind one another). Shorter
+// outputs and batch items take one block per workgroup.
+constexpr int kApplyBlocks = 4;
+static_assert(kCarryChunk % kApplyBlocks == 0, "a group of blocks lies in one chunk");
+
+template <int BL, int BPW>
 __global__ __launch_bounds__(apply_threads<BL>()) void k_carry_apply(CrtArgs a) {
     constexpr int NT = apply_threads<BL>();
     __shared__ CarryFn wave_fn[NT / 64];
-    __shared__ uint32_t cin_s;
+    __shared__ uint32_t cin_s[BPW];
     const CrtJob J = crt_job(a);
-    const int b = (int)J.block;
-    if (b == 0) return;
-    const int lane = threadIdx.x;  // wave 0 only below
+    const uint32_t b0 = BPW == 1 ? J.block : (uint32_t)blockIdx.x * BPW;
+    const uint32_t bn = b0 + BPW < J.blocks ? b0 + BPW : J.blocks;  // blocks b0 .. bn-1
+    const int lane = threadIdx.x;  // wave 0 only in the scanned branch
     if (carry_scanned(J.blocks)) {
         if (threadIdx.x < 64) {
-            // Carry into this block's chunk: the chunk functions before it,
+            // Carry into this group's chunk: the chunk functions before it,
             // composed in order, lane l taking a contiguous slice (the lanes
             // with a slice are a prefix of the wave).
-            const uint32_t j = (uint32_t)b / (uint32_t)kCarryChunk;
+            const uint32_t j = b0 / (uint32_t)kCarryChunk;
             uint32_t cin_chunk = 0;
             if (j > 0) {
                 const uint32_t chunks = carry_chunks(J.blocks);
@@ -644,21 +653,24 @@ __global__ __launch_bounds__(apply_threads<BL>()) void k_carry_apply(CrtArgs a) 
                 const int last = (int)((j + per - 1) / per) - 1;
                 cin_chunk = __shfl(agg.base, last, 64);  // the composition applied to 0
             }
-            // The block's prefix within its chunk, from k_carry_scan.
-            const uint32_t pb = J.agg_base[b], pt = J.agg_thresh[b];
-            if (lane == 0) cin_s = pt == 0 ? cin_chunk : pb + (cin_chunk >= pt ? 1u : 0u);
+            // Each block's prefix within its chunk, from k_carry_scan (a zero
+            // threshold marks the first block of the chunk: the identity).
+            if (lane < BPW && b0 + lane < bn) {
+                const uint32_t pb = J.agg_base[b0 + lane], pt = J.agg_thresh[b0 + lane];
+                cin_s[lane] = pt == 0 ? cin_chunk : pb + (cin_chunk >= pt ? 1u : 0u);
+            }
         }
     } else {
-        // b <= kCarryDirectBlocks: thread t composes the functions of blocks
-        // [t*per, t*per + per) below b (the threads with a slice are a
+        // b0 <= kCarryDirectBlocks: thread t composes the functions of blocks
+        // [t*per, t*per + per) below b0 (the threads with a slice are a
         // prefix of the workgroup), each wave scans its threads' functions
         // in order, and thread 0 applies the waves' results in order to the
-        // zero carry into block 0.
-        // Every load of the slice is issued before the first composition.
+        // zero carry into block 0, then the group's own functions in turn.
+        // Every load of a slice is issued before the first composition.
         constexpr int PER_MAX = ((int)kCarryDirectBlocks + NT - 1) / NT;
-        const uint32_t per = ((uint32_t)b + NT - 1) / NT;
+        const uint32_t per = (b0 + NT - 1) / NT;  // 0 when b0 == 0: no thread has a slice
         const uint32_t s0 = threadIdx.x * per;
-        const uint32_t s1 = s0 + per < (uint32_t)b ? s0 + per : (uint32_t)b;
+        const uint32_t s1 = s0 + per < b0 ? s0 + per : b0;
         CarryFn fs[PER_MAX];
 #pragma unroll
         for (int i = 0; i < PER_MAX; ++i)
@@ -674,9 +686,9 @@ __global__ __launch_bounds__(apply_threads<BL>()) void k_carry_apply(CrtArgs a) 
             const CarryFn prev = shfl_up_fn(agg, d);
             if (wl >= d) agg = compose(prev, agg);
         }
-        // Threads with a slice: nv = ceil(b / per); wave w holds nv - 64w of
+        // Threads with a slice: nv = ceil(b0 / per); wave w holds nv - 64w of
         // them (clamped to 0..64), its result at its last such lane.
-        const int nv = (int)(((uint32_t)b + per - 1) / per);
+        const int nv = per ? (int)((b0 + per - 1) / per) : 0;
         const int in_wave = nv - 64 * wave < 0 ? 0 : (nv - 64 * wave > 64 ? 64 : nv - 64 * wave);
         if (in_wave > 0 && wl == in_wave - 1) wave_fn[wave] = agg;
         __syncthreads();
@@ -684,19 +696,38 @@ __global__ __launch_bounds__(apply_threads<BL>()) void k_carry_apply(CrtArgs a) 
             const int waves = (nv + 63) / 64;
             uint32_t cc = 0;
             for (int w = 0; w < waves; ++w) cc = wave_fn[w].base + (cc >= wave_fn[w].thresh ? 1u : 0u);
-            cin_s = cc;
+            for (uint32_t k = b0; k < bn; ++k) {
+                cin_s[k - b0] = cc;
+                cc = J.agg_base[k] + (cc >= J.agg_thresh[k] ? 1u : 0u);
+            }
         }
     }
     __syncthreads();
-    const uint32_t c = cin_s;
-    if (c == 0) return;
-    const uint64_t L0 = (uint64_t)b * BL;
-    if (threadIdx.x == 0 && L0 < J.out_len) J.out[L0] += c;
-    if (c >= J.first_thresh[b]) {
-        const uint32_t rend = J.ripple_end[b];
-        for (uint32_t rel = 1 + threadIdx.x; rel <= rend; rel += NT) {
-            const uint64_t j = L0 + rel;
-            if (j < J.out_len) J.out[j] += 1;
+    // Each block's carry: its first limb takes it, and the limbs up to the
+    // block's ripple end one more when it reaches the first limb's threshold.
+    // The blocks' limbs are disjoint.
+    // Ripple limbs are read four per thread before any is written back.
+    constexpr int U = 4;
+    for (uint32_t k = b0; k < bn; ++k) {
+        const uint32_t c = k == 0 ? 0u : cin_s[k - b0];
+        if (c == 0) continue;
+        const uint64_t L0 = (uint64_t)k * BL;
+        if (threadIdx.x == 0 && L0 < J.out_len) J.out[L0] += c;
+        if (c >= J.first_thresh[k]) {
+            const uint32_t rend = J.ripple_end[k];
+            for (uint32_t r0 = 1; r0 <= rend; r0 += U * NT) {
+                uint64_t v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t rel = r0 + threadIdx.x + u * NT;
+                    if (rel <= rend && L0 + rel < J.out_len) v[u] = J.out[L0 + rel];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t rel = r0 + threadIdx.x + u * NT;
+                    if (rel <= rend && L0 + rel < J.out_len) J.out[L0 + rel] = v[u] + 1;
+                }
+            }
         }
     }
 }
@@ -731,8 +762,13 @@ static bool launch_carries(const CrtArgs& a, hipStream_t s) {
                    dim3(kCarryChunk), s, a);
         if (!NTT_CHECK_LAUNCH()) return false;
     }
-    if (a.blocks > 1 || a.batch)
-        NTT_LAUNCH("carry_apply", k_carry_apply<BL>, dim3(a.blocks), dim3(apply_threads<BL>()), s, a);
+    if (a.batch)
+        NTT_LAUNCH("carry_apply", (k_carry_apply<BL, 1>), dim3(a.blocks), dim3(apply_threads<BL>()), s, a);
+    else if (carry_scanned(a.blocks))
+        NTT_LAUNCH("carry_apply", (k_carry_apply<BL, kApplyBlocks>), dim3((a.blocks + kApplyBlocks - 1) / kApplyBlocks),
+                   dim3(apply_threads<BL>()), s, a);
+    else if (a.blocks > 1)
+        NTT_LAUNCH("carry_apply", (k_carry_apply<BL, 1>), dim3(a.blocks), dim3(apply_threads<BL>()), s, a);
     return NTT_CHECK_LAUNCH();
 }
 
