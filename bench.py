@@ -40,6 +40,7 @@ from nttanafis_amd.workloads import (CONV, SQR, batch_operands, batch_shapes, op
 
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_G = 1024 * 2.4e9 / 4 / 1e9  # wave64 FP64 instructions per second, G
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak (an FMA counted as two flops)
 
 
 def splitmix(length: int, seed: int) -> np.ndarray:
@@ -190,13 +191,19 @@ def committed_profile(workload: str) -> dict:
     kernel kind `KERNEL <kind> <launches per step> <average us> <algorithmic
     bytes per launch>` (kernel trace), `TRAFFIC <kind> <bytes>` (FETCH_SIZE
     and WRITE_SIZE per dispatch, each scaled by what it reads for the plain
-    copy kernel of the same run) and `VALU <kind> <wave instructions>`."""
+    copy kernel of the same run), `VALU <kind> <wave instructions>` and
+    `MIX <kind> valu <n> fma64 <share> mul64 <share> ...` (the instruction
+    mix of the kind's VALU instructions)."""
     path = os.path.join(ROOT, "profiles", f"{workload}_kernels.txt")
-    out = {"valu": {}, "traffic": {}, "kernels": {}, "split": None}
+    out = {"valu": {}, "traffic": {}, "kernels": {}, "split": None, "mix": {}}
     if os.path.exists(path):
         for line in open(path):
             parts = line.split()
-            if len(parts) == 3 and parts[0] == "VALU":
+            if len(parts) > 3 and parts[0] == "MIX":
+                # MIX <kind> valu <n> <name> <share> ...: shares of the kind's VALU instructions
+                vals = parts[2:]
+                out["mix"][parts[1]] = {vals[i]: float(vals[i + 1]) for i in range(0, len(vals) - 1, 2)}
+            elif len(parts) == 3 and parts[0] == "VALU":
                 out["valu"][parts[1]] = int(float(parts[2]))
             elif len(parts) == 3 and parts[0] == "TRAFFIC":
                 out["traffic"][parts[1]] = int(float(parts[2]))
@@ -1026,7 +1033,7 @@ def main() -> int:
         us = row["event_us"] * row["launches_per_step"]
         if row.get("bytes_per_launch") and us > dom_us:
             dominant, dom_us = k, us
-    roofline, valu = None, None
+    roofline, valu, fp64 = None, None, None
     split_matches = committed["split"] == [head["log_len"], head["transform_split"], head["transform_forms"] or []]
     if dominant:
         row = kernels[dominant]
@@ -1050,6 +1057,19 @@ def main() -> int:
             rate = insts / (row["event_us"] * 1e-6) / 1e9
             valu = {"kernel": dominant, "insts_per_launch": insts, "achieved": round(rate, 1), "peak": VALU_PEAK_G,
                     "unit": "G wave-instr/s", "frac": round(rate / VALU_PEAK_G, 4)}
+            mix = committed["mix"].get(dominant)
+            if mix and "fma64" in mix:
+                # FP64 work of the dominant kernel: its instruction mix (the
+                # committed profile's counters) over its live launch time; an
+                # FMA is two flops, a multiply or an add one, 64 lanes each.
+                t = row["event_us"] * 1e-6
+                fma = insts * mix["fma64"]
+                flops = 64 * insts * (2 * mix["fma64"] + mix.get("mul64", 0.0) + mix.get("add64", 0.0))
+                fp64 = {"kernel": dominant, "fma_wave_insts_per_launch": round(fma),
+                        "fma_tflops": round(128 * fma / t / 1e12, 2),
+                        "tflops": round(flops / t / 1e12, 2), "peak_tflops": FP64_PEAK_TFLOPS,
+                        "frac": round(flops / t / 1e12 / FP64_PEAK_TFLOPS, 4),
+                        "source": "instruction mix of the committed profile over the live launch time"}
 
     kind = head["kind"]
     unit = head["rate_unit"]
@@ -1085,6 +1105,7 @@ def main() -> int:
         "check": head["check"],
         "roofline": roofline,
         "valu": valu,
+        "fp64": fp64,
     }
     if kind == "mul" and not args.no_host:
         wl = workload(args.workload)
