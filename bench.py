@@ -956,7 +956,7 @@ def main() -> int:
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations' legs")
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=240)
+    ap.add_argument("--serve-requests", type=int, default=1200)
     ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--replay-reps", type=int, default=20,
                     help="replays of each kernel kind's last launch after the timed steps (0 in profiled runs)")
