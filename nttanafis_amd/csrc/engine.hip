@@ -1213,9 +1213,10 @@ static void tune_split(Executor& ex, int K, int use, const std::function<bool()>
         if (e0) hipEventDestroy(e0);
         return;
     }
-    // Time per run of a candidate: one warm run (timed), then enough runs
-    // for about 2 ms of work (3 to 20), so that short products are not
-    // decided by launch jitter.
+    // Time per run of a candidate: one warm run (timed), then two rounds of
+    // enough runs for about 2 ms of work each (3 to 20), the faster round
+    // kept, so that short products are not decided by launch jitter nor
+    // long ones by a single slow round.
     auto time_of = [&](const Split& sp) {
         t_force = sp;
         hipEventRecord(e0, ex.stream);
@@ -1224,17 +1225,21 @@ static void tune_split(Executor& ex, int K, int use, const std::function<bool()>
         float warm = 1e30f;
         if (ok && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&warm, e0, e1);
         const int reps = warm <= 0.1f ? 20 : std::max(3, std::min(20, (int)(2.0f / warm)));
-        hipEventRecord(e0, ex.stream);
-        for (int r = 0; r < reps && ok; ++r) ok = run();
-        hipEventRecord(e1, ex.stream);
-        float ms = 1e30f;
-        if (ok && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
-        return ok ? ms / reps : 1e30f;
+        float best_round = 1e30f;
+        for (int round = 0; round < 2 && ok; ++round) {
+            hipEventRecord(e0, ex.stream);
+            for (int r = 0; r < reps && ok; ++r) ok = run();
+            hipEventRecord(e1, ex.stream);
+            float ms = 1e30f;
+            if (ok && hipEventSynchronize(e1) == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
+            best_round = std::min(best_round, ms / reps);
+        }
+        return ok ? best_round : 1e30f;
     };
     // A later candidate replaces the best only when it is faster by the
-    // margin (3 %), so near-equal candidates resolve to the first one (the
+    // margin (2 %), so near-equal candidates resolve to the first one (the
     // largest row, no flags) instead of to timing noise.
-    constexpr float kMargin = 0.97f;
+    constexpr float kMargin = 0.98f;
     Split best = cand[0];
     float best_ms = 1e30f;
     if (cand.size() > 1)
