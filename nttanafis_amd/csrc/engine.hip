@@ -3083,6 +3083,39 @@ int ntt_selftest_host(uint64_t seed) {
             if (out != g.base + (cin >= g.thresh ? 1u : 0u)) ++fails;
         }
     }
+    // Runs of limbs (sum v mod 2^64 and wrap count w each, many of them
+    // all-ones): the functions c -> w + [c >= limb_thresh(v)] composed in
+    // order, and composed as contiguous slices whose results are composed
+    // in order (the carry kernels' two forms), equal adding the carry limb
+    // by limb.
+    for (int t = 0; t < 20000; ++t) {
+        uint64_t v[12];
+        uint32_t w[12];
+        const int len = 1 + (int)(rng.next() % 12);
+        for (int i = 0; i < len; ++i) {
+            const uint64_t r = rng.next() % 5;
+            v[i] = r == 0 ? rng.next() : r == 1 ? 0 : ~0ull - (rng.next() % 3);
+            w[i] = (uint32_t)(rng.next() % 3);
+        }
+        CarryFn whole{w[0], limb_thresh(v[0])};
+        for (int i = 1; i < len; ++i) whole = compose(whole, CarryFn{w[i], limb_thresh(v[i])});
+        const int per = 1 + (int)(rng.next() % 4);
+        CarryFn sliced{0u, kInf};
+        for (int s0 = 0; s0 < len; s0 += per) {
+            CarryFn g{w[s0], limb_thresh(v[s0])};
+            for (int i = s0 + 1; i < len && i < s0 + per; ++i) g = compose(g, CarryFn{w[i], limb_thresh(v[i])});
+            sliced = s0 == 0 ? g : compose(sliced, g);
+        }
+        for (uint32_t cin = 0; cin < 4; ++cin) {
+            uint32_t carry = cin;
+            for (int i = 0; i < len; ++i) {
+                const uint64_t sum = v[i] + carry;
+                carry = w[i] + (sum < v[i] ? 1u : 0u);
+            }
+            if (whole.base + (cin >= whole.thresh ? 1u : 0u) != carry) ++fails;
+            if (sliced.base + (cin >= sliced.thresh ? 1u : 0u) != carry) ++fails;
+        }
+    }
     return fails;
 }
 
