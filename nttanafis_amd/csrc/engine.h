@@ -177,6 +177,7 @@ struct HostStage {
     static constexpr size_t kChunk = (size_t)4 << 20;
     HostBuf buf[kBuffers];
     hipEvent_t ev[kBuffers] = {};
+    int next = 0;  // buffer of the next host-to-device chunk
     bool init();
     // Host to device, queued on s; returns once every chunk has been copied
     // out of `host` (the last DMAs may still run: later calls wait for them).

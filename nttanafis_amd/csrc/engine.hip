@@ -1315,8 +1315,10 @@ void HostStage::release() {
 bool HostStage::to_device(hipStream_t s, void* dev, const void* host, size_t bytes) {
     const char* src = static_cast<const char*>(host);
     char* dst = static_cast<char*>(dev);
-    int k = 0;
-    for (size_t o = 0; o < bytes; o += kChunk, k = (k + 1) % kBuffers) {
+    // The ring continues where the previous copy left it, so a second
+    // operand's first chunks go to buffers whose DMAs are not the latest.
+    for (size_t o = 0; o < bytes; o += kChunk, next = (next + 1) % kBuffers) {
+        const int k = next;
         const size_t m = bytes - o < kChunk ? bytes - o : kChunk;
         // The buffer's previous DMA (this call's or an earlier one's) must
         // have finished before the buffer is refilled.
