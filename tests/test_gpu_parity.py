@@ -207,6 +207,43 @@ def test_conv32s_2p22_golden(digests):
     check_record(dst, digests["conv32s_2p22"])
 
 
+def conv_counts(n):
+    """Terms of coefficient k of a convolution of two length-n vectors:
+    min(k + 1, 2n - 1 - k)."""
+    k = np.arange(2 * n - 1, dtype=np.uint64)
+    return np.minimum(k + np.uint64(1), np.uint64(2 * n - 1) - k)
+
+
+def test_conv16_2p22_all_max():
+    """Config-3 shape with every coefficient at its bound (65,535): output
+    coefficient k is 65535^2 times its term count, the largest values the
+    plan's bound admits (closed form)."""
+    n = 1 << 22
+    say("conv16 2^22 all 65535")
+    a = np.full(n, 65535, dtype=np.uint64)
+    plan = ntt.ConvPlan.new(n, n, 65535, 65535, False)
+    assert plan.stride() == 1
+    dst = np.zeros(plan.output_len(), dtype=np.uint64)
+    assert plan.convolve(dst, a, a.copy()), ntt.last_error()
+    assert (dst == conv_counts(n) * np.uint64(65535 * 65535)).all()
+
+
+def test_conv32s_2p22_all_min():
+    """Signed 32-bit config-3 shape with every coefficient -2^31: output
+    coefficient k is 2^62 times its term count (up to 2^84, two limbs in
+    two's complement), the largest magnitude the plan's bound admits."""
+    n = 1 << 22
+    say("conv32s 2^22 all -2^31")
+    a = np.full(n, np.uint64((1 << 64) - (1 << 31)), dtype=np.uint64)
+    plan = ntt.ConvPlan.new(n, n, 1 << 31, 1 << 31, True)
+    assert plan.stride() == 2
+    dst = np.zeros(plan.output_len(), dtype=np.uint64)
+    assert plan.convolve(dst, a, a.copy()), ntt.last_error()
+    m = conv_counts(n)
+    assert (dst[0::2] == (m & np.uint64(3)) << np.uint64(62)).all()
+    assert (dst[1::2] == m >> np.uint64(2)).all()
+
+
 @pytest.mark.parametrize("bits,signed", [(1, False), (16, False), (32, True), (49, False), (50, True),
                                          (51, False), (52, False), (52, True), (53, False), (63, True),
                                          (64, False)])
