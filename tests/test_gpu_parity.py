@@ -8,7 +8,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import (MASK, all_ones_product, check_record, conv16_operands, conv32s_operands,
+from conftest import (MASK, all_ones_product, check_record, conv16_operands, conv32s_operands, from_int,
                       signed_conv_int, to_int, unsigned_conv_int)
 
 import nttanafis_amd as ntt
@@ -153,6 +153,30 @@ def test_all_ones_2p20():
     say("all ones 2^20")
     a = np.full(1 << 20, MASK, dtype=np.uint64)
     assert (engine_mul(a, a.copy()) == all_ones_product(1 << 20, 1 << 20)).all()
+
+
+def test_sparse_operands_2p20(oracle):
+    """2^20-limb operands that are mostly zero (the served stream's sparse
+    operands, and every zero group the packing skips): a few nonzero limbs
+    against a random operand, and two operands whose only nonzero limb is
+    the top one; checked against Python integers."""
+    n = 1 << 20
+    say("sparse 2^20")
+    rng = np.random.default_rng(20)
+    b = oracle.operand(n, 2)
+    a = np.zeros(n, dtype=np.uint64)
+    spots = [0, 1, 12345, 524287, 524288, n - 2, n - 1]
+    a[spots] = rng.integers(1, 2**64, len(spots), dtype=np.uint64, endpoint=False)
+    bi = to_int(b)
+    want = sum(int(a[i]) * bi << (64 * i) for i in spots)
+    assert to_int(engine_mul(a, b)) == want
+    c = np.zeros(n, dtype=np.uint64)
+    d = np.zeros(n, dtype=np.uint64)
+    c[-1] = np.uint64(0xFEDCBA9876543210)
+    d[-1] = np.uint64(0x0123456789ABCDEF)
+    got = engine_mul(c, d)
+    assert (got == from_int(int(c[-1]) * int(d[-1]) << (64 * (2 * n - 2)), 2 * n)).all()
+    assert (engine_sqr(c) == from_int(int(c[-1]) ** 2 << (64 * (2 * n - 2)), 2 * n)).all()
 
 
 def test_largest_tested_product():
