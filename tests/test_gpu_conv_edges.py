@@ -8,7 +8,9 @@ convolution tests hold them (src/tests/convolution.rs):
   (signed_extremes_reach_the_bound_in_columns_transforms_and_blocks,
   :379-408);
 - unbalanced saturated and mixed operands in either order
-  (unbalanced_convolutions_match_the_oracle_across_block_windows, :217-239).
+  (unbalanced_convolutions_match_the_oracle_across_block_windows, :217-239);
+- one array used twice, refused calls and signed bounds (:160-180,
+  :311-342, :413-424).
 
 Each convolution runs through ConvPlan and through a prepared ConvWorkspace
 and is compared with exact Python integers at the plan's stride.
@@ -112,3 +114,61 @@ def test_unbalanced_saturated_and_mixed(long, short):
         check(sat_short, sat_long, max_short, max_long, False)
         mixed = [(i * 0x9E37) & max_long for i in range(long)]
         check(mixed, sat_short, max_long, max_short, False)
+
+
+def test_one_operand_used_twice_matches_two_copies():
+    """The same array twice (the square path, src/convolve.rs:156) equals two
+    distinct copies (one_operand_slice_used_twice_matches_two_distinct_copies,
+    src/tests/convolution.rs:160-180), for the column path, transforms and
+    every coefficient width."""
+    rng = np.random.default_rng(160)
+    for n in (1, 7, 24, 25, 70, 3000):
+        for bits in (0, 1, 16, 33, 63, 64):
+            mx = bound_of(bits)
+            a = rng.integers(0, 2**64, n, dtype=np.uint64, endpoint=False) & np.uint64(mx)
+            plan = ntt.ConvPlan.new(n, n, mx, mx, False)
+            distinct = np.zeros(plan.output_len(), dtype=np.uint64)
+            aliased = np.full(plan.output_len(), MASK, dtype=np.uint64)
+            assert plan.convolve(distinct, a, a.copy()), ntt.last_error()
+            assert plan.convolve(aliased, a, a), ntt.last_error()
+            assert (aliased == distinct).all(), (n, bits)
+            ws = ntt.ConvWorkspace()
+            ws.prepare(plan)
+            aliased[:] = MASK
+            assert ws.convolve(aliased, a, a), ntt.last_error()
+            assert (aliased == distinct).all(), (n, bits, "workspace")
+
+
+def test_rejected_calls_preserve_the_destination():
+    """Wrong operand lengths, a coefficient above its bound, a short
+    destination and an unprepared workspace are refused and leave the
+    destination untouched (rejected_calls_preserve_the_destination,
+    src/tests/convolution.rs:311-342), on the column path and the transforms."""
+    for left, right in [(3, 2), (30, 30)]:
+        plan = ntt.ConvPlan.new(left, right, 7, 7, False)
+        a = np.full(left, 7, dtype=np.uint64)
+        b = np.full(right, 7, dtype=np.uint64)
+        above = a.copy()
+        above[-1] = 8
+        untouched = np.full(plan.output_len() + 1, MASK, dtype=np.uint64)
+        ws = ntt.ConvWorkspace()
+        assert not ws.convolve(untouched, a, b)
+        ws.prepare(plan)
+        for first, second in [(a[1:], b), (a, b[1:]), (above, b)]:
+            assert not plan.convolve(untouched, first, second)
+            assert not ws.convolve(untouched, first, second)
+        short = untouched[:plan.output_len() - 1]
+        assert not plan.convolve(short, a, b)
+        assert not ws.convolve(short, a, b)
+        assert (untouched == np.uint64(MASK)).all()
+
+
+def test_signed_plans_reject_wide_bounds_and_values_beyond_them():
+    """A signed bound needs a sign bit in one limb, and a value beyond the
+    bound is refused without writing (src/tests/convolution.rs:413-424)."""
+    assert ntt.ConvPlan.new(4, 4, (1 << 63), 1, True) is None
+    plan = ntt.ConvPlan.new(4, 4, 3, 3, True)
+    dst = np.full(plan.output_len(), 7, dtype=np.uint64)
+    beyond = np.array([1, MASK - 3, 0, 2], dtype=np.uint64)
+    assert not plan.convolve(dst, beyond, np.array([1, 2, 3, 0], dtype=np.uint64))
+    assert (dst == 7).all()
