@@ -244,8 +244,10 @@ static double transform_price(int np, int log_n) {
 }
 constexpr double kLaunchPrice = 5e6;
 // Price of one 64 x 64-bit limb product of a schoolbook block on one CU,
-// in the units of transform_price (the whole device).
-constexpr double kSchoolProductPrice = 310.0;
+// in the units of transform_price (the whole device), set at the measured
+// crossover of single squares: 384^2 limbs take 13.0-13.3 us either way
+// (schoolbook 192^2: 9.2 us, 512^2: 16.8 us; transforms 11.5 and 13.8 us).
+constexpr double kSchoolProductPrice = 414.0;
 constexpr int kDeviceCUs = 256;
 
 int row_parts(const Geometry& g, RowPart* parts) {
