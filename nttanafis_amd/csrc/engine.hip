@@ -1381,22 +1381,32 @@ bool HostStage::to_host(hipStream_t s, void* host, const void* dev, size_t bytes
 
 bool copy_to_device(HostStage& st, hipStream_t s, void* dev, const void* host, size_t bytes) {
     if (bytes == 0) return true;
-    if (bytes <= HostStage::kChunk / 8 || host_pinned(host) || !st.init())
+    if (host_pinned(host) || !st.init())
         return hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s) == hipSuccess;
     return st.to_device(s, dev, host, bytes);
 }
 
 bool copy_to_host(HostStage& st, hipStream_t s, void* host, const void* dev, size_t bytes) {
     if (bytes == 0) return hipStreamSynchronize(s) == hipSuccess;
-    if (bytes <= HostStage::kChunk / 8 || host_pinned(host) || !st.init())
+    if (host_pinned(host) || !st.init())
         return hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s) == hipSuccess &&
                hipStreamSynchronize(s) == hipSuccess;
     return st.to_host(s, host, dev, bytes);
 }
 
 // Host-buffer calls moving at most this many bytes (operands and result)
-// run on mapped host memory.
-constexpr uint64_t kMappedCallBytes = (uint64_t)256 << 10;
+// run on mapped host memory; NTTAMD_MAPPED_BYTES sets another limit (tests
+// and sweeps). Measured per call from pageable arrays against the staged
+// copies: n x n limbs, n = 1,024: 43 / 76 us, 16,384: 72 / 132 us, 131,072
+// (4 MiB moved): 251 / 277 us, 262,144: 511 / 442 us.
+constexpr uint64_t kMappedCallBytes = (uint64_t)4 << 20;
+static uint64_t mapped_call_bytes() {
+    static const uint64_t limit = [] {
+        const char* e = getenv("NTTAMD_MAPPED_BYTES");
+        return e ? (uint64_t)strtoull(e, nullptr, 10) : kMappedCallBytes;
+    }();
+    return limit;
+}
 
 static bool stage_and_run(Executor& ex, uint64_t out_len, const uint64_t* a, uint64_t la,
                           const uint64_t* b, uint64_t lb, bool same,
@@ -1408,7 +1418,7 @@ static bool stage_and_run(Executor& ex, uint64_t out_len, const uint64_t* a, uin
         return false;
     }
     const uint64_t in_limbs = la + (same ? 0 : lb);
-    if ((in_limbs + out_len) * 8 <= kMappedCallBytes && ex.mapped.ensure((in_limbs + out_len) * 8)) {
+    if ((in_limbs + out_len) * 8 <= mapped_call_bytes() && ex.mapped.ensure((in_limbs + out_len) * 8)) {
         // A small call: the kernels read the operands from, and write the
         // result to, mapped pinned host memory, so no DMA round trip (each
         // ~10 us, against ~15 us of kernels for a 1,024-limb product) sits

@@ -1,8 +1,9 @@
 """Host-buffer calls through the pinned staging ring.
 
-Pageable host operands and results larger than an eighth of a staging chunk
-(4 MiB) move through two pinned chunk buffers; pinned arrays and short
-buffers are copied directly. Both routes must give the oracle's product,
+Pageable host operands and results of calls moving more than 4 MiB go
+through four pinned 4 MiB chunk buffers, smaller calls run their kernels on
+mapped pinned memory, and pinned arrays are copied directly. Every route
+must give the oracle's product,
 including sizes that end inside a chunk, a destination longer than the
 product (its suffix untouched), a convolution and a square.
 """
@@ -35,7 +36,7 @@ def pinned_copy(x):
 
 
 # 524,288 limbs fill one 4 MiB chunk exactly; the others end inside one.
-@pytest.mark.parametrize("la,lb", [(524288, 524288), (600001, 333333), (1048577, 65536), (70000, 70001)])
+@pytest.mark.parametrize("la,lb", [(524288, 524288), (600001, 333333), (1048577, 65536), (300000, 250001)])
 def test_pageable_and_pinned_host_buffers_match_oracle(oracle, la, lb):
     say(f"host buffers {la} x {lb}")
     a, b = oracle.operand(la, 11), oracle.operand(lb, 12)
@@ -94,23 +95,23 @@ def test_repeated_calls_reuse_the_ring(oracle):
         assert (sdst == oracle.mul(c, d)).all(), seed
 
 
-# Calls moving at most 256 KiB (operands and product) run their kernels on
-# mapped pinned host memory; (8000, 8383) is the largest product at that
-# limit, (8000, 8400) just past it.
-@pytest.mark.parametrize("la,lb", [(1, 1), (64, 64), (1024, 1024), (1024, 192), (5000, 17), (8000, 8383),
-                                   (8000, 8400)])
+# Calls moving at most 4 MiB (operands and product) run their kernels on
+# mapped pinned host memory; (131072, 131072) is the largest square shape at
+# that limit, (131072, 131073) just past it.
+@pytest.mark.parametrize("la,lb", [(1, 1), (64, 64), (1024, 1024), (1024, 192), (5000, 17), (8000, 8400),
+                                   (131072, 131072), (131072, 131073)])
 def test_small_host_calls_match_oracle(oracle, la, lb):
     say(f"small host call {la} x {lb}")
     a, b = oracle.operand(la, 40), oracle.operand(lb, 41)
     plan = ntt.MulPlan.new(la, lb)
     dst = np.full(la + lb + 3, 9, dtype=np.uint64)
     assert plan.mul(dst, a, b), ntt.last_error()
-    assert (dst[:la + lb] == oracle.mul(a, b)).all()
+    assert (dst[:la + lb] == oracle.mul(a, b, threads=8)).all()
     assert (dst[la + lb:] == 9).all()
     sq = ntt.MulPlan.new(la, la)
     sdst = np.zeros(2 * la, dtype=np.uint64)
     assert sq.sqr(sdst, a), ntt.last_error()
-    assert (sdst == oracle.mul(a, a)).all()
+    assert (sdst == oracle.mul(a, a, threads=8)).all()
     # The same product from pinned arrays and through a workspace.
     pdst = ntt.pinned_array(la + lb)
     assert plan.mul(pdst, pinned_copy(a), pinned_copy(b)), ntt.last_error()
