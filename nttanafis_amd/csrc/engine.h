@@ -112,15 +112,17 @@ struct DevBuf {
     }
 };
 
-// Pinned (page-locked) host memory, grow-only.
+// Pinned (page-locked) host memory, grow-only. By default it is
+// non-coherent (HIP_HOST_COHERENT unset): fine for memory that only the DMA
+// engines and the host touch, since the GPU may cache it.
 struct HostBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
-    bool ensure(size_t n) {
+    bool ensure(size_t n, unsigned flags = hipHostMallocDefault) {
         if (n <= bytes) return true;
         release();
         size_t want = std::max<size_t>(n, 4096);
-        if (hipHostMalloc(&ptr, want, hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc(&ptr, want, flags) != hipSuccess) {
             ptr = nullptr;
             return false;
         }
@@ -139,14 +141,18 @@ struct HostBuf {
 };
 
 // Pinned host memory mapped into the device's address space: kernels read
-// and write it directly over the host link.
+// and write it directly over the host link. It is allocated coherent
+// (fine-grained): the host rewrites it between calls, so no GPU cache may
+// keep a line of it from an earlier kernel (with the default, non-coherent
+// memory a later kernel could read stale operand lines and a served request
+// came out wrong).
 struct MappedBuf {
     HostBuf host;
     void* dev = nullptr;
     bool ensure(size_t n) {
         if (n <= host.bytes && dev) return true;
         dev = nullptr;
-        if (!host.ensure(n)) return false;
+        if (!host.ensure(n, hipHostMallocCoherent)) return false;
         if (hipHostGetDevicePointer(&dev, host.ptr, 0) != hipSuccess) {
             dev = nullptr;
             host.release();

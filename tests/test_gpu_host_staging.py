@@ -131,3 +131,36 @@ def test_small_host_convolutions_match_oracle(oracle, n):
     out = np.zeros(plan.output_len(), dtype=np.uint64)
     assert plan.convolve(out, x, y), ntt.last_error()
     assert (out == oracle.convolve(x, y, 65535, 65535, False)).all()
+
+
+def test_mapped_calls_in_sequence(oracle):
+    """Hundreds of mapped calls of changing shapes back to back through one
+    thread's executor (the served stream's per-call pattern): each call
+    rewrites the mapped buffer that the previous call's kernels read and
+    wrote, and must see only its own operands. Products, squares and 16-bit
+    convolutions from pageable and pinned arrays, against the oracle."""
+    say("mapped calls in sequence")
+    rng = np.random.default_rng(1131)
+    for i in range(240):
+        la = int(np.exp(rng.uniform(0, np.log(60000))))
+        lb = int(np.exp(rng.uniform(0, np.log(60000))))
+        kind = i % 3
+        a = oracle.operand(la, 3000 + i)
+        b = oracle.operand(lb, 4000 + i)
+        if i % 4 == 1:
+            a = pinned_copy(a)
+            b = pinned_copy(b)
+        if kind == 0:
+            dst = np.zeros(la + lb, dtype=np.uint64)
+            assert ntt.MulPlan.new(la, lb).mul(dst, a, b), ntt.last_error()
+            assert (dst == oracle.mul(a, b, threads=8)).all(), (i, la, lb)
+        elif kind == 1:
+            dst = np.zeros(2 * la, dtype=np.uint64)
+            assert ntt.MulPlan.new(la, la).sqr(dst, a), ntt.last_error()
+            assert (dst == oracle.mul(a, a, threads=8)).all(), (i, la)
+        else:
+            x, y = a & np.uint64(0xFFFF), b & np.uint64(0xFFFF)
+            plan = ntt.ConvPlan.new(la, lb, 65535, 65535, False)
+            out = np.zeros(plan.output_len(), dtype=np.uint64)
+            assert plan.convolve(out, x, y), ntt.last_error()
+            assert (out == oracle.convolve(x, y, 65535, 65535, False, threads=8)).all(), (i, la, lb)
