@@ -256,3 +256,38 @@ def test_large_request_not_starved_by_small_traffic(pools):
         a, b = request_operands(reqs[i], pools)
         assert (outs[i] == alone(reqs[i], a, b)).all(), f"request {i}"
     server.close()
+
+
+def test_served_stream_window_with_long_by_short_split_product(oracle):
+    """Requests 1100-1159 of the 1,200-request served stream (sizes up to
+    4,194,304 limbs, operands and results in pinned memory, as the served
+    measurement runs them), served twice into adjacent slices of one output
+    pool: request 1131, a 3,245,183 x 387-limb split product, against the
+    oracle; every other request against the plain path."""
+    base = served_stream(1200, 1.0, hi=4194304)
+    big_pools = operand_pools(4194304, alloc=ntt.pinned_array)
+    window = list(range(1100, 1160))
+    ops = {i: request_operands(base[i], big_pools) for i in window}
+    a, b = ops[1131]
+    assert (len(a), len(b)) == (3245183, 387)
+    want = {i: alone(base[i], *ops[i]) for i in window}
+    assert (want[1131] == oracle.mul(np.ascontiguousarray(a), np.ascontiguousarray(b), threads=16)).all()
+    lens = [output_len(base[i]) for i in window]
+    pool = ntt.pinned_array(int(sum(lens)))
+    offs = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64)
+    dsts = {i: pool[int(o):int(o) + n] for i, o, n in zip(window, offs, lens)}
+    server = ntt.Server(budget_bytes=0, slots=4)
+    for rep in range(2):
+        say(f"stream window, pass {rep}")
+        pool[:] = 0
+        tickets = []
+        for i in window:
+            t = submit(server, base[i], *ops[i], dsts[i])
+            assert t >= 0, ntt.last_error()
+            tickets.append(t)
+        assert server.drain()
+        for t in tickets:
+            server.release(t)
+        bad = [i for i in window if not (dsts[i] == want[i]).all()]
+        assert not bad, (rep, bad)
+    server.close()
